@@ -1,0 +1,118 @@
+/*
+ * qgtl_hip_abi.h -- the thin C ABI between libqgtl_mi355x's C host code and
+ * its HIP kernels (the .hip files of quantum_geometric_tensor_amd/csrc/hip).
+ *
+ * Internal to the library: no QGTL counterpart.  Plain C types only; device
+ * pointers are void*; streams and events are opaque handles.  Every function
+ * returns 0 or a negative qgt_error_t code (QGT_ERROR_GPU_*).
+ */
+#ifndef QGTL_HIP_ABI_H
+#define QGTL_HIP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QGTL_TILE_BITS_MAX 13    /* LDS tile of a sweep pass: up to 2^13 complex64 amplitudes */
+#define QGTL_MAX_QUBITS 30
+#define QGTL_PERM_MAX 32
+#define QGTL_INJECT_MAX 64
+
+typedef struct { float re, im; } qgtl_c64_t;
+
+enum {
+    QGTL_STORE_INPLACE = 0,   /* write the tile back where it was read */
+    QGTL_STORE_RING = 1,      /* tile = bits 0..m-1; apply CNOT(i->i+1 mod n), i ascending, while storing */
+    QGTL_STORE_PERM = 2       /* whole state in one tile; apply perm_* (CNOT/SWAP list) while storing */
+};
+
+/* One LDS-tiled sweep pass: a set of commuting 2x2 matrices on tile bits,
+ * followed by an optional index permutation on store. */
+typedef struct {
+    int32_t n;                                 /* qubits of the state */
+    int32_t m;                                 /* tile bits */
+    int32_t nlow;                              /* tile bits 0..nlow-1 are physical bits 0..nlow-1 */
+    int32_t nrest;                             /* non-tile bits (enumerated by the tile index) */
+    int32_t store;                             /* QGTL_STORE_* */
+    int32_t ngates;                            /* matrices in this pass (<= m) */
+    int32_t nperm;                             /* QGTL_STORE_PERM: gates in perm_* */
+    int32_t pad0;
+    int8_t hi_pos[QGTL_TILE_BITS_MAX];         /* physical bit of tile bit nlow+i */
+    int8_t rest_pos[QGTL_MAX_QUBITS];          /* physical bit of tile-index bit i */
+    int8_t gate_bit[QGTL_TILE_BITS_MAX];       /* tile bit of matrix g */
+    int8_t perm_type[QGTL_PERM_MAX];           /* 0 = CNOT(a -> b), 1 = SWAP(a, b) */
+    int8_t perm_a[QGTL_PERM_MAX];
+    int8_t perm_b[QGTL_PERM_MAX];
+    qgtl_c64_t mat[QGTL_TILE_BITS_MAX][4];     /* row-major 2x2 */
+} qgtl_pass_desc_t;
+
+/* Derivative-state injection in the frame after a block:
+ * d_k[y] = A_k[s][s] psi[y] + A_k[s][1-s] psi[y ^ v_k], s = parity(y & w_k). */
+typedef struct {
+    int32_t n;
+    int32_t K;
+    uint32_t v[QGTL_INJECT_MAX];
+    uint32_t w[QGTL_INJECT_MAX];
+    qgtl_c64_t A[QGTL_INJECT_MAX][4];
+} qgtl_inject_desc_t;
+
+/* ---- runtime ---- */
+int qgtl_hip_init(int device);
+int qgtl_hip_device_count(void);
+void *qgtl_hip_stream(void);                          /* the library's stream (default priority) */
+int qgtl_hip_malloc(void **ptr, size_t bytes);
+int qgtl_hip_free(void *ptr);
+int qgtl_hip_h2d(void *dst, const void *src, size_t bytes);   /* async on the library stream */
+int qgtl_hip_d2h(void *dst, const void *src, size_t bytes);   /* async on the library stream */
+int qgtl_hip_d2d(void *dst, const void *src, size_t bytes);
+int qgtl_hip_sync(void);
+const char *qgtl_hip_last_error(void);
+
+typedef struct qgtl_hip_event_s *qgtl_hip_event_t;
+int qgtl_hip_event_create(qgtl_hip_event_t *ev);
+int qgtl_hip_event_record(qgtl_hip_event_t ev);
+int qgtl_hip_event_elapsed_ms(qgtl_hip_event_t start, qgtl_hip_event_t stop, float *ms);
+int qgtl_hip_event_destroy(qgtl_hip_event_t ev);
+
+/* ---- complex64 sweeps (mixed mode) ---- */
+int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0> */
+/* nstates states at src + s*stride -> dst + s*stride (elements of complex64) */
+int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
+                      int nstates, int tile_variant);
+int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
+                        double *partials, int *tiles_out);
+/* out[k*4 + c] = sum over t of partials[(k*tiles + t)*4 + c], fixed order, FP64 */
+int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, double *out);
+int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t);
+int qgtl_hip_swap_c64(void *base, size_t stride, int nstates, int n, int a, int b);
+int qgtl_hip_c64_to_c128(const void *src, void *dst, size_t count);
+
+/* ---- Gram (mixed): C = Z Z^T with Z rows (Re d_s, Im d_s), split-BF16x3 on MFMA ---- */
+int qgtl_hip_gram_tiles(int P);                                          /* upper 128x128 tiles */
+/* terms = 3: split-BF16x3; terms = 6: exact three-piece split, six products */
+int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms);
+int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C);   /* C: R x R tiles, FP64 */
+/* Q = normalised G - a a^H from C and injection sums (P x 4 doubles). */
+int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q);
+/* MFMA lane-layout probe (diagnostics): 32 x 32 floats to out_dev */
+int qgtl_hip_debug_mfma_layout(float *out_dev);
+
+/* ---- complex128 plain path ---- */
+int qgtl_hip_fill_zero_state_c128(void *psi, int n);
+int qgtl_hip_gate1q_c128(void *psi, int n, int q, const double *m8);
+int qgtl_hip_cgate_c128(void *psi, int n, int c, int t, const double *m8);
+int qgtl_hip_swap_c128(void *psi, int n, int a, int b);
+/* G (P x P complex, upper triangle) = D^H D; partials sized by gram_f64_partials(P, N) doubles */
+size_t qgtl_hip_gram_f64_partials(int P, size_t N);
+int qgtl_hip_gram_f64(const void *D, int P, size_t N, double *partials, double *G);
+/* a_k = <D_k|psi>, P complex */
+int qgtl_hip_dotc_c128(const void *D, int P, size_t N, const void *psi, double *partials, double *a);
+int qgtl_hip_qgt_finalize_plain(const double *G, const double *a, int P, double *Q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

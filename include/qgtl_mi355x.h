@@ -1,0 +1,113 @@
+/*
+ * qgtl_mi355x.h -- additive entry points of libqgtl_mi355x.so.
+ *
+ * QGTL has no counterpart for these symbols: they expose FP64 output, plan
+ * reuse, batching, timing and the natural-gradient solvers that QGTL's
+ * ComplexFloat API cannot carry.  The drop-in API in qgtl_api.h is built on
+ * top of them.  All functions return 0 on success and a negative qgt_error_t
+ * code on failure; qgtl_last_error() then holds a message for the calling
+ * thread.
+ *
+ * Conventions: complex arrays are interleaved (re, im) doubles; qubit q is
+ * bit (1 << q) of a basis index; parameters are numbered in gate order
+ * (src/quantum_geometric/core/quantum_parameter_shift.c:278-337).
+ */
+#ifndef QGTL_MI355X_H
+#define QGTL_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One gate of a flat circuit.  type is a gate_type_t value.  Single-qubit
+ * gates: q0 = target, q1 = -1.  CNOT/CY/CZ: q0 = control, q1 = target
+ * (create_cnot_gate's {control, target}, quantum_gate_operations.c:420-423).
+ * SWAP: q0, q1.  param: variational parameter index carried by an RX/RY/RZ
+ * gate, or -1. */
+typedef struct {
+    int32_t type;
+    int32_t q0;
+    int32_t q1;
+    int32_t param;
+    double angle;
+} qgtl_op_t;
+
+enum {
+    QGTL_MODE_MIXED = 0,      /* complex64 sweeps, split-BF16x3 MFMA Gram, FP64 reductions/correction */
+    QGTL_MODE_FP64_PLAIN = 1  /* complex128, one kernel per gate, one derivative state at a time, FP64 Gram */
+};
+
+/* Per-call timing and traffic (device times come from HIP events recorded on
+ * the library's stream around each launch when profiling is enabled). */
+typedef struct {
+    double wall_ms;                 /* host wall time of the call */
+    double sweep_ms;                /* sum of sweep-pass kernel durations */
+    double inject_ms;               /* derivative-state injection kernels */
+    double gram_ms;                 /* Gram kernels (incl. split-K reduction) */
+    double finalize_ms;             /* FP64 correction / normalisation */
+    double other_ms;                /* everything else that was timed */
+    int64_t sweep_launches;
+    int64_t sweep_state_passes;     /* states x passes over the full state */
+    double sweep_bytes;             /* algorithmic bytes moved by sweep passes (16 B per amp per state-pass) */
+    int64_t gram_launches;
+    double gram_flops;              /* 4 N P (P+1) */
+    double gram_kernel_ms;          /* duration of the main Gram kernel */
+    int64_t kernel_launches;        /* all launches of the call */
+    int32_t P;
+    int32_t n;
+} qgtl_stats_t;
+
+/* Tunables chosen by the autotuner (or defaults when NULL / zero fields). */
+typedef struct {
+    int32_t chunk_states;     /* derivative states propagated together (0 = auto) */
+    int32_t gram_chunks;      /* split-K factor of the Gram (0 = auto) */
+    int32_t gram_variant;     /* 3: split-BF16x3, 6: split-BF16x6 (exact 3-piece split), 0: by size */
+    int32_t reserved[5];
+} qgtl_tune_t;
+
+int qgtl_init(int device);
+const char *qgtl_last_error(void);
+int qgtl_device_count(void);
+/* Enable per-launch HIP event timing (fills the *_ms fields of qgtl_stats_t). */
+void qgtl_set_profiling(int enable);
+
+/* Hardware-efficient ansatz builder: per layer RY(q), RZ(q) on every qubit,
+ * then CNOT(i -> i+1 mod n) for i = 0..n-1.  Returns the op count written
+ * (ops needs qgtl_hea_num_ops(n, layers) entries). */
+int qgtl_hea_num_ops(int n, int layers);
+int qgtl_hea_ops(int n, int layers, const double *theta, qgtl_op_t *ops);
+int qgtl_num_params(int nops, const qgtl_op_t *ops);
+
+/* Full P x P QGT, Q[mu][nu] = <d_mu psi|d_nu psi> - <d_mu psi|psi><psi|d_nu psi>,
+ * written as complex128 row-major.  Q may be NULL (timing runs). */
+int qgtl_qgt(int n, int nops, const qgtl_op_t *ops, int mode, double *Q, qgtl_stats_t *stats);
+
+/* Final state |psi> as complex128 (mixed mode computes in complex64). */
+int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *psi);
+
+/* Reusable plan: structure fixed at creation, angles replaceable. */
+typedef struct qgtl_plan qgtl_plan_t;
+qgtl_plan_t *qgtl_plan_create(int n, int nops, const qgtl_op_t *ops, int mode, const qgtl_tune_t *tune);
+int qgtl_plan_set_angles(qgtl_plan_t *plan, int nops, const double *angles);
+int qgtl_plan_execute(qgtl_plan_t *plan, double *Q, qgtl_stats_t *stats);
+int qgtl_plan_num_params(const qgtl_plan_t *plan);
+void qgtl_plan_destroy(qgtl_plan_t *plan);
+
+/* Release every cached device buffer of the calling process. */
+void qgtl_release_cache(void);
+
+/* Diagnostics: v_mfma_f32_32x32x16_bf16 lane-layout probe; writes the 32 x 32
+ * product D[i][j] = (1 + i/16)(i%16 + 16 (j%8)) as the Gram kernel reads it. */
+int qgtl_debug_mfma_layout(float *host_out);
+
+/* Diagnostics: raw FP64 Gram C = Z Z^T (R x R) and injection sums (P x 4)
+ * of the last mixed-mode execution of plan, in D-slot order. */
+int qgtl_plan_debug_buffers(qgtl_plan_t *plan, double *C_out, double *inj_out, int *R_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,176 @@
+"""quantum_geometric_tensor_amd -- Python view of libqgtl_mi355x.so.
+
+The library is C host code plus hand-written HIP kernels for gfx950 (see
+csrc/).  This module only binds its C ABI with ctypes for tests, the bench
+and scripting: every computation runs in the shared library on the GPU.
+There is no CPU fallback; importing works without a GPU, calling a compute
+function without the built library or without a GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libqgtl_mi355x.so")
+
+GATE = dict(I=0, X=1, Y=2, Z=3, H=4, S=5, T=6, RX=7, RY=8, RZ=9, CNOT=10, CY=11,
+            CZ=12, SWAP=13, U1=15, PHASE=19, SDG=26, TDG=27, SX=29)
+
+MODE_MIXED = 0
+MODE_FP64_PLAIN = 1
+
+OP_DTYPE = np.dtype([("type", np.int32), ("q0", np.int32), ("q1", np.int32),
+                     ("param", np.int32), ("angle", np.float64)], align=True)
+
+
+class QGTLError(RuntimeError):
+    """A negative qgt_error_t return, with the library's message."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"qgtl error {code}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("sweep_ms", ctypes.c_double),
+                ("inject_ms", ctypes.c_double), ("gram_ms", ctypes.c_double),
+                ("finalize_ms", ctypes.c_double), ("other_ms", ctypes.c_double),
+                ("sweep_launches", ctypes.c_int64), ("sweep_state_passes", ctypes.c_int64),
+                ("sweep_bytes", ctypes.c_double), ("gram_launches", ctypes.c_int64),
+                ("gram_flops", ctypes.c_double), ("gram_kernel_ms", ctypes.c_double),
+                ("kernel_launches", ctypes.c_int64), ("P", ctypes.c_int32), ("n", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [("chunk_states", ctypes.c_int32), ("gram_chunks", ctypes.c_int32),
+                ("gram_variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libqgtl_mi355x.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.qgtl_init.argtypes = [ip]
+        L.qgtl_last_error.restype = ctypes.c_char_p
+        L.qgtl_set_profiling.argtypes = [ip]
+        L.qgtl_hea_num_ops.argtypes = [ip, ip]
+        L.qgtl_hea_ops.argtypes = [ip, ip, vp, vp]
+        L.qgtl_num_params.argtypes = [ip, vp]
+        L.qgtl_qgt.argtypes = [ip, ip, vp, ip, vp, ctypes.POINTER(Stats)]
+        L.qgtl_statevector.argtypes = [ip, ip, vp, ip, vp]
+        L.qgtl_plan_create.argtypes = [ip, ip, vp, ip, ctypes.POINTER(Tune)]
+        L.qgtl_plan_create.restype = vp
+        L.qgtl_plan_set_angles.argtypes = [vp, ip, vp]
+        L.qgtl_plan_execute.argtypes = [vp, vp, ctypes.POINTER(Stats)]
+        L.qgtl_plan_num_params.argtypes = [vp]
+        L.qgtl_plan_destroy.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise QGTLError(rc, lib().qgtl_last_error().decode(errors="replace"))
+
+
+def init(device=0):
+    _check(lib().qgtl_init(device))
+
+
+def set_profiling(enable):
+    lib().qgtl_set_profiling(1 if enable else 0)
+
+
+def hea_ops(n, layers, theta):
+    """Hardware-efficient ansatz: RY(q), RZ(q) on every qubit, then CNOT(i -> i+1 mod n)."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    if theta.size != 2 * n * layers:
+        raise ValueError("theta must hold 2*n*layers angles")
+    ops = np.zeros(lib().qgtl_hea_num_ops(n, layers), OP_DTYPE)
+    lib().qgtl_hea_ops(n, layers, _p(theta), _p(ops))
+    return ops
+
+
+def make_ops(gates):
+    """gates: iterable of (name, q0, q1, param, angle)."""
+    gates = list(gates)
+    ops = np.zeros(len(gates), OP_DTYPE)
+    for i, (name, q0, q1, param, angle) in enumerate(gates):
+        ops[i] = (GATE[name], q0, q1, param, angle)
+    return ops
+
+
+def num_params(ops):
+    return int(np.sum(ops["param"] >= 0))
+
+
+def qgt(n, ops, mode=MODE_MIXED, stats=False):
+    """Full P x P quantum geometric tensor (complex128)."""
+    ops = np.ascontiguousarray(ops, OP_DTYPE)
+    P = num_params(ops)
+    Q = np.zeros((P, P), np.complex128)
+    st = Stats()
+    _check(lib().qgtl_qgt(n, len(ops), _p(ops), mode, _p(Q), ctypes.byref(st)))
+    return (Q, st.as_dict()) if stats else Q
+
+
+def statevector(n, ops, mode=MODE_MIXED):
+    ops = np.ascontiguousarray(ops, OP_DTYPE)
+    psi = np.zeros(1 << n, np.complex128)
+    _check(lib().qgtl_statevector(n, len(ops), _p(ops), mode, _p(psi)))
+    return psi
+
+
+class Plan:
+    """A compiled circuit with device buffers, re-run with new angles."""
+
+    def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0):
+        self.ops = np.ascontiguousarray(ops, OP_DTYPE).copy()
+        self.n = n
+        tune = Tune(chunk_states, gram_chunks, gram_terms)
+        h = lib().qgtl_plan_create(n, len(self.ops), _p(self.ops), mode, ctypes.byref(tune))
+        if not h:
+            raise QGTLError(-1, lib().qgtl_last_error().decode(errors="replace"))
+        self._h = h
+        self.P = lib().qgtl_plan_num_params(h)
+
+    def set_angles(self, angles):
+        angles = np.ascontiguousarray(angles, np.float64)
+        _check(lib().qgtl_plan_set_angles(self._h, len(angles), _p(angles)))
+
+    def execute(self, want_q=True):
+        st = Stats()
+        Q = np.zeros((self.P, self.P), np.complex128) if want_q else None
+        _check(lib().qgtl_plan_execute(self._h, _p(Q) if want_q else None, ctypes.byref(st)))
+        return Q, st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().qgtl_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rel_err(Q, Qref):
+    """Normwise relative error max|Q - Qref| / max|Qref|."""
+    return float(np.max(np.abs(Q - Qref)) / np.max(np.abs(Qref)))

@@ -1,0 +1,300 @@
+// Complex Gram G = D^H D of the derivative states on MFMA, split-BF16x3.
+//
+// D holds P complex64 states of N amplitudes (state s at D + s*N).  The real
+// matrix Z has rows 2s = Re d_s and 2s+1 = Im d_s, so C = Z Z^T gives
+//   Re G_kl = C[2k][2l] + C[2k+1][2l+1],  Im G_kl = C[2k][2l+1] - C[2k+1][2l].
+// Each fp32 value x is split into bf16 pieces.  Split-BF16x3: hi = rn(x),
+// lo = rn(x - hi); a product keeps hi*hi + hi*lo + lo*hi (three
+// v_mfma_f32_32x32x16_bf16 per 32x32x16 step, fp32 accumulation; 2^-17
+// relative per element, which averages out over long states).  Split-BF16x6:
+// x = hi + mid + lo exactly (3 x 8 = 24 significand bits) and the product
+// keeps the six terms down to 2^-16 (hh, hm, mh, hl, lh, mm): fp32-level
+// products for short states, where nothing averages.  K (the amplitude index) is split into
+// chunks; each workgroup writes an fp32 partial 128x128 tile of C and a
+// second kernel sums the chunks in FP64 in a fixed order.  Chunk c's tiles
+// all run on XCD c % 8 so the chunk is fetched into one L2 only.
+//
+// QGTL counterpart: the per-element overlap loop
+// src/quantum_geometric/core/quantum_geometric_tensor_network.c:1127-1137 and
+// its FP64 form src/quantum_geometric/distributed/differential_geometry.c:2838-2842.
+#include "qgtl_hip_common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kThreads = 256;
+constexpr int kTile = 128;          // output tile (rows of Z)
+constexpr int kKS = 32;             // amplitudes per LDS stage
+constexpr int kRow = kKS + 8;       // bf16 per LDS row (80 B: 16-B aligned, spreads banks)
+constexpr int kXcd = 8;
+
+__device__ __forceinline__ uint32_t bf16_rn_bits(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+__device__ __forceinline__ void split2(float a, float b, uint32_t &hi, uint32_t &lo) {
+    const uint32_t ha = bf16_rn_bits(a), hb = bf16_rn_bits(b);
+    const float ra = a - __uint_as_float(ha << 16), rb = b - __uint_as_float(hb << 16);
+    hi = ha | (hb << 16);
+    lo = bf16_rn_bits(ra) | (bf16_rn_bits(rb) << 16);
+}
+
+__device__ __forceinline__ void split3(float a, float b, uint32_t &hi, uint32_t &mid, uint32_t &lo) {
+    const uint32_t ha = bf16_rn_bits(a), hb = bf16_rn_bits(b);
+    const float ra = a - __uint_as_float(ha << 16), rb = b - __uint_as_float(hb << 16);
+    const uint32_t ma = bf16_rn_bits(ra), mb = bf16_rn_bits(rb);
+    const float sa = ra - __uint_as_float(ma << 16), sb = rb - __uint_as_float(mb << 16);
+    hi = ha | (hb << 16);
+    mid = ma | (mb << 16);
+    lo = bf16_rn_bits(sa) | (bf16_rn_bits(sb) << 16);
+}
+
+__device__ __forceinline__ int upper_tiles(int nt) { return nt * (nt + 1) / 2; }
+
+__device__ __forceinline__ void tile_coords(int t, int nt, int &ti, int &tj) {
+    int i = 0, rem = t;
+    while (rem >= nt - i) { rem -= nt - i; i++; }
+    ti = i;
+    tj = i + rem;
+}
+
+// Load 64 states x kKS amplitudes of D starting at state s0, amplitude k0,
+// into LDS rows (2s, 2s+1) = (Re, Im) as S bf16 pieces (S = 2 or 3 arrays).
+template <int S>
+__device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, size_t N, int s0, size_t k0,
+                                           uint32_t *const *rows) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int L = i * kThreads + threadIdx.x;
+        const int sl = L >> 4, pair = L & 15;
+        const int s = s0 + sl;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (s < P && k0 + 2 * pair < N) v = *reinterpret_cast<const float4 *>(D + 2 * ((size_t)s * N + k0 + 2 * pair));
+        const int wr = (2 * sl) * (kRow / 2) + pair, wi = (2 * sl + 1) * (kRow / 2) + pair;
+        if (S == 2) {
+            uint32_t hr, lr, hm, lm;
+            split2(v.x, v.z, hr, lr);  // Re of amplitudes 2pair, 2pair+1
+            split2(v.y, v.w, hm, lm);  // Im
+            rows[0][wr] = hr; rows[1][wr] = lr;
+            rows[0][wi] = hm; rows[1][wi] = lm;
+        } else {
+            uint32_t a0, a1, a2, b0, b1, b2;
+            split3(v.x, v.z, a0, a1, a2);
+            split3(v.y, v.w, b0, b1, b2);
+            rows[0][wr] = a0; rows[1][wr] = a1; rows[2][wr] = a2;
+            rows[0][wi] = b0; rows[1][wi] = b1; rows[2][wi] = b2;
+        }
+    }
+}
+
+// S = 2: split-BF16x3 (3 MFMA terms);  S = 3: split-BF16x6 (6 MFMA terms).
+template <int S>
+__global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
+                                                              int nt, int nchunks, size_t chunk_len,
+                                                              float *__restrict__ partials) {
+    __shared__ uint32_t lds[2 * S][kTile * kRow / 2];   // A pieces, then B pieces
+
+    const int T = upper_tiles(nt);
+    const int bid = blockIdx.x;
+    const int xcd = bid % kXcd, slot = bid / kXcd;
+    const int tile = slot % T, chunk = (slot / T) * kXcd + xcd;
+    if (chunk >= nchunks) return;
+    int ti, tj;
+    tile_coords(tile, nt, ti, tj);
+    const bool diag = (ti == tj);
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[a][b][e] = 0.f;
+
+    const size_t kbeg = (size_t)chunk * chunk_len;
+    const size_t kend = (kbeg + chunk_len < N) ? kbeg + chunk_len : N;
+    uint32_t *arows[S], *brows[S];
+#pragma unroll
+    for (int p = 0; p < S; p++) {
+        arows[p] = lds[p];
+        brows[p] = diag ? lds[p] : lds[S + p];
+    }
+
+    for (size_t k0 = kbeg; k0 < kend; k0 += kKS) {
+        stage_rows<S>(D, P, N, ti * (kTile / 2), k0, arows);
+        if (!diag) stage_rows<S>(D, P, N, tj * (kTile / 2), k0, brows);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKS / 16; kk++) {
+            const int koff = kk * 16 + 8 * (lane >> 5);
+            bf16x8 fa[S][2], fb[S][2];
+#pragma unroll
+            for (int p = 0; p < S; p++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const int ra = wm * 64 + b * 32 + (lane & 31);
+                    const int rb = wn * 64 + b * 32 + (lane & 31);
+                    fa[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(arows[p]) + ra * kRow + koff);
+                    fb[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(brows[p]) + rb * kRow + koff);
+                }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    // smallest terms first
+                    if (S == 3) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][a], fb[0][b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2][b], acc[a][b], 0, 0, 0);
+                    }
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[0][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1][b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+    float *out = partials + ((size_t)chunk * T + tile) * (kTile * kTile);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int row = wm * 64 + a * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+                const int col = wn * 64 + b * 32 + (lane & 31);
+                out[row * kTile + col] = acc[a][b][e];
+            }
+}
+
+// C[ti*128 + r][tj*128 + c] = sum over chunks (fixed order, FP64); C is R x R.
+__global__ __launch_bounds__(kThreads) void gram_reduce_kernel(const float *__restrict__ partials, int nt,
+                                                               int nchunks, double *__restrict__ C) {
+    const int T = upper_tiles(nt);
+    const int R = nt * kTile;
+    const size_t total = (size_t)T * kTile * kTile;
+    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < total;
+         idx += (size_t)gridDim.x * kThreads) {
+        const int tile = (int)(idx / (kTile * kTile));
+        const int e = (int)(idx % (kTile * kTile));
+        double s = 0.0;
+        for (int c = 0; c < nchunks; c++) s += (double)partials[((size_t)c * T + tile) * (kTile * kTile) + e];
+        int ti, tj;
+        tile_coords(tile, nt, ti, tj);
+        C[(size_t)(ti * kTile + e / kTile) * R + tj * kTile + (e % kTile)] = s;
+    }
+}
+
+// Q_kl = G_kl / (4 sqrt(G_kk G_ll)) - a_k conj(a_l), a_k = <d_k|psi> / (2 sqrt(|d_k|^2 |psi|^2)).
+// The normalisation applies the exact identities ||d_k psi|| = 1/2 and
+// ||psi|| = 1 to remove the norm drift of the complex64 sweeps.
+__global__ __launch_bounds__(kThreads) void qgt_finalize_kernel(const double *__restrict__ C, int R,
+                                                                const double *__restrict__ inj, int P,
+                                                                double *__restrict__ Q) {
+    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < (size_t)P * P;
+         idx += (size_t)gridDim.x * kThreads) {
+        const int k = (int)(idx / P), l = (int)(idx % P);
+        if (k > l) continue;
+        const double gre = C[(size_t)(2 * k) * R + 2 * l] + C[(size_t)(2 * k + 1) * R + 2 * l + 1];
+        const double gim = C[(size_t)(2 * k) * R + 2 * l + 1] - C[(size_t)(2 * k + 1) * R + 2 * l];
+        const double gkk = C[(size_t)(2 * k) * R + 2 * k] + C[(size_t)(2 * k + 1) * R + 2 * k + 1];
+        const double gll = C[(size_t)(2 * l) * R + 2 * l] + C[(size_t)(2 * l + 1) * R + 2 * l + 1];
+        const double sg = 0.25 / sqrt(gkk * gll);
+        const double sk = 0.5 / sqrt(inj[4 * k + 2] * inj[4 * k + 3]);
+        const double sl = 0.5 / sqrt(inj[4 * l + 2] * inj[4 * l + 3]);
+        const double akr = inj[4 * k] * sk, aki = inj[4 * k + 1] * sk;
+        const double alr = inj[4 * l] * sl, ali = inj[4 * l + 1] * sl;
+        // a_k conj(a_l)
+        const double pr = akr * alr + aki * ali, pi = aki * alr - akr * ali;
+        const double qr = gre * sg - pr, qi = (k == l) ? 0.0 : gim * sg - pi;
+        Q[2 * ((size_t)k * P + l)] = qr;
+        Q[2 * ((size_t)k * P + l) + 1] = qi;
+        if (k != l) {
+            Q[2 * ((size_t)l * P + k)] = qr;
+            Q[2 * ((size_t)l * P + k) + 1] = -qi;
+        }
+    }
+}
+
+// Lane layout probe of v_mfma_f32_32x32x16_bf16 with the operand layout the
+// Gram assumes (lane l holds row/col l % 32, k = 8 (l / 32) + e):
+// A[i][k] = (k == i % 16) ? 1 + i / 16 : 0 and B[k][j] = k + 16 (j % 8), so
+// D[i][j] = (1 + i / 16) (i % 16 + 16 (j % 8)), all exact in bf16/fp32.
+__global__ void mfma_layout_probe_kernel(float *out) {
+    const int lane = threadIdx.x;
+    bf16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = lane & 31, k = 8 * (lane >> 5) + e, j = lane & 31;
+        a[e] = (__bf16)(float)((k == (i % 16)) ? 1 + i / 16 : 0);
+        b[e] = (__bf16)(float)(k + 16 * (j % 8));
+    }
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0.f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int row = (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3), col = lane & 31;
+        out[row * 32 + col] = acc[e];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int qgtl_hip_gram_tiles(int P) {
+    const int nt = (2 * P + kTile - 1) / kTile;
+    return nt * (nt + 1) / 2;
+}
+
+int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms) {
+    if (P < 1 || nchunks < 1 || nchunks % kXcd != 0 || (N & 1))
+        return qgtl::set_error_msg(-118, "qgtl_hip_gram_bf16x3: nchunks must be a multiple of 8 and N even");
+    const int nt = (2 * P + kTile - 1) / kTile;
+    const int T = nt * (nt + 1) / 2;
+    const size_t chunk_len = ((N + nchunks - 1) / nchunks + kKS - 1) / kKS * kKS;
+    if (terms == 6)
+        hipLaunchKernelGGL(gram_split_kernel<3>, dim3((unsigned)(T * nchunks)), dim3(kThreads), 0, qgtl::stream(),
+                           (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+    else
+        hipLaunchKernelGGL(gram_split_kernel<2>, dim3((unsigned)(T * nchunks)), dim3(kThreads), 0, qgtl::stream(),
+                           (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+    QGTL_LAUNCH_CHECK("gram_bf16x3");
+    return 0;
+}
+
+int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C) {
+    const int nt = (2 * P + kTile - 1) / kTile;
+    const size_t total = (size_t)(nt * (nt + 1) / 2) * kTile * kTile;
+    unsigned grid = (unsigned)((total + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), partials, nt, nchunks, C);
+    QGTL_LAUNCH_CHECK("gram_reduce");
+    return 0;
+}
+
+int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q) {
+    const int nt = (2 * P + kTile - 1) / kTile;
+    const size_t work = (size_t)P * P;
+    unsigned grid = (unsigned)((work + kThreads - 1) / kThreads);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(qgt_finalize_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), C, nt * kTile, inj, P, Q);
+    QGTL_LAUNCH_CHECK("qgt_finalize");
+    return 0;
+}
+
+int qgtl_hip_debug_mfma_layout(float *out_dev) {
+    hipLaunchKernelGGL(mfma_layout_probe_kernel, dim3(1), dim3(64), 0, qgtl::stream(), out_dev);
+    QGTL_LAUNCH_CHECK("mfma_layout_probe");
+    return 0;
+}
+
+}  // extern "C"

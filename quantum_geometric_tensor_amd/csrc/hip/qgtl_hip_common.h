@@ -1,0 +1,26 @@
+// Shared definitions for the HIP translation units of libqgtl_mi355x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "qgtl_hip_abi.h"
+
+namespace qgtl {
+
+hipStream_t stream();
+int set_error(hipError_t e, const char *what);
+int set_error_msg(int code, const char *msg);
+
+}  // namespace qgtl
+
+// Map a HIP status to the library's error codes and remember the message.
+#define QGTL_HIP_CHECK(call)                                                   \
+    do {                                                                       \
+        hipError_t _e = (call);                                                \
+        if (_e != hipSuccess) return qgtl::set_error(_e, #call);               \
+    } while (0)
+
+#define QGTL_LAUNCH_CHECK(what)                                                \
+    do {                                                                       \
+        hipError_t _e = hipGetLastError();                                     \
+        if (_e != hipSuccess) return qgtl::set_error(_e, what);                \
+    } while (0)

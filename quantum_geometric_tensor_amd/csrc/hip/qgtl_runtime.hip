@@ -1,0 +1,125 @@
+// Device, stream, memory and event layer of the thin C ABI (qgtl_hip_abi.h).
+// One process drives one GPU through one non-blocking stream of default
+// priority; every kernel and copy of the library is ordered on it.
+#include <stdio.h>
+#include <string.h>
+
+#include "qgtl_hip_common.h"
+
+namespace {
+hipStream_t g_stream = nullptr;
+int g_device = -1;
+thread_local char g_msg[256] = "";
+
+int code_for(hipError_t e) {
+    switch (e) {
+    case hipErrorOutOfMemory: return -119;          // QGT_ERROR_GPU_OUT_OF_MEMORY
+    case hipErrorInvalidValue: return -118;         // QGT_ERROR_GPU_INVALID_VALUE
+    case hipErrorLaunchFailure:
+    case hipErrorInvalidConfiguration: return -121; // QGT_ERROR_GPU_LAUNCH_FAILED
+    default: return -120;                           // QGT_ERROR_GPU_INTERNAL
+    }
+}
+}  // namespace
+
+namespace qgtl {
+hipStream_t stream() { return g_stream; }
+
+int set_error(hipError_t e, const char *what) {
+    snprintf(g_msg, sizeof g_msg, "%s: %s", what, hipGetErrorString(e));
+    return code_for(e);
+}
+
+int set_error_msg(int code, const char *msg) {
+    snprintf(g_msg, sizeof g_msg, "%s", msg);
+    return code;
+}
+}  // namespace qgtl
+
+extern "C" {
+
+const char *qgtl_hip_last_error(void) { return g_msg; }
+
+int qgtl_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int qgtl_hip_init(int device) {
+    if (g_stream && device == g_device) return 0;
+    int count = 0;
+    QGTL_HIP_CHECK(hipGetDeviceCount(&count));
+    if (count <= 0) return qgtl::set_error_msg(-6, "no HIP device visible");
+    if (device < 0 || device >= count) return qgtl::set_error_msg(-118, "device index out of range");
+    QGTL_HIP_CHECK(hipSetDevice(device));
+    if (g_stream) {
+        (void)hipStreamDestroy(g_stream);
+        g_stream = nullptr;
+    }
+    QGTL_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return 0;
+}
+
+void *qgtl_hip_stream(void) { return (void *)g_stream; }
+
+int qgtl_hip_malloc(void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    QGTL_HIP_CHECK(hipMalloc(ptr, bytes));
+    return 0;
+}
+
+int qgtl_hip_free(void *ptr) {
+    if (ptr) QGTL_HIP_CHECK(hipFree(ptr));
+    return 0;
+}
+
+int qgtl_hip_h2d(void *dst, const void *src, size_t bytes) {
+    if (bytes) QGTL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    return 0;
+}
+
+int qgtl_hip_d2h(void *dst, const void *src, size_t bytes) {
+    if (bytes) QGTL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    return 0;
+}
+
+int qgtl_hip_d2d(void *dst, const void *src, size_t bytes) {
+    if (bytes) QGTL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    return 0;
+}
+
+int qgtl_hip_sync(void) {
+    QGTL_HIP_CHECK(hipStreamSynchronize(g_stream));
+    QGTL_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+struct qgtl_hip_event_s { hipEvent_t e; };
+
+int qgtl_hip_event_create(qgtl_hip_event_t *ev) {
+    *ev = new qgtl_hip_event_s;
+    QGTL_HIP_CHECK(hipEventCreate(&(*ev)->e));
+    return 0;
+}
+
+int qgtl_hip_event_record(qgtl_hip_event_t ev) {
+    QGTL_HIP_CHECK(hipEventRecord(ev->e, g_stream));
+    return 0;
+}
+
+int qgtl_hip_event_elapsed_ms(qgtl_hip_event_t a, qgtl_hip_event_t b, float *ms) {
+    QGTL_HIP_CHECK(hipEventElapsedTime(ms, a->e, b->e));
+    return 0;
+}
+
+int qgtl_hip_event_destroy(qgtl_hip_event_t ev) {
+    if (!ev) return 0;
+    (void)hipEventDestroy(ev->e);
+    delete ev;
+    return 0;
+}
+
+}  // extern "C"

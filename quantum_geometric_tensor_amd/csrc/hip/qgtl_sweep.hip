@@ -1,0 +1,370 @@
+// Complex64 state-vector sweeps for the mixed-precision QGT path.
+//
+// A "pass" loads one tile of 2^m amplitudes (the tile bits vary, every other
+// bit is fixed by the tile index) into LDS, applies a set of commuting 2x2
+// matrices on tile bits (three at a time from registers, so one LDS round
+// trip serves three gates), and stores the tile, optionally through an index
+// permutation: the CNOT ring of the hardware-efficient ansatz
+// (CNOT(i -> i+1 mod n), i ascending) or, when the whole state is one tile,
+// any CNOT/SWAP list.  Counterparts in QGTL: the per-gate loops
+// apply_rotation_y/z and apply_controlled_x
+// (src/quantum_geometric/core/quantum_circuit_operations.c:171-265) and
+// apply_single_qubit_gate / apply_cnot_gate
+// (src/quantum_geometric/hybrid/quantum_classical_algorithms.c:174-210).
+//
+// Ring store, derived from the ring map y_j = x_0 ^ ... ^ x_j (j >= 1),
+// y_0 = x_1 ^ ... ^ x_{n-1}: for a tile over bits 0..m-1 with tile index r
+// (bits m..n-1), the low output bits depend on the tile only, and the output
+// high bits are prefix_xor(r), complemented when y_{m-1} = 1.  The tile thus
+// lands in two contiguous half tiles: every store is coalesced.
+#include "qgtl_hip_common.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kHiTableBits = 9;     // tile bits outside the contiguous low run
+
+__device__ __forceinline__ int lds_pad(int j) { return j + (j >> 3); }
+
+__device__ __forceinline__ float2 cmadd(float2 acc, float2 a, float2 b) {
+    acc.x = fmaf(a.x, b.x, acc.x);
+    acc.x = fmaf(-a.y, b.y, acc.x);
+    acc.y = fmaf(a.x, b.y, acc.y);
+    acc.y = fmaf(a.y, b.x, acc.y);
+    return acc;
+}
+
+__device__ __forceinline__ void apply2(float2 &a, float2 &b, const float2 *m) {
+    float2 na = cmadd(cmadd(make_float2(0.f, 0.f), m[0], a), m[1], b);
+    float2 nb = cmadd(cmadd(make_float2(0.f, 0.f), m[2], a), m[3], b);
+    a = na;
+    b = nb;
+}
+
+__device__ __forceinline__ uint32_t insert_zero(uint32_t x, int b) {
+    uint32_t lo = x & ((1u << b) - 1u);
+    return ((x >> b) << (b + 1)) | lo;
+}
+
+// Three (or fewer) gates on tile bits bits[0] < bits[1] < bits[2]: each
+// thread owns 2^G amplitudes that differ only in those bits.
+template <int G>
+__device__ __forceinline__ void apply_group(float2 *tile, int m, const int *bits, const float2 (*mats)[4]) {
+    const int count = 1 << (m - G);
+    for (int o = threadIdx.x; o < count; o += kThreads) {
+        uint32_t base = (uint32_t)o;
+#pragma unroll
+        for (int i = 0; i < G; i++) base = insert_zero(base, bits[i]);
+        float2 v[1 << G];
+        int idx[1 << G];
+#pragma unroll
+        for (int c = 0; c < (1 << G); c++) {
+            uint32_t id = base;
+#pragma unroll
+            for (int i = 0; i < G; i++)
+                if ((c >> i) & 1) id |= 1u << bits[i];
+            idx[c] = lds_pad((int)id);
+            v[c] = tile[idx[c]];
+        }
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+#pragma unroll
+            for (int c = 0; c < (1 << G); c++) {
+                if ((c >> i) & 1) continue;
+                apply2(v[c], v[c | (1 << i)], mats[i]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < (1 << G); c++) tile[idx[c]] = v[c];
+    }
+}
+
+// Inverse of the local part of the ring map for the low tile (see header).
+__device__ __forceinline__ uint32_t ring_local_inverse(uint32_t y, int m, uint32_t par_r) {
+    uint32_t x0 = (y & 1u) ^ ((y >> (m - 1)) & 1u) ^ par_r;
+    uint32_t x1 = ((y >> 1) & 1u) ^ x0;
+    uint32_t rest = (y ^ (y << 1)) & ~3u & ((1u << m) - 1u);
+    return rest | (x1 << 1) | x0;
+}
+
+template <int TB>
+__global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc_t d,
+                                                            const float2 *__restrict__ src_all,
+                                                            float2 *dst_all, size_t stride) {
+    __shared__ float2 tile[(1 << TB) + (1 << TB) / 8];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+
+    const int m = d.m, nlow = d.nlow, nhi = m - nlow;
+    const uint32_t r = blockIdx.x;
+    const float2 *src = src_all + (size_t)blockIdx.y * stride;
+    float2 *dst = dst_all + (size_t)blockIdx.y * stride;
+
+    uint32_t base = 0;
+    for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
+    for (int h = threadIdx.x; h < (1 << nhi); h += kThreads) {
+        uint32_t off = 0;
+        for (int i = 0; i < nhi; i++) off |= ((uint32_t)(h >> i) & 1u) << d.hi_pos[i];
+        hi_off[h] = off;
+    }
+    __syncthreads();
+
+    const int T = 1 << m;
+    const uint32_t lowmask = (1u << nlow) - 1u;
+    for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
+        const int j = 2 * p;
+        const uint32_t g = base + hi_off[j >> nlow] + ((uint32_t)j & lowmask);
+        const float4 v = *reinterpret_cast<const float4 *>(src + g);
+        tile[lds_pad(j)] = make_float2(v.x, v.y);
+        tile[lds_pad(j + 1)] = make_float2(v.z, v.w);
+    }
+    __syncthreads();
+
+    for (int g0 = 0; g0 < d.ngates; g0 += 3) {
+        const int G = min(3, d.ngates - g0);
+        int bits[3];
+        float2 mats[3][4];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int gi = min(g0 + i, d.ngates - 1);
+            bits[i] = d.gate_bit[gi];
+#pragma unroll
+            for (int e = 0; e < 4; e++) mats[i][e] = make_float2(d.mat[gi][e].re, d.mat[gi][e].im);
+        }
+        if (G == 3) apply_group<3>(tile, m, bits, mats);
+        else if (G == 2) apply_group<2>(tile, m, bits, mats);
+        else apply_group<1>(tile, m, bits, mats);
+        __syncthreads();
+    }
+
+    if (d.store == QGTL_STORE_INPLACE) {
+        for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
+            const int j = 2 * p;
+            const uint32_t g = base + hi_off[j >> nlow] + ((uint32_t)j & lowmask);
+            const float2 a = tile[lds_pad(j)], b = tile[lds_pad(j + 1)];
+            *reinterpret_cast<float4 *>(dst + g) = make_float4(a.x, a.y, b.x, b.y);
+        }
+    } else if (d.store == QGTL_STORE_RING) {
+        // tile = bits 0..m-1, tile index r = bits m..n-1
+        const int nh = d.n - m;
+        uint32_t pre = r;
+        pre ^= pre << 1; pre ^= pre << 2; pre ^= pre << 4; pre ^= pre << 8; pre ^= pre << 16;
+        const uint32_t ones = (nh >= 32) ? 0xffffffffu : ((1u << nh) - 1u);
+        pre &= ones;
+        const uint32_t par_r = (uint32_t)__popc(r) & 1u;
+        for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
+            const uint32_t y = 2u * (uint32_t)p;
+            const float2 a = tile[lds_pad((int)ring_local_inverse(y, m, par_r))];
+            const float2 b = tile[lds_pad((int)ring_local_inverse(y + 1u, m, par_r))];
+            const uint32_t yhi = ((y >> (m - 1)) & 1u) ? (pre ^ ones) : pre;
+            const size_t g = ((size_t)yhi << m) | y;
+            *reinterpret_cast<float4 *>(dst + g) = make_float4(a.x, a.y, b.x, b.y);
+        }
+    } else {  // QGTL_STORE_PERM: whole state in this tile, undo the gate list per output index
+        for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
+            float2 o[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                uint32_t x = 2u * (uint32_t)p + (uint32_t)e;
+                for (int i = d.nperm - 1; i >= 0; i--) {
+                    const uint32_t ab = (x >> d.perm_a[i]) & 1u, bb = (x >> d.perm_b[i]) & 1u;
+                    if (d.perm_type[i] == 0) x ^= ab << d.perm_b[i];              // CNOT a -> b
+                    else x ^= (ab ^ bb) * ((1u << d.perm_a[i]) | (1u << d.perm_b[i]));  // SWAP
+                }
+                o[e] = tile[lds_pad((int)x)];
+            }
+            *reinterpret_cast<float4 *>(dst + 2 * p) = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
+        }
+    }
+}
+
+// Derivative states in the frame after a block (see qgtl_inject_desc_t), with
+// FP64 per-tile sums of <d|psi>, |d|^2 and |psi|^2.
+constexpr int kInjectPerThread = 8;
+constexpr int kInjectTile = kThreads * kInjectPerThread;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_desc_t d,
+                                                              const float2 *__restrict__ psi,
+                                                              float2 *dst_all, size_t stride,
+                                                              double *partials) {
+    const int k = blockIdx.y;
+    const uint32_t N = 1u << d.n;
+    const uint32_t v = d.v[k], w = d.w[k];
+    const float2 a00 = make_float2(d.A[k][0].re, d.A[k][0].im), a01 = make_float2(d.A[k][1].re, d.A[k][1].im);
+    const float2 a10 = make_float2(d.A[k][2].re, d.A[k][2].im), a11 = make_float2(d.A[k][3].re, d.A[k][3].im);
+    float2 *dst = dst_all + (size_t)k * stride;
+    double sr = 0.0, si = 0.0, sd = 0.0, sp = 0.0;
+#pragma unroll
+    for (int i = 0; i < kInjectPerThread; i++) {
+        const uint32_t y = blockIdx.x * kInjectTile + i * kThreads + threadIdx.x;
+        if (y < N) {
+            const float2 a = psi[y], b = psi[y ^ v];
+            const bool s = (__popc(y & w) & 1) != 0;
+            const float2 mss = s ? a11 : a00, mso = s ? a10 : a01;
+            const float2 o = cmadd(cmadd(make_float2(0.f, 0.f), mss, a), mso, b);
+            dst[y] = o;
+            const double dr = o.x, di = o.y, pr = a.x, pi = a.y;
+            sr += dr * pr + di * pi;
+            si += dr * pi - di * pr;
+            sd += dr * dr + di * di;
+            sp += pr * pr + pi * pi;
+        }
+    }
+    __shared__ double red[kThreads / 64][4];
+    sr = wave_sum(sr); si = wave_sum(si); sd = wave_sum(sd); sp = wave_sum(sp);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = sr; red[wave][1] = si; red[wave][2] = sd; red[wave][3] = sp;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int wv = 0; wv < kThreads / 64; wv++) s += red[wv][threadIdx.x];
+        partials[((size_t)k * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double *__restrict__ partials,
+                                                                   int tiles, double *out) {
+    const int k = blockIdx.x;
+    __shared__ double red[kThreads][4];
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = threadIdx.x; t < tiles; t += kThreads)
+#pragma unroll
+        for (int c = 0; c < 4; c++) s[c] += partials[((size_t)k * tiles + t) * 4 + c];
+#pragma unroll
+    for (int c = 0; c < 4; c++) red[threadIdx.x][c] = s[c];
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+#pragma unroll
+            for (int c = 0; c < 4; c++) red[threadIdx.x][c] += red[threadIdx.x + h][c];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[(size_t)k * 4 + threadIdx.x] = red[0][threadIdx.x];
+}
+
+__global__ void fill_zero_state_c64_kernel(float2 *psi, uint32_t N) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        psi[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
+}
+
+__global__ void cnot_c64_kernel(float2 *base, size_t stride, uint32_t quarter, int c, int t) {
+    float2 *psi = base + (size_t)blockIdx.y * stride;
+    const int lo = min(c, t), hi = max(c, t);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < quarter; x += gridDim.x * blockDim.x) {
+        const uint32_t b0 = insert_zero(insert_zero(x, lo), hi);
+        const uint32_t i = b0 | (1u << c), j = i | (1u << t);
+        const float2 a = psi[i];
+        psi[i] = psi[j];
+        psi[j] = a;
+    }
+}
+
+__global__ void swap_c64_kernel(float2 *base, size_t stride, uint32_t quarter, int qa, int qb) {
+    float2 *psi = base + (size_t)blockIdx.y * stride;
+    const int lo = min(qa, qb), hi = max(qa, qb);
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < quarter; x += gridDim.x * blockDim.x) {
+        const uint32_t b0 = insert_zero(insert_zero(x, lo), hi);
+        const uint32_t i = b0 | (1u << qa), j = b0 | (1u << qb);
+        const float2 a = psi[i];
+        psi[i] = psi[j];
+        psi[j] = a;
+    }
+}
+
+__global__ void c64_to_c128_kernel(const float2 *src, double2 *dst, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = make_double2(src[i].x, src[i].y);
+}
+
+unsigned grid_for(size_t work, unsigned threads = 256, unsigned cap = 8192) {
+    size_t g = (work + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qgtl_hip_fill_zero_state_c64(void *psi, int n) {
+    const uint32_t N = 1u << n;
+    hipLaunchKernelGGL(fill_zero_state_c64_kernel, dim3(grid_for(N)), dim3(256), 0, qgtl::stream(),
+                       (float2 *)psi, N);
+    QGTL_LAUNCH_CHECK("fill_zero_state_c64");
+    return 0;
+}
+
+int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
+                      int nstates, int tile_variant) {
+    (void)tile_variant;
+    if (desc->m < 1 || desc->m > QGTL_TILE_BITS_MAX || desc->nlow < 1 || desc->nlow > desc->m ||
+        desc->m - desc->nlow > kHiTableBits || desc->ngates > desc->m || desc->nperm > QGTL_PERM_MAX ||
+        desc->m + desc->nrest != desc->n || nstates < 1)
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: inconsistent pass descriptor");
+    if (desc->store == QGTL_STORE_RING && (desc->nlow != desc->m || desc->m < 2 || desc->n <= desc->m))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: ring store needs the low tile of a wider state");
+    if (desc->store == QGTL_STORE_PERM && desc->m != desc->n)
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: permutation store needs the whole state in one tile");
+    if ((desc->store == QGTL_STORE_RING || desc->store == QGTL_STORE_PERM) && src == dst && desc->store == QGTL_STORE_RING)
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: ring store cannot run in place");
+    const dim3 grid(1u << desc->nrest, (unsigned)nstates);
+    hipLaunchKernelGGL(pass_c64_kernel<QGTL_TILE_BITS_MAX>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                       (const float2 *)src, (float2 *)dst, stride);
+    QGTL_LAUNCH_CHECK("pass_c64");
+    return 0;
+}
+
+int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
+                        double *partials, int *tiles_out) {
+    if (desc->K < 1 || desc->K > QGTL_INJECT_MAX || desc->n < 1 || desc->n > QGTL_MAX_QUBITS)
+        return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: bad descriptor");
+    const uint32_t N = 1u << desc->n;
+    const unsigned tiles = (N + kInjectTile - 1) / kInjectTile;
+    *tiles_out = (int)tiles;
+    hipLaunchKernelGGL(inject_c64_kernel, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0, qgtl::stream(),
+                       *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
+    QGTL_LAUNCH_CHECK("inject_c64");
+    return 0;
+}
+
+int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, double *out) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)K), dim3(kThreads), 0, qgtl::stream(),
+                       partials, tiles, out);
+    QGTL_LAUNCH_CHECK("reduce_partials");
+    return 0;
+}
+
+int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t) {
+    if (n < 2) return qgtl::set_error_msg(-118, "cnot needs two qubits");
+    const uint32_t quarter = 1u << (n - 2);
+    hipLaunchKernelGGL(cnot_c64_kernel, dim3(grid_for(quarter), (unsigned)nstates), dim3(256), 0,
+                       qgtl::stream(), (float2 *)base, stride, quarter, c, t);
+    QGTL_LAUNCH_CHECK("cnot_c64");
+    return 0;
+}
+
+int qgtl_hip_swap_c64(void *base, size_t stride, int nstates, int n, int a, int b) {
+    if (n < 2) return qgtl::set_error_msg(-118, "swap needs two qubits");
+    const uint32_t quarter = 1u << (n - 2);
+    hipLaunchKernelGGL(swap_c64_kernel, dim3(grid_for(quarter), (unsigned)nstates), dim3(256), 0,
+                       qgtl::stream(), (float2 *)base, stride, quarter, a, b);
+    QGTL_LAUNCH_CHECK("swap_c64");
+    return 0;
+}
+
+int qgtl_hip_c64_to_c128(const void *src, void *dst, size_t count) {
+    hipLaunchKernelGGL(c64_to_c128_kernel, dim3(grid_for(count)), dim3(256), 0, qgtl::stream(),
+                       (const float2 *)src, (double2 *)dst, count);
+    QGTL_LAUNCH_CHECK("c64_to_c128");
+    return 0;
+}
+
+}  // extern "C"

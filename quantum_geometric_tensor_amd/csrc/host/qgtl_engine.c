@@ -1,0 +1,702 @@
+/*
+ * qgtl_engine.c -- runs a compiled plan on the GPU through the thin HIP ABI.
+ *
+ * Mixed mode (complex64 sweeps, MFMA split-BF16x3 Gram, FP64 reductions):
+ *   for each block b: apply b to the primal; create the derivative states of
+ *   b's parameters from the block output (inject); carry them through the
+ *   later blocks; then G = D^H D on MFMA and Q = normalised G - a a^H in FP64.
+ *   When every pass is in place (states of <= 2^13 amplitudes) the derivative
+ *   states advance breadth-first, one launch per pass for all of them; wider
+ *   states advance depth-first in chunks, so a chunk's working set stays in
+ *   the 256 MB Infinity Cache between passes.
+ * Plain FP64 mode: one complex128 kernel per gate, derivative states one at a
+ *   time (gate insertion of -i/2 G), tiled FP64 Gram, FP64 correction.
+ *
+ * QGTL counterpart of the whole: compute_quantum_geometric_tensor looped by
+ * geometric_compute_full_qgt (src/quantum_geometric/core/quantum_geometric_curvature.c:289-320).
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "qgtl_internal.h"
+
+/* ------------------------------------------------------------------ errors */
+static _Thread_local char g_err[512] = "";
+
+int qgtl_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *qgtl_last_error(void) { return g_err; }
+
+static int hip_fail(int rc, const char *where)
+{
+    return qgtl_set_error(rc, "%s: %s", where, qgtl_hip_last_error());
+}
+
+#define HIPCALL(call)                                          \
+    do {                                                       \
+        int _rc = (call);                                      \
+        if (_rc) return hip_fail(_rc, #call);                  \
+    } while (0)
+
+/* ------------------------------------------------------------------ init */
+static int g_inited = 0;
+static int g_profiling = 0;
+
+int qgtl_init(int device)
+{
+    int rc = qgtl_hip_init(device);
+    if (rc) return hip_fail(rc, "qgtl_init");
+    g_inited = 1;
+    return 0;
+}
+
+static int ensure_init(void)
+{
+    if (g_inited) return 0;
+    const char *e = getenv("QGTL_DEVICE");
+    return qgtl_init(e ? atoi(e) : 0);
+}
+
+int qgtl_device_count(void) { return qgtl_hip_device_count(); }
+
+void qgtl_set_profiling(int enable) { g_profiling = enable ? 1 : 0; }
+
+/* ------------------------------------------------------------- profiling */
+enum { K_SWEEP = 0, K_INJECT, K_GRAM_MAIN, K_GRAM_AUX, K_FINAL, K_OTHER, K_KINDS };
+
+typedef struct {
+    qgtl_hip_event_t *ev;
+    int nev, cap, used;
+    int *kind;          /* per pair */
+    int npairs, pcap;
+} prof_t;
+
+static prof_t g_prof;
+
+static int prof_event(qgtl_hip_event_t *out)
+{
+    if (g_prof.used == g_prof.nev) {
+        if (g_prof.nev == g_prof.cap) {
+            int nc = g_prof.cap ? 2 * g_prof.cap : 1024;
+            qgtl_hip_event_t *ne = (qgtl_hip_event_t *)realloc(g_prof.ev, sizeof(*ne) * (size_t)nc);
+            if (!ne) return QGT_ERROR_NO_MEMORY;
+            g_prof.ev = ne;
+            g_prof.cap = nc;
+        }
+        HIPCALL(qgtl_hip_event_create(&g_prof.ev[g_prof.nev]));
+        g_prof.nev++;
+    }
+    *out = g_prof.ev[g_prof.used++];
+    return 0;
+}
+
+static int prof_start(void)
+{
+    if (!g_profiling) return 0;
+    qgtl_hip_event_t e;
+    int rc = prof_event(&e);
+    if (rc) return rc;
+    HIPCALL(qgtl_hip_event_record(e));
+    return 0;
+}
+
+static int prof_stop(int kind)
+{
+    if (!g_profiling) return 0;
+    qgtl_hip_event_t e;
+    int rc = prof_event(&e);
+    if (rc) return rc;
+    HIPCALL(qgtl_hip_event_record(e));
+    if (g_prof.npairs == g_prof.pcap) {
+        int nc = g_prof.pcap ? 2 * g_prof.pcap : 512;
+        int *nk = (int *)realloc(g_prof.kind, sizeof(int) * (size_t)nc);
+        if (!nk) return QGT_ERROR_NO_MEMORY;
+        g_prof.kind = nk;
+        g_prof.pcap = nc;
+    }
+    g_prof.kind[g_prof.npairs++] = kind;
+    return 0;
+}
+
+static void prof_reset(void)
+{
+    g_prof.used = 0;
+    g_prof.npairs = 0;
+}
+
+/* After a stream sync: fold event pairs into stats. */
+static int prof_collect(qgtl_stats_t *st)
+{
+    if (!g_profiling || !st) { prof_reset(); return 0; }
+    for (int i = 0; i < g_prof.npairs; i++) {
+        float ms = 0.f;
+        HIPCALL(qgtl_hip_event_elapsed_ms(g_prof.ev[2 * i], g_prof.ev[2 * i + 1], &ms));
+        switch (g_prof.kind[i]) {
+        case K_SWEEP: st->sweep_ms += ms; break;
+        case K_INJECT: st->inject_ms += ms; break;
+        case K_GRAM_MAIN: st->gram_ms += ms; st->gram_kernel_ms += ms; break;
+        case K_GRAM_AUX: st->gram_ms += ms; break;
+        case K_FINAL: st->finalize_ms += ms; break;
+        default: st->other_ms += ms; break;
+        }
+    }
+    prof_reset();
+    return 0;
+}
+
+#define LAUNCH(kind, call)                                     \
+    do {                                                       \
+        int _rc = prof_start();                                \
+        if (_rc) return _rc;                                   \
+        _rc = (call);                                          \
+        if (_rc) return hip_fail(_rc, #call);                  \
+        _rc = prof_stop(kind);                                 \
+        if (_rc) return _rc;                                   \
+        if (st) st->kernel_launches++;                         \
+    } while (0)
+
+static double now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* ------------------------------------------------------------------ HEA */
+int qgtl_hea_num_ops(int n, int layers) { return layers * (2 * n + (n >= 2 ? n : 0)); }
+
+int qgtl_hea_ops(int n, int layers, const double *theta, qgtl_op_t *ops)
+{
+    int k = 0;
+    for (int l = 0; l < layers; l++) {
+        for (int q = 0; q < n; q++) {
+            const int p = 2 * n * l + q;
+            qgtl_op_t op = {GATE_TYPE_RY, q, -1, p, theta[p]};
+            ops[k++] = op;
+        }
+        for (int q = 0; q < n; q++) {
+            const int p = 2 * n * l + n + q;
+            qgtl_op_t op = {GATE_TYPE_RZ, q, -1, p, theta[p]};
+            ops[k++] = op;
+        }
+        if (n >= 2)
+            for (int q = 0; q < n; q++) {
+                qgtl_op_t op = {GATE_TYPE_CNOT, q, (q + 1) % n, -1, 0.0};
+                ops[k++] = op;
+            }
+    }
+    return k;
+}
+
+int qgtl_num_params(int nops, const qgtl_op_t *ops)
+{
+    int P = 0;
+    for (int i = 0; i < nops; i++)
+        if (ops[i].param >= 0) P++;
+    return P;
+}
+
+/* ------------------------------------------------------------------ plans */
+static void plan_free_device(struct qgtl_plan *p)
+{
+    qgtl_hip_free(p->psi[0]);
+    qgtl_hip_free(p->psi[1]);
+    qgtl_hip_free(p->D);
+    qgtl_hip_free(p->tmp);
+    qgtl_hip_free(p->inj_partials);
+    qgtl_hip_free(p->inj_sums);
+    qgtl_hip_free(p->gram_partials);
+    qgtl_hip_free(p->C);
+    qgtl_hip_free(p->Qdev);
+    qgtl_hip_free(p->plain_partials);
+    p->psi[0] = p->psi[1] = p->D = p->tmp = NULL;
+    p->inj_partials = p->inj_sums = p->C = p->Qdev = p->plain_partials = NULL;
+    p->gram_partials = NULL;
+}
+
+qgtl_plan_t *qgtl_plan_create(int n, int nops, const qgtl_op_t *ops, int mode, const qgtl_tune_t *tune)
+{
+    if (mode != QGTL_MODE_MIXED && mode != QGTL_MODE_FP64_PLAIN) {
+        qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "unknown mode %d", mode);
+        return NULL;
+    }
+    if (nops < 0 || (nops > 0 && !ops)) {
+        qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null gate list");
+        return NULL;
+    }
+    struct qgtl_plan *p = (struct qgtl_plan *)calloc(1, sizeof *p);
+    if (!p) { qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory"); return NULL; }
+    p->n = n;
+    p->mode = mode;
+    p->nops = nops;
+    p->ops = (qgtl_op_t *)malloc(sizeof(qgtl_op_t) * (size_t)(nops > 0 ? nops : 1));
+    if (!p->ops) { free(p); qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory"); return NULL; }
+    if (nops) memcpy(p->ops, ops, sizeof(qgtl_op_t) * (size_t)nops);
+    if (tune) p->tune = *tune;
+    if (qgtl_plan_build(p) != 0) {
+        free(p->ops); free(p->blocks); free(p->params); free(p->slot_param); free(p);
+        return NULL;
+    }
+    p->N = (size_t)1 << n;
+    p->structure_hash = qgtl_structure_hash(n, nops, ops, mode);
+    return p;
+}
+
+int qgtl_plan_num_params(const qgtl_plan_t *p) { return p ? p->P : -1; }
+
+int qgtl_plan_set_angles(qgtl_plan_t *p, int nops, const double *angles)
+{
+    if (!p || !angles || nops != p->nops) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "angle count mismatch");
+    for (int i = 0; i < nops; i++) p->ops[i].angle = angles[i];
+    return qgtl_plan_build(p);
+}
+
+void qgtl_plan_destroy(qgtl_plan_t *p)
+{
+    if (!p) return;
+    plan_free_device(p);
+    free(p->ops);
+    free(p->blocks);
+    free(p->params);
+    free(p->slot_param);
+    free(p);
+}
+
+static int any_flip(const struct qgtl_plan *p)
+{
+    for (int b = 0; b < p->nblocks; b++)
+        for (int i = 0; i < p->blocks[b].npasses; i++)
+            if (p->blocks[b].pass_flips[i]) return 1;
+    return 0;
+}
+
+static int chunk_states(const struct qgtl_plan *p)
+{
+    if (p->tune.chunk_states > 0) return p->tune.chunk_states < QGTL_INJECT_MAX ? p->tune.chunk_states : QGTL_INJECT_MAX;
+    size_t c = ((size_t)64 << 20) / (p->N * 8);
+    if (c < 1) c = 1;
+    if (c > QGTL_INJECT_MAX) c = QGTL_INJECT_MAX;
+    return (int)c;
+}
+
+static int gram_chunks(const struct qgtl_plan *p)
+{
+    if (p->tune.gram_chunks > 0) return (p->tune.gram_chunks + 7) / 8 * 8;
+    const int T = qgtl_hip_gram_tiles(p->P);
+    int want = (1024 + T - 1) / T;
+    int maxc = (int)(p->N / 256);
+    if (want > maxc) want = maxc;
+    want = (want + 7) / 8 * 8;
+    if (want < 8) want = 8;
+    return want;
+}
+
+/* Split-BF16x3 needs the per-element 2^-17 representation error to average
+ * over many amplitudes; short states use the exact three-piece split. */
+static int gram_terms(const struct qgtl_plan *p)
+{
+    if (p->tune.gram_variant == 3 || p->tune.gram_variant == 6) return p->tune.gram_variant;
+    return p->n >= QGTL_GRAM_X3_MIN_QUBITS ? 3 : 6;
+}
+
+static int alloc_mixed(struct qgtl_plan *p)
+{
+    if (p->psi[0]) return 0;
+    const size_t N = p->N, P = (size_t)p->P;
+    HIPCALL(qgtl_hip_malloc(&p->psi[0], N * 8));
+    HIPCALL(qgtl_hip_malloc(&p->psi[1], N * 8));
+    if (P > 0) {
+        HIPCALL(qgtl_hip_malloc(&p->D, P * N * 8));
+        if (any_flip(p)) {
+            p->tmp_states = (size_t)chunk_states(p);
+            HIPCALL(qgtl_hip_malloc(&p->tmp, p->tmp_states * N * 8));
+        }
+        const size_t tiles = (N + 2047) / 2048;
+        p->inj_partials_len = (size_t)QGTL_INJECT_MAX * tiles * 4;
+        HIPCALL(qgtl_hip_malloc((void **)&p->inj_partials, p->inj_partials_len * sizeof(double)));
+        HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
+        p->gram_chunks = gram_chunks(p);
+        const size_t T = (size_t)qgtl_hip_gram_tiles(p->P);
+        HIPCALL(qgtl_hip_malloc((void **)&p->gram_partials, (size_t)p->gram_chunks * T * 128 * 128 * sizeof(float)));
+        const size_t nt = (2 * P + 127) / 128, R = nt * 128;
+        HIPCALL(qgtl_hip_malloc((void **)&p->C, R * R * sizeof(double)));
+        HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
+    }
+    return 0;
+}
+
+static int alloc_plain(struct qgtl_plan *p)
+{
+    if (p->psi[0]) return 0;
+    const size_t N = p->N, P = (size_t)p->P;
+    HIPCALL(qgtl_hip_malloc(&p->psi[0], N * 16));
+    if (P > 0) {
+        HIPCALL(qgtl_hip_malloc(&p->D, P * N * 16));
+        p->plain_partials_len = qgtl_hip_gram_f64_partials(p->P, N);
+        const size_t dot = ((N + 4095) / 4096) * P * 4;
+        if (dot > p->plain_partials_len) p->plain_partials_len = dot;
+        HIPCALL(qgtl_hip_malloc((void **)&p->plain_partials, p->plain_partials_len * sizeof(double)));
+        HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
+        HIPCALL(qgtl_hip_malloc((void **)&p->C, P * P * 2 * sizeof(double)));
+        HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
+    }
+    return 0;
+}
+
+/* Apply block b to nstates complex64 states at *cur (stride N); a pass that
+ * flips writes to *alt and swaps the two. */
+static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
+                       qgtl_stats_t *st)
+{
+    for (int i = 0; i < b->npasses; i++) {
+        void *dst = b->pass_flips[i] ? *alt : *cur;
+        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0));
+        if (st) {
+            st->sweep_launches++;
+            st->sweep_state_passes += nstates;
+            st->sweep_bytes += (double)nstates * (double)p->N * 16.0;
+        }
+        if (b->pass_flips[i]) { void *t = *cur; *cur = *alt; *alt = t; }
+    }
+    if (b->mstage_kernels)
+        for (int i = 0; i < b->nperm; i++) {
+            if (b->perm_type[i] == 0)
+                LAUNCH(K_SWEEP, qgtl_hip_cnot_c64(*cur, p->N, nstates, p->n, b->perm_a[i], b->perm_b[i]));
+            else
+                LAUNCH(K_SWEEP, qgtl_hip_swap_c64(*cur, p->N, nstates, p->n, b->perm_a[i], b->perm_b[i]));
+            if (st) {
+                st->sweep_launches++;
+                st->sweep_state_passes += nstates;
+                st->sweep_bytes += (double)nstates * (double)p->N * 16.0;
+            }
+        }
+    return 0;
+}
+
+static int block_flips(const qgtl_block_t *b)
+{
+    int f = 0;
+    for (int i = 0; i < b->npasses; i++) f += b->pass_flips[i];
+    return f;
+}
+
+/* Inject parameters slots [s0, s0+cnt) (all from block b) into dst. */
+static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st)
+{
+    qgtl_inject_desc_t d;
+    memset(&d, 0, sizeof d);
+    d.n = p->n;
+    d.K = cnt;
+    for (int j = 0; j < cnt; j++) {
+        const qgtl_param_t *pp = &p->params[p->slot_param[s0 + j]];
+        d.v[j] = pp->v;
+        d.w[j] = pp->w;
+        for (int e = 0; e < 4; e++) {
+            d.A[j][e].re = (float)pp->A.m[e].re;
+            d.A[j][e].im = (float)pp->A.m[e].im;
+        }
+    }
+    int tiles = 0;
+    LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles));
+    LAUNCH(K_INJECT, qgtl_hip_reduce_partials_f64(p->inj_partials, cnt, tiles, p->inj_sums + 4 * (size_t)s0));
+    if (st) st->sweep_bytes += (double)cnt * (double)p->N * 8.0 + (double)p->N * 8.0;
+    return 0;
+}
+
+static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
+{
+    int rc = alloc_mixed(p);
+    if (rc) return rc;
+    void *cur = p->psi[0], *alt = p->psi[1];
+    LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));
+    const int depth_first = any_flip(p);
+    int slot = 0;
+    for (int bi = 0; bi < p->nblocks; bi++) {
+        const qgtl_block_t *b = &p->blocks[bi];
+        if (!depth_first && slot > 0) {
+            void *dc = p->D, *da = NULL;
+            rc = apply_block(p, b, &dc, &da, slot, st);
+            if (rc) return rc;
+        }
+        rc = apply_block(p, b, &cur, &alt, 1, st);
+        if (rc) return rc;
+        const int K = b->param_end - b->param_begin;
+        if (K == 0) continue;
+        if (!depth_first) {
+            for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
+                const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
+                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st);
+                if (rc) return rc;
+            }
+        } else {
+            int flips_after = 0;
+            for (int b2 = bi + 1; b2 < p->nblocks; b2++) flips_after += block_flips(&p->blocks[b2]);
+            const int C = (int)p->tmp_states;
+            for (int j0 = 0; j0 < K; j0 += C) {
+                const int cnt = (K - j0 < C) ? K - j0 : C;
+                void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
+                void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
+                void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
+                rc = inject(p, slot + j0, cnt, cur, c_cur, st);
+                if (rc) return rc;
+                for (int b2 = bi + 1; b2 < p->nblocks; b2++) {
+                    rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st);
+                    if (rc) return rc;
+                }
+                if (c_cur != dslot) return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: chunk ended off its slot");
+            }
+        }
+        slot += K;
+    }
+    if (p->P == 0) return 0;
+    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p)));
+    LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, p->P, p->gram_chunks, p->C));
+    LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
+    if (st) {
+        st->gram_launches += 2;
+        st->gram_flops += 4.0 * (double)p->N * p->P * (p->P + 1);
+    }
+    return 0;
+}
+
+/* complex128 op application for the plain path */
+static int plain_apply(struct qgtl_plan *p, void *psi, const qgtl_op_t *op, qgtl_stats_t *st)
+{
+    double m8[8];
+    qgtl_mat2 m;
+    qgtl_op_t base = {GATE_TYPE_X, op->q1, -1, -1, 0.0};
+    switch (op->type) {
+    case GATE_TYPE_CNOT:
+    case GATE_TYPE_CZ:
+    case GATE_TYPE_CY:
+        base.type = op->type == GATE_TYPE_CNOT ? GATE_TYPE_X : op->type == GATE_TYPE_CZ ? GATE_TYPE_Z : GATE_TYPE_Y;
+        qgtl_gate_matrix(&base, &m);
+        for (int e = 0; e < 4; e++) { m8[2 * e] = m.m[e].re; m8[2 * e + 1] = m.m[e].im; }
+        LAUNCH(K_SWEEP, qgtl_hip_cgate_c128(psi, p->n, op->q0, op->q1, m8));
+        break;
+    case GATE_TYPE_SWAP:
+        LAUNCH(K_SWEEP, qgtl_hip_swap_c128(psi, p->n, op->q0, op->q1));
+        break;
+    default:
+        if (qgtl_gate_matrix(op, &m) != 0) return qgtl_set_error(QGT_ERROR_NOT_IMPLEMENTED, "gate type %d", op->type);
+        for (int e = 0; e < 4; e++) { m8[2 * e] = m.m[e].re; m8[2 * e + 1] = m.m[e].im; }
+        LAUNCH(K_SWEEP, qgtl_hip_gate1q_c128(psi, p->n, op->q0, m8));
+    }
+    if (st) {
+        st->sweep_launches++;
+        st->sweep_state_passes++;
+        st->sweep_bytes += (double)p->N * 32.0;
+    }
+    return 0;
+}
+
+static int run_plain(struct qgtl_plan *p, qgtl_stats_t *st)
+{
+    int rc = alloc_plain(p);
+    if (rc) return rc;
+    void *psi = p->psi[0];
+    LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c128(psi, p->n));
+    for (int g = 0; g < p->nops; g++) {
+        rc = plain_apply(p, psi, &p->ops[g], st);
+        if (rc) return rc;
+        const int k = p->ops[g].param;
+        if (k < 0) continue;
+        void *d = (char *)p->D + (size_t)k * p->N * 16;
+        LAUNCH(K_OTHER, qgtl_hip_d2d(d, psi, p->N * 16));
+        double gen[8] = {0};
+        switch (p->ops[g].type) {           /* (-i/2) G */
+        case GATE_TYPE_RX: gen[3] = -0.5; gen[5] = -0.5; break;
+        case GATE_TYPE_RY: gen[2] = -0.5; gen[4] = 0.5; break;
+        default: gen[1] = -0.5; gen[7] = 0.5; break;
+        }
+        LAUNCH(K_SWEEP, qgtl_hip_gate1q_c128(d, p->n, p->ops[g].q0, gen));
+        for (int g2 = g + 1; g2 < p->nops; g2++) {
+            rc = plain_apply(p, d, &p->ops[g2], st);
+            if (rc) return rc;
+        }
+    }
+    if (p->P == 0) return 0;
+    LAUNCH(K_GRAM_AUX, qgtl_hip_dotc_c128(p->D, p->P, p->N, psi, p->plain_partials, p->inj_sums));
+    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_f64(p->D, p->P, p->N, p->plain_partials, p->C));
+    LAUNCH(K_FINAL, qgtl_hip_qgt_finalize_plain(p->C, p->inj_sums, p->P, p->Qdev));
+    if (st) {
+        st->gram_launches += 1;
+        st->gram_flops += 4.0 * (double)p->N * p->P * (p->P + 1);
+    }
+    return 0;
+}
+
+int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
+{
+    if (!p) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null plan");
+    int rc = ensure_init();
+    if (rc) return rc;
+    qgtl_stats_t local;
+    qgtl_stats_t *st = stats ? stats : &local;
+    memset(st, 0, sizeof *st);
+    st->P = p->P;
+    st->n = p->n;
+    const double t0 = now_ms();
+    prof_reset();
+    rc = (p->mode == QGTL_MODE_MIXED) ? run_mixed(p, st) : run_plain(p, st);
+    if (rc) { qgtl_hip_sync(); prof_reset(); return rc; }
+    const size_t P = (size_t)p->P;
+    double *Qslot = NULL;
+    if (Q && P > 0) {
+        Qslot = (double *)malloc(P * P * 2 * sizeof(double));
+        if (!Qslot) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+        rc = qgtl_hip_d2h(Qslot, p->Qdev, P * P * 2 * sizeof(double));
+        if (rc) { free(Qslot); return hip_fail(rc, "download Q"); }
+    }
+    rc = qgtl_hip_sync();
+    if (rc) { free(Qslot); return hip_fail(rc, "qgtl_plan_execute"); }
+    st->wall_ms = now_ms() - t0;
+    rc = prof_collect(st);
+    if (rc) { free(Qslot); return rc; }
+    if (Qslot) {
+        if (p->mode == QGTL_MODE_MIXED) {
+            /* slot order (creation order) -> parameter numbers */
+            for (size_t a = 0; a < P; a++)
+                for (size_t b = 0; b < P; b++) {
+                    const size_t ka = (size_t)p->slot_param[a], kb = (size_t)p->slot_param[b];
+                    Q[2 * (ka * P + kb)] = Qslot[2 * (a * P + b)];
+                    Q[2 * (ka * P + kb) + 1] = Qslot[2 * (a * P + b) + 1];
+                }
+        } else {
+            memcpy(Q, Qslot, P * P * 2 * sizeof(double));
+        }
+        free(Qslot);
+    }
+    return 0;
+}
+
+/* --------------------------------------------------------- plan cache */
+#define PLAN_CACHE 4
+static qgtl_plan_t *g_cache[PLAN_CACHE];
+static uint64_t g_cache_tick[PLAN_CACHE];
+static uint64_t g_tick = 0;
+
+static qgtl_plan_t *cached_plan(int n, int nops, const qgtl_op_t *ops, int mode)
+{
+    const uint64_t h = qgtl_structure_hash(n, nops, ops, mode);
+    for (int i = 0; i < PLAN_CACHE; i++) {
+        qgtl_plan_t *p = g_cache[i];
+        if (p && p->structure_hash == h && p->n == n && p->nops == nops && p->mode == mode) {
+            int same = 1;
+            for (int j = 0; j < nops && same; j++)
+                same = p->ops[j].type == ops[j].type && p->ops[j].q0 == ops[j].q0 && p->ops[j].q1 == ops[j].q1 &&
+                       p->ops[j].param == ops[j].param;
+            if (!same) continue;
+            double *ang = (double *)malloc(sizeof(double) * (size_t)(nops > 0 ? nops : 1));
+            if (!ang) return NULL;
+            for (int j = 0; j < nops; j++) ang[j] = ops[j].angle;
+            int rc = qgtl_plan_set_angles(p, nops, ang);
+            free(ang);
+            if (rc) return NULL;
+            g_cache_tick[i] = ++g_tick;
+            return p;
+        }
+    }
+    qgtl_plan_t *p = qgtl_plan_create(n, nops, ops, mode, NULL);
+    if (!p) return NULL;
+    int victim = 0;
+    for (int i = 0; i < PLAN_CACHE; i++) {
+        if (!g_cache[i]) { victim = i; break; }
+        if (g_cache_tick[i] < g_cache_tick[victim]) victim = i;
+    }
+    qgtl_plan_destroy(g_cache[victim]);
+    g_cache[victim] = p;
+    g_cache_tick[victim] = ++g_tick;
+    return p;
+}
+
+void qgtl_release_cache(void)
+{
+    for (int i = 0; i < PLAN_CACHE; i++) {
+        qgtl_plan_destroy(g_cache[i]);
+        g_cache[i] = NULL;
+    }
+}
+
+int qgtl_qgt(int n, int nops, const qgtl_op_t *ops, int mode, double *Q, qgtl_stats_t *stats)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    qgtl_plan_t *p = cached_plan(n, nops, ops, mode);
+    if (!p) return QGT_ERROR_INVALID_ARGUMENT;   /* message already set */
+    return qgtl_plan_execute(p, Q, stats);
+}
+
+int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *psi_out)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    qgtl_plan_t *p = qgtl_plan_create(n, nops, ops, mode, NULL);
+    if (!p) return QGT_ERROR_INVALID_ARGUMENT;
+    qgtl_stats_t *st = NULL;
+    const size_t N = (size_t)1 << n;
+    void *out = NULL;
+    rc = qgtl_hip_malloc(&out, N * 16);
+    if (rc) { qgtl_plan_destroy(p); return hip_fail(rc, "statevector buffer"); }
+    if (mode == QGTL_MODE_MIXED) {
+        rc = qgtl_hip_malloc(&p->psi[0], N * 8);
+        if (!rc) rc = qgtl_hip_malloc(&p->psi[1], N * 8);
+        if (rc) { qgtl_hip_free(out); qgtl_plan_destroy(p); return hip_fail(rc, "statevector buffer"); }
+        void *cur = p->psi[0], *alt = p->psi[1];
+        rc = qgtl_hip_fill_zero_state_c64(cur, n);
+        for (int b = 0; !rc && b < p->nblocks; b++) rc = apply_block(p, &p->blocks[b], &cur, &alt, 1, st);
+        if (!rc) rc = qgtl_hip_c64_to_c128(cur, out, N);
+    } else {
+        rc = qgtl_hip_fill_zero_state_c128(out, n);
+        for (int g = 0; !rc && g < nops; g++) rc = plain_apply(p, out, &ops[g], st);
+    }
+    if (!rc) rc = qgtl_hip_d2h(psi_out, out, N * 16);
+    if (!rc) rc = qgtl_hip_sync();
+    qgtl_hip_free(out);
+    qgtl_plan_destroy(p);
+    prof_reset();
+    return rc ? (rc < -100 ? hip_fail(rc, "qgtl_statevector") : rc) : 0;
+}
+
+/* Diagnostics: run the MFMA lane-layout probe and copy its 32 x 32 output
+ * to host_out (see mfma_layout_probe_kernel for the expected values). */
+
+
+int qgtl_debug_mfma_layout(float *host_out)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    void *d = NULL;
+    HIPCALL(qgtl_hip_malloc(&d, 32 * 32 * sizeof(float)));
+    rc = qgtl_hip_debug_mfma_layout((float *)d);
+    if (!rc) rc = qgtl_hip_d2h(host_out, d, 32 * 32 * sizeof(float));
+    if (!rc) rc = qgtl_hip_sync();
+    qgtl_hip_free(d);
+    return rc ? hip_fail(rc, "qgtl_debug_mfma_layout") : 0;
+}
+
+/* Diagnostics: after qgtl_plan_execute in mixed mode, copy the FP64 Gram
+ * C = Z Z^T (R x R, R = 128 ceil(2P/128), slot order) and the injection sums
+ * (P x 4: Re/Im <d|psi>, |d|^2, |psi|^2, slot order). */
+int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int *R_out)
+{
+    if (!p || p->mode != QGTL_MODE_MIXED || !p->C) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "no mixed-mode result");
+    const size_t R = (size_t)((2 * p->P + 127) / 128) * 128;
+    if (R_out) *R_out = (int)R;
+    if (C_out) HIPCALL(qgtl_hip_d2h(C_out, p->C, R * R * sizeof(double)));
+    if (inj_out) HIPCALL(qgtl_hip_d2h(inj_out, p->inj_sums, (size_t)p->P * 4 * sizeof(double)));
+    HIPCALL(qgtl_hip_sync());
+    return 0;
+}

@@ -1,0 +1,89 @@
+/*
+ * qgtl_internal.h -- host-side structures of libqgtl_mi355x (C11).
+ *
+ * A circuit is compiled into blocks.  A block is a U-layer (one 2x2 matrix
+ * per qubit, the product of that qubit's single-qubit gates, FP64) followed
+ * by an M-stage (a list of CNOT/SWAP gates, a GF(2)-linear permutation of
+ * basis indices).  Derivative states of the block's parameters are created
+ * after the block from the block's output state, and carried through the
+ * later blocks by LDS-tiled sweep passes.
+ */
+#ifndef QGTL_INTERNAL_H
+#define QGTL_INTERNAL_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#include "qgtl_hip_abi.h"
+#include "qgtl_mi355x.h"
+#include "qgtl_types.h"
+
+typedef struct { double re, im; } qgtl_cd;
+typedef struct { qgtl_cd m[4]; } qgtl_mat2;       /* row-major 2x2 */
+
+#define QGTL_MAX_PASSES_PER_BLOCK 6
+#define QGTL_GRAM_X3_MIN_QUBITS 14   /* smallest n that uses split-BF16x3 by default */
+
+typedef struct {
+    int has_u[QGTL_MAX_QUBITS];
+    qgtl_mat2 u[QGTL_MAX_QUBITS];
+    int nperm;
+    int8_t perm_type[QGTL_PERM_MAX];     /* 0 CNOT(a -> b), 1 SWAP(a, b) */
+    int8_t perm_a[QGTL_PERM_MAX];
+    int8_t perm_b[QGTL_PERM_MAX];
+    int is_ring;                         /* M-stage is CNOT(i -> i+1 mod n), i = 0..n-1 */
+    int param_begin, param_end;          /* parameters created in this block */
+    /* device passes that apply this block */
+    int npasses;
+    qgtl_pass_desc_t pass[QGTL_MAX_PASSES_PER_BLOCK];
+    int pass_flips[QGTL_MAX_PASSES_PER_BLOCK];  /* 1: output goes to the other buffer */
+    int mstage_kernels;                  /* 1: M-stage applied by per-gate kernels after the passes */
+} qgtl_block_t;
+
+typedef struct {
+    int block;
+    int qubit;
+    int op_index;
+    qgtl_mat2 A;                         /* d_k (pre-M) = A psi_block (pre-M) */
+    uint32_t v, w;                       /* injection masks after the M-stage */
+} qgtl_param_t;
+
+struct qgtl_plan {
+    int n;
+    int mode;
+    int nops;
+    int P;
+    qgtl_op_t *ops;
+    int nblocks;
+    qgtl_block_t *blocks;
+    qgtl_param_t *params;                /* indexed by parameter number */
+    int *slot_param;                     /* D slot -> parameter number */
+    qgtl_tune_t tune;
+    /* device buffers */
+    size_t N;
+    void *psi[2];                        /* primal (c64 or c128) */
+    void *D;                             /* P states */
+    void *tmp;                           /* chunk ping-pong (c64) */
+    size_t tmp_states;
+    double *inj_partials;
+    size_t inj_partials_len;
+    double *inj_sums;                    /* P x 4 */
+    float *gram_partials;
+    int gram_chunks;
+    double *C;                           /* mixed: R x R;  plain: G (P x P complex) */
+    double *Qdev;                        /* P x P complex */
+    double *plain_partials;
+    size_t plain_partials_len;
+    uint64_t structure_hash;
+};
+
+/* plan construction (qgtl_plan.c) */
+int qgtl_plan_build(struct qgtl_plan *plan);
+int qgtl_gate_matrix(const qgtl_op_t *op, qgtl_mat2 *m);
+int qgtl_is_param_gate(int type);
+uint64_t qgtl_structure_hash(int n, int nops, const qgtl_op_t *ops, int mode);
+
+/* errors (qgtl_engine.c) */
+int qgtl_set_error(int code, const char *fmt, ...);
+
+#endif

@@ -1,0 +1,407 @@
+/*
+ * qgtl_plan.c -- compile a flat gate list into blocks and device passes.
+ *
+ * Gate conventions follow QGTL: RX = [[c, -is], [-is, c]], RY = [[c, -s], [s, c]],
+ * RZ = diag(e^{-i t/2}, e^{i t/2}) with c = cos(t/2), s = sin(t/2)
+ * (src/quantum_geometric/core/quantum_gate_operations.c:11-47); qubit q is
+ * bit 1 << q (src/quantum_geometric/hybrid/quantum_classical_algorithms.c:174-192);
+ * CNOT flips the target where the control bit is set (:195-210).
+ *
+ * Block rule: a single-qubit gate on qubit q joins the current block's
+ * U-layer unless the block's M-stage has already touched q; a CNOT or SWAP
+ * joins the M-stage.  CZ and CY are rewritten as H.CNOT.H and S.CNOT.Sdg.
+ *
+ * Derivative of parameter k carried by gate g on qubit q: with the block's
+ * U_q = S g R (S = gates after g in the block, R = before), the derivative
+ * state before the M-stage is S (-i/2 G) g R = A_k U_q, A_k = S (-i/2 G) S^H,
+ * applied to the block's output.  Through the M-stage permutation pi,
+ * d_k[y] = A[s][s] psi[y] + A[s][1-s] psi[y ^ pi(e_q)], s = (pi^-1 y)_q.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "qgtl_internal.h"
+
+#define CMIN_LOW_BITS 5   /* contiguous low bits kept in every high-qubit pass (256-B runs) */
+
+static qgtl_cd cd(double re, double im) { qgtl_cd z = {re, im}; return z; }
+static qgtl_cd cmul(qgtl_cd a, qgtl_cd b) { return cd(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static qgtl_cd cadd(qgtl_cd a, qgtl_cd b) { return cd(a.re + b.re, a.im + b.im); }
+
+static qgtl_mat2 mat_mul(const qgtl_mat2 *a, const qgtl_mat2 *b)
+{
+    qgtl_mat2 r;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            r.m[2 * i + j] = cadd(cmul(a->m[2 * i], b->m[j]), cmul(a->m[2 * i + 1], b->m[2 + j]));
+    return r;
+}
+
+static qgtl_mat2 mat_adj(const qgtl_mat2 *a)
+{
+    qgtl_mat2 r;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) r.m[2 * i + j] = cd(a->m[2 * j + i].re, -a->m[2 * j + i].im);
+    return r;
+}
+
+static qgtl_mat2 mat_identity(void)
+{
+    qgtl_mat2 r;
+    memset(&r, 0, sizeof r);
+    r.m[0].re = 1.0;
+    r.m[3].re = 1.0;
+    return r;
+}
+
+static void set4(qgtl_mat2 *m, qgtl_cd a, qgtl_cd b, qgtl_cd c, qgtl_cd d)
+{
+    m->m[0] = a; m->m[1] = b; m->m[2] = c; m->m[3] = d;
+}
+
+int qgtl_is_param_gate(int type)
+{
+    return type == GATE_TYPE_RX || type == GATE_TYPE_RY || type == GATE_TYPE_RZ;
+}
+
+int qgtl_gate_matrix(const qgtl_op_t *op, qgtl_mat2 *m)
+{
+    const double r = M_SQRT1_2;
+    const double c = cos(op->angle / 2.0), s = sin(op->angle / 2.0);
+    const qgtl_cd z = cd(0, 0), one = cd(1, 0);
+    switch (op->type) {
+    case GATE_TYPE_I: set4(m, one, z, z, one); return 0;
+    case GATE_TYPE_X: set4(m, z, one, one, z); return 0;
+    case GATE_TYPE_Y: set4(m, z, cd(0, -1), cd(0, 1), z); return 0;
+    case GATE_TYPE_Z: set4(m, one, z, z, cd(-1, 0)); return 0;
+    case GATE_TYPE_H: set4(m, cd(r, 0), cd(r, 0), cd(r, 0), cd(-r, 0)); return 0;
+    case GATE_TYPE_S: set4(m, one, z, z, cd(0, 1)); return 0;
+    case GATE_TYPE_SDG: set4(m, one, z, z, cd(0, -1)); return 0;
+    case GATE_TYPE_T: set4(m, one, z, z, cd(r, r)); return 0;
+    case GATE_TYPE_TDG: set4(m, one, z, z, cd(r, -r)); return 0;
+    case GATE_TYPE_SX: set4(m, cd(0.5, 0.5), cd(0.5, -0.5), cd(0.5, -0.5), cd(0.5, 0.5)); return 0;
+    case GATE_TYPE_RX: set4(m, cd(c, 0), cd(0, -s), cd(0, -s), cd(c, 0)); return 0;
+    case GATE_TYPE_RY: set4(m, cd(c, 0), cd(-s, 0), cd(s, 0), cd(c, 0)); return 0;
+    case GATE_TYPE_RZ: set4(m, cd(c, -s), z, z, cd(c, s)); return 0;
+    case GATE_TYPE_U1:
+    case GATE_TYPE_PHASE: set4(m, one, z, z, cd(cos(op->angle), sin(op->angle))); return 0;
+    default: return -1;
+    }
+}
+
+/* (-i/2) G for the generator G of RX/RY/RZ. */
+static qgtl_mat2 generator(int type)
+{
+    qgtl_mat2 g;
+    const qgtl_cd z = cd(0, 0);
+    if (type == GATE_TYPE_RX) set4(&g, z, cd(0, -0.5), cd(0, -0.5), z);
+    else if (type == GATE_TYPE_RY) set4(&g, z, cd(-0.5, 0), cd(0.5, 0), z);
+    else set4(&g, cd(0, -0.5), z, z, cd(0, 0.5));
+    return g;
+}
+
+static int is_single_qubit(int type)
+{
+    qgtl_op_t probe = {type, 0, -1, -1, 0.0};
+    qgtl_mat2 m;
+    return qgtl_gate_matrix(&probe, &m) == 0;
+}
+
+uint64_t qgtl_structure_hash(int n, int nops, const qgtl_op_t *ops, int mode)
+{
+    uint64_t h = 1469598103934665603ull;
+#define MIX(x) do { h ^= (uint64_t)(uint32_t)(x); h *= 1099511628211ull; } while (0)
+    MIX(n); MIX(mode); MIX(nops);
+    for (int i = 0; i < nops; i++) { MIX(ops[i].type); MIX(ops[i].q0); MIX(ops[i].q1); MIX(ops[i].param); }
+#undef MIX
+    return h;
+}
+
+static uint32_t perm_apply(const qgtl_block_t *b, uint32_t x)
+{
+    for (int i = 0; i < b->nperm; i++) {
+        const uint32_t ab = (x >> b->perm_a[i]) & 1u, bb = (x >> b->perm_b[i]) & 1u;
+        if (b->perm_type[i] == 0) x ^= ab << b->perm_b[i];
+        else x ^= (ab ^ bb) * ((1u << b->perm_a[i]) | (1u << b->perm_b[i]));
+    }
+    return x;
+}
+
+static uint32_t perm_apply_inverse(const qgtl_block_t *b, uint32_t y)
+{
+    for (int i = b->nperm - 1; i >= 0; i--) {
+        const uint32_t ab = (y >> b->perm_a[i]) & 1u, bb = (y >> b->perm_b[i]) & 1u;
+        if (b->perm_type[i] == 0) y ^= ab << b->perm_b[i];
+        else y ^= (ab ^ bb) * ((1u << b->perm_a[i]) | (1u << b->perm_b[i]));
+    }
+    return y;
+}
+
+static void to_c64(const qgtl_mat2 *m, qgtl_c64_t out[4])
+{
+    for (int e = 0; e < 4; e++) {
+        out[e].re = (float)m->m[e].re;
+        out[e].im = (float)m->m[e].im;
+    }
+}
+
+static void build_passes(qgtl_block_t *b, int n)
+{
+    const int TB = QGTL_TILE_BITS_MAX;
+    b->npasses = 0;
+    b->mstage_kernels = 0;
+    if (n <= TB) {
+        qgtl_pass_desc_t *p = &b->pass[0];
+        memset(p, 0, sizeof *p);
+        p->n = n; p->m = n; p->nlow = n; p->nrest = 0;
+        for (int q = 0; q < n; q++)
+            if (b->has_u[q]) {
+                p->gate_bit[p->ngates] = (int8_t)q;
+                to_c64(&b->u[q], p->mat[p->ngates]);
+                p->ngates++;
+            }
+        p->store = QGTL_STORE_INPLACE;
+        if (b->nperm > 0) {
+            if (b->nperm <= QGTL_PERM_MAX) {
+                p->store = QGTL_STORE_PERM;
+                p->nperm = b->nperm;
+                memcpy(p->perm_type, b->perm_type, (size_t)b->nperm);
+                memcpy(p->perm_a, b->perm_a, (size_t)b->nperm);
+                memcpy(p->perm_b, b->perm_b, (size_t)b->nperm);
+            } else {
+                b->mstage_kernels = 1;
+            }
+        }
+        if (p->ngates > 0 || p->store != QGTL_STORE_INPLACE) {
+            b->pass_flips[0] = 0;
+            b->npasses = 1;
+        }
+        return;
+    }
+    /* high qubits, in groups that leave CMIN_LOW_BITS contiguous low bits */
+    int hq[QGTL_MAX_QUBITS], nh = 0;
+    for (int q = TB; q < n; q++)
+        if (b->has_u[q]) hq[nh++] = q;
+    const int group = TB - CMIN_LOW_BITS;
+    for (int g0 = 0; g0 < nh; g0 += group) {
+        const int len = (nh - g0 < group) ? nh - g0 : group;
+        const int c = TB - len;
+        qgtl_pass_desc_t *p = &b->pass[b->npasses];
+        memset(p, 0, sizeof *p);
+        p->n = n; p->m = TB; p->nlow = c;
+        int in_tile[QGTL_MAX_QUBITS] = {0};
+        for (int i = 0; i < c; i++) in_tile[i] = 1;
+        for (int i = 0; i < len; i++) {
+            p->hi_pos[i] = (int8_t)hq[g0 + i];
+            in_tile[hq[g0 + i]] = 1;
+            p->gate_bit[i] = (int8_t)(c + i);
+            to_c64(&b->u[hq[g0 + i]], p->mat[i]);
+        }
+        p->ngates = len;
+        for (int q = 0; q < n; q++)
+            if (!in_tile[q]) p->rest_pos[p->nrest++] = (int8_t)q;
+        p->store = QGTL_STORE_INPLACE;
+        b->pass_flips[b->npasses] = 0;
+        b->npasses++;
+    }
+    qgtl_pass_desc_t *p = &b->pass[b->npasses];
+    memset(p, 0, sizeof *p);
+    p->n = n; p->m = TB; p->nlow = TB;
+    for (int q = TB; q < n; q++) p->rest_pos[p->nrest++] = (int8_t)q;
+    for (int q = 0; q < TB; q++)
+        if (b->has_u[q]) {
+            p->gate_bit[p->ngates] = (int8_t)q;
+            to_c64(&b->u[q], p->mat[p->ngates]);
+            p->ngates++;
+        }
+    p->store = b->is_ring ? QGTL_STORE_RING : QGTL_STORE_INPLACE;
+    if (!b->is_ring && b->nperm > 0) b->mstage_kernels = 1;
+    if (p->ngates > 0 || b->is_ring) {
+        b->pass_flips[b->npasses] = b->is_ring;
+        b->npasses++;
+    }
+}
+
+static void detect_ring(qgtl_block_t *b, int n)
+{
+    b->is_ring = 0;
+    if (n < 2 || b->nperm != n) return;
+    for (int i = 0; i < n; i++)
+        if (b->perm_type[i] != 0 || b->perm_a[i] != i || b->perm_b[i] != (i + 1) % n) return;
+    b->is_ring = 1;
+}
+
+typedef struct {
+    int type, q0, q1, param, op_index;
+    double angle;
+} flat_gate_t;
+
+/* Expand CZ / CY into single-qubit gates and CNOTs. */
+static int expand(const struct qgtl_plan *plan, flat_gate_t **out, int *count)
+{
+    flat_gate_t *g = (flat_gate_t *)malloc(sizeof(flat_gate_t) * (size_t)(3 * plan->nops + 1));
+    if (!g) return QGT_ERROR_NO_MEMORY;
+    int k = 0;
+    for (int i = 0; i < plan->nops; i++) {
+        const qgtl_op_t *op = &plan->ops[i];
+        flat_gate_t base = {op->type, op->q0, op->q1, op->param, i, op->angle};
+        if (op->type == GATE_TYPE_CZ) {
+            flat_gate_t h = {GATE_TYPE_H, op->q1, -1, -1, i, 0.0}, cx = {GATE_TYPE_CNOT, op->q0, op->q1, -1, i, 0.0};
+            g[k++] = h; g[k++] = cx; g[k++] = h;
+        } else if (op->type == GATE_TYPE_CY) {
+            flat_gate_t sd = {GATE_TYPE_SDG, op->q1, -1, -1, i, 0.0}, cx = {GATE_TYPE_CNOT, op->q0, op->q1, -1, i, 0.0};
+            flat_gate_t s = {GATE_TYPE_S, op->q1, -1, -1, i, 0.0};
+            g[k++] = sd; g[k++] = cx; g[k++] = s;
+        } else {
+            g[k++] = base;
+        }
+    }
+    *out = g;
+    *count = k;
+    return 0;
+}
+
+int qgtl_plan_build(struct qgtl_plan *plan)
+{
+    const int n = plan->n;
+    if (n < 1 || n > QGTL_MAX_QUBITS)
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qubit count %d outside 1..%d", n, QGTL_MAX_QUBITS);
+    /* validate ops and count parameters */
+    int P = 0;
+    for (int i = 0; i < plan->nops; i++) {
+        const qgtl_op_t *op = &plan->ops[i];
+        if (op->q0 < 0 || op->q0 >= n)
+            return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "gate %d: qubit %d out of range", i, op->q0);
+        const int two = (op->type == GATE_TYPE_CNOT || op->type == GATE_TYPE_CZ || op->type == GATE_TYPE_CY ||
+                         op->type == GATE_TYPE_SWAP);
+        if (two && (op->q1 < 0 || op->q1 >= n || op->q1 == op->q0))
+            return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "gate %d: bad second qubit %d", i, op->q1);
+        if (!two && !is_single_qubit(op->type))
+            return qgtl_set_error(QGT_ERROR_NOT_IMPLEMENTED, "gate %d: gate type %d not supported on this path", i, op->type);
+        if (op->param >= 0) {
+            if (!qgtl_is_param_gate(op->type))
+                return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "gate %d: only RX/RY/RZ carry parameters", i);
+            P++;
+        }
+    }
+    int *seen = (int *)calloc((size_t)(P > 0 ? P : 1), sizeof(int));
+    if (!seen) return QGT_ERROR_NO_MEMORY;
+    for (int i = 0; i < plan->nops; i++) {
+        const int k = plan->ops[i].param;
+        if (k < 0) continue;
+        if (k >= P || seen[k]) {
+            free(seen);
+            return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "parameter indices must be 0..P-1, each once (gate %d)", i);
+        }
+        seen[k] = 1;
+    }
+    free(seen);
+    plan->P = P;
+
+    flat_gate_t *g = NULL;
+    int ng = 0;
+    int rc = expand(plan, &g, &ng);
+    if (rc) return rc;
+
+    free(plan->blocks);
+    free(plan->params);
+    plan->blocks = (qgtl_block_t *)calloc((size_t)(ng + 1), sizeof(qgtl_block_t));
+    plan->params = (qgtl_param_t *)calloc((size_t)(P > 0 ? P : 1), sizeof(qgtl_param_t));
+    qgtl_mat2 *u_at = (qgtl_mat2 *)calloc((size_t)(P > 0 ? P : 1), sizeof(qgtl_mat2));
+    int *order = (int *)calloc((size_t)(P > 0 ? P : 1), sizeof(int));   /* params in creation order */
+    if (!plan->blocks || !plan->params || !u_at || !order) {
+        free(g); free(u_at); free(order);
+        return QGT_ERROR_NO_MEMORY;
+    }
+    int nb = 0, npar = 0;
+    int touched[QGTL_MAX_QUBITS];
+    qgtl_block_t *cur = &plan->blocks[0];
+    for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+    memset(touched, 0, sizeof touched);
+    cur->param_begin = 0;
+    int block_open = 0;
+
+    for (int i = 0; i < ng; i++) {
+        const flat_gate_t *op = &g[i];
+        const int twoq = (op->type == GATE_TYPE_CNOT || op->type == GATE_TYPE_SWAP);
+        if (!twoq && touched[op->q0]) {
+            /* close the block */
+            cur->param_end = npar;
+            nb++;
+            cur = &plan->blocks[nb];
+            for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+            memset(touched, 0, sizeof touched);
+            cur->param_begin = npar;
+            block_open = 0;
+        }
+        block_open = 1;
+        if (twoq) {
+            if (cur->nperm >= QGTL_PERM_MAX) {
+                /* start a fresh block rather than overflow the M-stage list */
+                cur->param_end = npar;
+                nb++;
+                cur = &plan->blocks[nb];
+                for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+                memset(touched, 0, sizeof touched);
+                cur->param_begin = npar;
+            }
+            cur->perm_type[cur->nperm] = (int8_t)(op->type == GATE_TYPE_SWAP ? 1 : 0);
+            cur->perm_a[cur->nperm] = (int8_t)op->q0;
+            cur->perm_b[cur->nperm] = (int8_t)op->q1;
+            cur->nperm++;
+            touched[op->q0] = touched[op->q1] = 1;
+        } else {
+            qgtl_op_t tmp = {op->type, op->q0, -1, -1, op->angle};
+            qgtl_mat2 m;
+            qgtl_gate_matrix(&tmp, &m);
+            cur->u[op->q0] = mat_mul(&m, &cur->u[op->q0]);
+            cur->has_u[op->q0] = 1;
+            if (op->param >= 0) {
+                qgtl_param_t *pp = &plan->params[op->param];
+                pp->block = nb;
+                pp->qubit = op->q0;
+                pp->op_index = op->op_index;
+                u_at[op->param] = cur->u[op->q0];
+                /* remember the generator through A (finished at block close) */
+                pp->A = generator(op->type);
+                order[npar++] = op->param;
+            }
+        }
+    }
+    if (block_open || nb == 0) {
+        cur->param_end = npar;
+        nb++;
+    }
+    plan->nblocks = nb;
+
+    /* finish blocks: ring detection, passes, injection operators */
+    for (int bi = 0; bi < nb; bi++) {
+        qgtl_block_t *b = &plan->blocks[bi];
+        detect_ring(b, n);
+        build_passes(b, n);
+        for (int j = b->param_begin; j < b->param_end; j++) {
+            qgtl_param_t *pp = &plan->params[order[j]];
+            const int q = pp->qubit;
+            qgtl_mat2 at_adj = mat_adj(&u_at[order[j]]);
+            qgtl_mat2 S = mat_mul(&b->u[q], &at_adj);           /* gates after the parameter */
+            qgtl_mat2 Sadj = mat_adj(&S);
+            qgtl_mat2 t = mat_mul(&S, &pp->A);
+            pp->A = mat_mul(&t, &Sadj);
+            pp->v = perm_apply(b, 1u << q);
+            uint32_t w = 0;
+            for (int j2 = 0; j2 < n; j2++)
+                if ((perm_apply_inverse(b, 1u << j2) >> q) & 1u) w |= 1u << j2;
+            pp->w = w;
+        }
+    }
+    /* D slot j holds the j-th parameter created (gate order); slot_param maps
+     * slots back to the caller's parameter numbers. */
+    free(plan->slot_param);
+    plan->slot_param = order;
+    order = NULL;
+    free(order);
+    free(u_at);
+    free(g);
+    return 0;
+}
