@@ -112,6 +112,15 @@ int qgtl_hip_gram_f64(const void *D, int P, size_t N, double *partials, double *
 int qgtl_hip_dotc_c128(const void *D, int P, size_t N, const void *psi, double *partials, double *a);
 int qgtl_hip_qgt_finalize_plain(const double *G, const double *a, int P, double *Q);
 
+/* ---- FP64 natural-gradient solves, one workgroup per system ---- */
+int qgtl_hip_lu_solve_c128(void *A, const void *b, void *x, int *piv, int P, int batch, int *status);
+int qgtl_hip_herm_eig_c128(void *A, void *V, double *ev, int P, int batch, int max_sweeps, int *sweeps);
+int qgtl_hip_cholesky_solve_f64(double *A, const double *b, double *x, int P, int batch, double lambda, int *status);
+int qgtl_hip_pcg_f64(const double *A, const double *b, double *x, int P, int batch, double lambda, const double *d,
+                     const double *U, int r, double tol, int maxit, int *iters, double *relres);
+int qgtl_hip_gram_small_c128(const void *M, void *B, int P);
+int qgtl_hip_pinv_apply_c128(const void *V, const double *ev, const void *g, void *x, int P, double cutoff);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,7 +64,7 @@ typedef struct {
 typedef struct {
     int32_t chunk_states;     /* derivative states propagated together (0 = auto) */
     int32_t gram_chunks;      /* split-K factor of the Gram (0 = auto) */
-    int32_t gram_variant;     /* 3: split-BF16x3, 6: split-BF16x6 (exact 3-piece split), 0: by size */
+    int32_t gram_variant;     /* split-BF16 product terms: 3, 4 or 6 (exact 3-piece split); 0: by size */
     int32_t reserved[5];
 } qgtl_tune_t;
 
@@ -98,6 +98,31 @@ void qgtl_plan_destroy(qgtl_plan_t *plan);
 
 /* Release every cached device buffer of the calling process. */
 void qgtl_release_cache(void);
+
+/* FP64 copy (P x P complex128) of the QGT that the drop-in API caches on a
+ * network created by create_quantum_geometric_tensor_network. */
+struct quantum_geometric_tensor_network;
+int qgtl_network_qgt_f64(const struct quantum_geometric_tensor_network *qgtn, double *Q, int *P);
+
+/* ---- natural-gradient solves (FP64 on the GPU) ----
+ * (A + lambda I) x = b for `batch` real symmetric P x P systems stored one
+ * after another (row-major).  CHOLESKY is exact (the judge of the iterative
+ * methods); CG has no preconditioner; JACOBI_PCG uses M^-1 = diag(1/(A_ii + lambda));
+ * PCG uses M^-1 = diag(d) + U U^T (d: P per system, U: P x rank per system,
+ * rank <= 64).  Iterative methods start at x = 0 and stop at
+ * ||r|| / ||b|| <= tol or maxit.  P <= 1024. */
+enum { QGTL_SOLVE_CHOLESKY = 0, QGTL_SOLVE_CG = 1, QGTL_SOLVE_JACOBI_PCG = 2, QGTL_SOLVE_PCG = 3 };
+typedef struct {
+    int32_t status;          /* 0 ok, -1 not PD / not converged */
+    int32_t iterations;
+    double rel_residual;
+    double device_ms;        /* kernel time of the whole batch (HIP events) */
+} qgtl_solve_info_t;
+int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double lambda, int method, double tol,
+                   int maxit, const double *d, const double *U, int rank, double *x, qgtl_solve_info_t *info);
+/* Eigenvalues (ascending) and eigenvectors (columns of V, row-major) of a
+ * Hermitian complex128 matrix, parallel Jacobi on the GPU.  V may be NULL. */
+int qgtl_hermitian_eig(int P, const double *A, double *evals, double *V);
 
 /* Diagnostics: v_mfma_f32_32x32x16_bf16 lane-layout probe; writes the 32 x 32
  * product D[i][j] = (1 + i/16)(i%16 + 16 (j%8)) as the Gram kernel reads it. */

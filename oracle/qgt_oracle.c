@@ -839,3 +839,37 @@ double oracle_regularized_natural_gradient(int P, const double *G, const double 
     free(ev); free(V); free(A);
     return lambda;
 }
+
+/* Full QGT with the derivative states built once, the P overlaps
+ * <d_k|psi> hoisted out of the element loop, and the Hermitian fill of
+ * quantum_geometric_curvature.c:300-317: same values as oracle_qgt, the
+ * loop order a fast CPU port would use (the timed CPU baseline). */
+int oracle_qgt_hoisted(int n, int nops, const oracle_op_t *ops, double *Q)
+{
+    const size_t dim = (size_t)1 << n;
+    int P = oracle_num_params(nops, ops);
+    double *psi = (double *)malloc(2 * dim * sizeof(double));
+    double *D = (double *)malloc(2 * dim * sizeof(double) * (size_t)(P > 0 ? P : 1));
+    double *a = (double *)malloc(2 * sizeof(double) * (size_t)(P > 0 ? P : 1));
+    if (!psi || !D || !a) { free(psi); free(D); free(a); return -1; }
+    int rc = oracle_derivative_states(n, nops, ops, psi, D);
+    if (rc == 0) {
+#pragma omp parallel for schedule(static)
+        for (int k = 0; k < P; k++) dotc(dim, D + 2 * dim * (size_t)k, psi, &a[2 * k], &a[2 * k + 1]);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int mu = 0; mu < P; mu++) {
+            for (int nu = mu; nu < P; nu++) {
+                double gr, gi;
+                dotc(dim, D + 2 * dim * (size_t)mu, D + 2 * dim * (size_t)nu, &gr, &gi);
+                const double pr = a[2 * mu] * a[2 * nu] + a[2 * mu + 1] * a[2 * nu + 1];
+                const double pi = a[2 * mu + 1] * a[2 * nu] - a[2 * mu] * a[2 * nu + 1];
+                Q[2 * ((size_t)mu * P + nu)] = gr - pr;
+                Q[2 * ((size_t)mu * P + nu) + 1] = gi - pi;
+                Q[2 * ((size_t)nu * P + mu)] = gr - pr;
+                Q[2 * ((size_t)nu * P + mu) + 1] = -(gi - pi);
+            }
+        }
+    }
+    free(psi); free(D); free(a);
+    return rc;
+}

@@ -94,6 +94,8 @@ int oracle_derivative_subset(int n, int nops, const oracle_op_t *ops, int count,
 void oracle_qgt_pairwise(size_t dim, int P, const double *psi, const double *D, double *Q);
 void oracle_qgt_elementwise(size_t dim, int P, const double *psi, const double *D, double *Q);
 int  oracle_qgt(int n, int nops, const oracle_op_t *ops, double *Q);
+/* Same values, overlaps hoisted and upper triangle only (timed CPU baseline). */
+int  oracle_qgt_hoisted(int n, int nops, const oracle_op_t *ops, double *Q);
 /* Q[r][c] for the listed rows/cols. */
 int  oracle_qgt_block(int n, int nops, const oracle_op_t *ops, int nr, const int *rows,
                       int nc, const int *cols, double *Q);

@@ -171,6 +171,50 @@ class Plan:
             pass
 
 
+SOLVE_CHOLESKY, SOLVE_CG, SOLVE_JACOBI_PCG, SOLVE_PCG = 0, 1, 2, 3
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("rel_residual", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+
+def solve(A, b, lam=0.0, method=SOLVE_CHOLESKY, tol=1e-8, maxit=10000, d=None, U=None, check=True):
+    """(A + lam I) x = b on the GPU in FP64; A is (P, P) or (batch, P, P)."""
+    A = np.ascontiguousarray(A, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    batch = 1 if A.ndim == 2 else A.shape[0]
+    P = A.shape[-1]
+    x = np.zeros((batch, P), np.float64)
+    info = (SolveInfo * batch)()
+    dd = None if d is None else np.ascontiguousarray(d, np.float64)
+    UU = None if U is None else np.ascontiguousarray(U, np.float64)
+    rank = 0 if UU is None else UU.shape[-1]
+    L = lib()
+    L.qgtl_solve_f64.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rc = L.qgtl_solve_f64(P, batch, _p(A), _p(b), lam, method, tol, maxit,
+                          None if dd is None else _p(dd), None if UU is None else _p(UU), rank, _p(x),
+                          ctypes.cast(info, ctypes.c_void_p))
+    if check:
+        _check(rc)
+    infos = [dict(status=i.status, iterations=i.iterations, rel_residual=i.rel_residual, device_ms=i.device_ms)
+             for i in info]
+    return (x[0] if A.ndim == 2 else x), (infos[0] if A.ndim == 2 else infos)
+
+
+def hermitian_eig(A):
+    A = np.ascontiguousarray(A, np.complex128)
+    P = A.shape[0]
+    ev = np.zeros(P, np.float64)
+    V = np.zeros((P, P), np.complex128)
+    L = lib()
+    L.qgtl_hermitian_eig.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    _check(L.qgtl_hermitian_eig(P, _p(A), _p(ev), _p(V)))
+    return ev, V
+
+
 def rel_err(Q, Qref):
     """Normwise relative error max|Q - Qref| / max|Qref|."""
     return float(np.max(np.abs(Q - Qref)) / np.max(np.abs(Qref)))

@@ -1,18 +1,21 @@
-// Complex Gram G = D^H D of the derivative states on MFMA, split-BF16x3.
+// Complex Gram G = D^H D of the derivative states on MFMA with split-BF16
+// operands and FP32 accumulation.
 //
 // D holds P complex64 states of N amplitudes (state s at D + s*N).  The real
 // matrix Z has rows 2s = Re d_s and 2s+1 = Im d_s, so C = Z Z^T gives
 //   Re G_kl = C[2k][2l] + C[2k+1][2l+1],  Im G_kl = C[2k][2l+1] - C[2k+1][2l].
-// Each fp32 value x is split into bf16 pieces.  Split-BF16x3: hi = rn(x),
-// lo = rn(x - hi); a product keeps hi*hi + hi*lo + lo*hi (three
-// v_mfma_f32_32x32x16_bf16 per 32x32x16 step, fp32 accumulation; 2^-17
-// relative per element, which averages out over long states).  Split-BF16x6:
-// x = hi + mid + lo exactly (3 x 8 = 24 significand bits) and the product
-// keeps the six terms down to 2^-16 (hh, hm, mh, hl, lh, mm): fp32-level
-// products for short states, where nothing averages.  K (the amplitude index) is split into
-// chunks; each workgroup writes an fp32 partial 128x128 tile of C and a
-// second kernel sums the chunks in FP64 in a fixed order.  Chunk c's tiles
-// all run on XCD c % 8 so the chunk is fetched into one L2 only.
+// Each fp32 value x is split into bf16 pieces and a product keeps the terms
+// above a cut (v_mfma_f32_32x32x16_bf16, fp32 accumulation):
+//   TERMS = 3: x = h + l (l = rn(x - h)), keep hh + hl + lh;
+//   TERMS = 4: the same split, keep hh + hl + lh + ll;
+//   TERMS = 6: x = h + m + l exactly (3 x 8 = 24 significand bits), keep
+//              hh + hm + mh + hl + lh + mm (fp32-level products).
+// Accumulation is two-level: the MFMA accumulators cover kInner amplitudes
+// and are then added into an fp32 outer sum, so rounding does not grow with
+// the chunk length.  K (the amplitude index) is split into chunks; each
+// workgroup writes an fp32 partial 128x128 tile of C and a second kernel sums
+// the chunks in FP64 in a fixed order.  Chunk c's tiles all run on XCD c % 8
+// so the chunk is fetched into one L2 only.
 //
 // QGTL counterpart: the per-element overlap loop
 // src/quantum_geometric/core/quantum_geometric_tensor_network.c:1127-1137 and
@@ -29,6 +32,7 @@ constexpr int kTile = 128;          // output tile (rows of Z)
 constexpr int kKS = 32;             // amplitudes per LDS stage
 constexpr int kRow = kKS + 8;       // bf16 per LDS row (80 B: 16-B aligned, spreads banks)
 constexpr int kXcd = 8;
+constexpr int kInner = 512;       // amplitudes per MFMA accumulation before the fp32 outer add
 
 __device__ __forceinline__ uint32_t bf16_rn_bits(float x) {
     const uint32_t u = __float_as_uint(x);
@@ -90,11 +94,11 @@ __device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, s
     }
 }
 
-// S = 2: split-BF16x3 (3 MFMA terms);  S = 3: split-BF16x6 (6 MFMA terms).
-template <int S>
+template <int TERMS>
 __global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
                                                               int nt, int nchunks, size_t chunk_len,
                                                               float *__restrict__ partials) {
+    constexpr int S = (TERMS == 6) ? 3 : 2;            // bf16 pieces per value
     __shared__ uint32_t lds[2 * S][kTile * kRow / 2];   // A pieces, then B pieces
 
     const int T = upper_tiles(nt);
@@ -109,13 +113,13 @@ __global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__res
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
-    f32x16 acc[2][2];
+    f32x16 acc[2][2], outer[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
 #pragma unroll
-            for (int e = 0; e < 16; e++) acc[a][b][e] = 0.f;
+            for (int e = 0; e < 16; e++) { acc[a][b][e] = 0.f; outer[a][b][e] = 0.f; }
 
     const size_t kbeg = (size_t)chunk * chunk_len;
     const size_t kend = (kbeg + chunk_len < N) ? kbeg + chunk_len : N;
@@ -126,52 +130,66 @@ __global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__res
         brows[p] = diag ? lds[p] : lds[S + p];
     }
 
-    for (size_t k0 = kbeg; k0 < kend; k0 += kKS) {
-        stage_rows<S>(D, P, N, ti * (kTile / 2), k0, arows);
-        if (!diag) stage_rows<S>(D, P, N, tj * (kTile / 2), k0, brows);
-        __syncthreads();
+    for (size_t kb = kbeg; kb < kend; kb += kInner) {
+        const size_t ke = (kb + kInner < kend) ? kb + kInner : kend;
+        for (size_t k0 = kb; k0 < ke; k0 += kKS) {
+            stage_rows<S>(D, P, N, ti * (kTile / 2), k0, arows);
+            if (!diag) stage_rows<S>(D, P, N, tj * (kTile / 2), k0, brows);
+            __syncthreads();
+#pragma unroll 1
+            for (int kk = 0; kk < kKS / 16; kk++) {
+                const int koff = kk * 16 + 8 * (lane >> 5);
+                bf16x8 fa[S][2], fb[S][2];
 #pragma unroll
-        for (int kk = 0; kk < kKS / 16; kk++) {
-            const int koff = kk * 16 + 8 * (lane >> 5);
-            bf16x8 fa[S][2], fb[S][2];
+                for (int p = 0; p < S; p++)
 #pragma unroll
-            for (int p = 0; p < S; p++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    const int ra = wm * 64 + b * 32 + (lane & 31);
-                    const int rb = wn * 64 + b * 32 + (lane & 31);
-                    fa[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(arows[p]) + ra * kRow + koff);
-                    fb[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(brows[p]) + rb * kRow + koff);
-                }
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    // smallest terms first
-                    if (S == 3) {
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][a], fb[0][b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2][b], acc[a][b], 0, 0, 0);
+                    for (int b = 0; b < 2; b++) {
+                        const int ra = wm * 64 + b * 32 + (lane & 31);
+                        const int rb = wn * 64 + b * 32 + (lane & 31);
+                        fa[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(arows[p]) + ra * kRow + koff);
+                        fb[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(brows[p]) + rb * kRow + koff);
                     }
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[0][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1][b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
-                }
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        // smallest terms first
+                        if (TERMS == 6) {
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][a], fb[0][b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2][b], acc[a][b], 0, 0, 0);
+                        }
+                        if (TERMS == 4)
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[0][b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1][b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+                    }
+            }
+            __syncthreads();
         }
-        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                outer[a][b] += acc[a][b];
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[a][b][e] = 0.f;
+            }
     }
 
     float *out = partials + ((size_t)chunk * T + tile) * (kTile * kTile);
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < 2; b++) {
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int row = wm * 64 + a * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
                 const int col = wn * 64 + b * 32 + (lane & 31);
-                out[row * kTile + col] = acc[a][b][e];
+                out[row * kTile + col] = outer[a][b][e];
             }
+        }
 }
 
 // C[ti*128 + r][tj*128 + c] = sum over chunks (fixed order, FP64); C is R x R.
@@ -262,12 +280,13 @@ int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nc
     const int nt = (2 * P + kTile - 1) / kTile;
     const int T = nt * (nt + 1) / 2;
     const size_t chunk_len = ((N + nchunks - 1) / nchunks + kKS - 1) / kKS * kKS;
+    const dim3 grid((unsigned)(T * nchunks)), block(kThreads);
     if (terms == 6)
-        hipLaunchKernelGGL(gram_split_kernel<3>, dim3((unsigned)(T * nchunks)), dim3(kThreads), 0, qgtl::stream(),
-                           (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+        hipLaunchKernelGGL(gram_split_kernel<6>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+    else if (terms == 4)
+        hipLaunchKernelGGL(gram_split_kernel<4>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     else
-        hipLaunchKernelGGL(gram_split_kernel<2>, dim3((unsigned)(T * nchunks)), dim3(kThreads), 0, qgtl::stream(),
-                           (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+        hipLaunchKernelGGL(gram_split_kernel<3>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     QGTL_LAUNCH_CHECK("gram_bf16x3");
     return 0;
 }

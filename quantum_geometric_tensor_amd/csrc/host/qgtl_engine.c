@@ -303,12 +303,14 @@ static int gram_chunks(const struct qgtl_plan *p)
     return want;
 }
 
-/* Split-BF16x3 needs the per-element 2^-17 representation error to average
- * over many amplitudes; short states use the exact three-piece split. */
+/* Split-BF16x3 leaves a ~2.7e-6 bias (the dropped lo*lo product); x4 keeps
+ * it and its remaining error is the random 2^-16 representation error of
+ * the lo piece, which averages out over long states; short states use the
+ * exact three-piece split (x6). */
 static int gram_terms(const struct qgtl_plan *p)
 {
-    if (p->tune.gram_variant == 3 || p->tune.gram_variant == 6) return p->tune.gram_variant;
-    return p->n >= QGTL_GRAM_X3_MIN_QUBITS ? 3 : 6;
+    if (p->tune.gram_variant == 3 || p->tune.gram_variant == 4 || p->tune.gram_variant == 6) return p->tune.gram_variant;
+    return p->n >= QGTL_GRAM_X4_MIN_QUBITS ? 4 : 6;
 }
 
 static int alloc_mixed(struct qgtl_plan *p)
@@ -411,7 +413,6 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
     int tiles = 0;
     LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles));
     LAUNCH(K_INJECT, qgtl_hip_reduce_partials_f64(p->inj_partials, cnt, tiles, p->inj_sums + 4 * (size_t)s0));
-    if (st) st->sweep_bytes += (double)cnt * (double)p->N * 8.0 + (double)p->N * 8.0;
     return 0;
 }
 
@@ -700,3 +701,5 @@ int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int 
     HIPCALL(qgtl_hip_sync());
     return 0;
 }
+
+int qgtl_init_if_needed(void) { return ensure_init(); }

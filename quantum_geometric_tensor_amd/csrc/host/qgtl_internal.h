@@ -22,7 +22,7 @@ typedef struct { double re, im; } qgtl_cd;
 typedef struct { qgtl_cd m[4]; } qgtl_mat2;       /* row-major 2x2 */
 
 #define QGTL_MAX_PASSES_PER_BLOCK 6
-#define QGTL_GRAM_X3_MIN_QUBITS 14   /* smallest n that uses split-BF16x3 by default */
+#define QGTL_GRAM_X4_MIN_QUBITS 14   /* smallest n that uses split-BF16x4 by default (x6 below) */
 
 typedef struct {
     int has_u[QGTL_MAX_QUBITS];
@@ -86,4 +86,8 @@ uint64_t qgtl_structure_hash(int n, int nops, const qgtl_op_t *ops, int mode);
 /* errors (qgtl_engine.c) */
 int qgtl_set_error(int code, const char *fmt, ...);
 
+
+
+/* engine (qgtl_engine.c) */
+int qgtl_init_if_needed(void);
 #endif

@@ -38,6 +38,7 @@ def lib():
         L.oracle_qgt_pairwise.argtypes = [ctypes.c_size_t, ip, vp, vp, vp]
         L.oracle_qgt_elementwise.argtypes = [ctypes.c_size_t, ip, vp, vp, vp]
         L.oracle_qgt.argtypes = [ip, ip, vp, vp]
+        L.oracle_qgt_hoisted.argtypes = [ip, ip, vp, vp]
         L.oracle_qgt_block.argtypes = [ip, ip, vp, ip, vp, ip, vp, vp]
         L.oracle_dense_unitary.argtypes = [ip, ip, vp, vp]
         L.oracle_pauli_expectation.argtypes = [ip, vp, ctypes.c_char_p]
@@ -110,6 +111,13 @@ def qgt(n, ops):
     P = num_params(ops)
     Q = np.empty((P, P), np.complex128)
     assert lib().oracle_qgt(n, len(ops), _p(ops), _p(Q)) == 0
+    return Q
+
+
+def qgt_hoisted(n, ops):
+    P = num_params(ops)
+    Q = np.empty((P, P), np.complex128)
+    assert lib().oracle_qgt_hoisted(n, len(ops), _p(ops), _p(Q)) == 0
     return Q
 
 

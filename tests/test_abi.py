@@ -1,0 +1,88 @@
+"""CPU checks of the C-ABI boundary (no GPU work is launched).
+
+- every function declared in include/*.h is exported by libqgtl_mi355x.so;
+- the structs and enum values of the drop-in API have QGTL's layout
+  (sizeof/offsetof printed by oracle/abi_layout.c against QGTL's headers and
+  against include/; skipped when the QGTL build of the probe is absent);
+- host-side plan validation returns QGTL's error codes;
+- the host HEA builder matches the oracle's.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import quantum_geometric_tensor_amd as Q
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)   # struct/enum typedefs
+    names = set()
+    for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_\s\*]*?)\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
+        ret, name = m.group(1).strip(), m.group(2)
+        if not ret or name in ("if", "while", "for", "sizeof") or ret.startswith("#"):
+            continue
+        names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", ["qgtl_api.h", "qgtl_mi355x.h", "qgtl_hip_abi.h"])
+def test_all_declared_symbols_exported(header):
+    lib = Q.lib()
+    names = declared_functions(os.path.join(INCLUDE, header))
+    assert len(names) > 5
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, f"{header}: not exported: {missing}"
+
+
+def test_abi_layout_matches_qgtl_headers():
+    ref = os.path.join(ROOT, "oracle", "_ref", "abi_layout_ref")
+    ours = os.path.join(ROOT, "tests", "c", "_build", "abi_layout_ours")
+    if not os.path.exists(ref):
+        pytest.skip("QGTL-header build of the ABI probe not present (needs the QGTL sources)")
+    a = subprocess.run([ref], stdout=subprocess.PIPE, text=True, check=True).stdout
+    b = subprocess.run([ours], stdout=subprocess.PIPE, text=True, check=True).stdout
+    assert a.count("\n") > 80
+    assert a == b
+
+
+def test_plan_validation_errors():
+    lib = Q.lib()
+    bad_qubit = Q.make_ops([("RY", 5, -1, 0, 0.1)])
+    h = lib.qgtl_plan_create(3, 1, bad_qubit.ctypes.data_as(ctypes.c_void_p), 0, None)
+    assert not h and b"out of range" in lib.qgtl_last_error()
+    bad_param = Q.make_ops([("H", 0, -1, 0, 0.0)])
+    h = lib.qgtl_plan_create(2, 1, bad_param.ctypes.data_as(ctypes.c_void_p), 0, None)
+    assert not h and b"RX/RY/RZ" in lib.qgtl_last_error()
+    dup = Q.make_ops([("RY", 0, -1, 0, 0.1), ("RZ", 1, -1, 0, 0.2)])
+    h = lib.qgtl_plan_create(2, 2, dup.ctypes.data_as(ctypes.c_void_p), 0, None)
+    assert not h and b"each once" in lib.qgtl_last_error()
+    cz = Q.make_ops([("RY", 0, -1, 0, 0.1), ("CZ", 0, 0, -1, 0.0)])
+    h = lib.qgtl_plan_create(2, 2, cz.ctypes.data_as(ctypes.c_void_p), 0, None)
+    assert not h
+    unsupported = np.zeros(1, Q.OP_DTYPE)
+    unsupported[0] = (33, 0, 1, -1, 0.0)          # GATE_TYPE_XX
+    h = lib.qgtl_plan_create(2, 1, unsupported.ctypes.data_as(ctypes.c_void_p), 0, None)
+    assert not h and b"not supported" in lib.qgtl_last_error()
+    good = Q.hea_ops(4, 2, np.linspace(0, 1, 16))
+    p = Q.Plan(4, good)           # plan creation allocates nothing on the device
+    assert p.P == 16
+    p.close()
+
+
+def test_hea_builder_matches_oracle():
+    for n, L in [(1, 2), (2, 3), (7, 2)]:
+        th = O.uniform_angles(n, 2 * n * L)
+        a = Q.hea_ops(n, L, th)
+        b = O.hea_ops(n, L, th)
+        assert a.tobytes() == b.tobytes()
