@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time only the mixed-precision path (for profiler runs): no FP64 legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the QGT path on a single MI355X "
@@ -83,6 +85,12 @@ def main():
     sweep_bytes_per_launch = st["sweep_bytes"] / max(1, st["sweep_launches"])
     achieved = sweep_bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
     plan.close()
+    if args.mixed_only:
+        print(json.dumps({"mixed_only": True, "value": qgt_per_s, "unit": "QGT/s", "ms_per_step": ms_per_step,
+                          "breakdown_ms": {"sweep": st["sweep_ms"], "inject": st["inject_ms"],
+                                           "gram": st["gram_ms"], "launches": st["kernel_launches"]},
+                          "sweep_achieved_GBps": achieved}), flush=True)
+        return
 
     # plain FP64 GPU path (the speed-up denominator and the GPU FP64 cross-check)
     plain = Q.Plan(n, ops, Q.MODE_FP64_PLAIN)

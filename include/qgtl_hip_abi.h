@@ -12,6 +12,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "qgtl_mi355x.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -20,6 +22,7 @@ extern "C" {
 #define QGTL_MAX_QUBITS 30
 #define QGTL_PERM_MAX 32
 #define QGTL_INJECT_MAX 64
+#define QGTL_MAX_PHASES 4
 
 typedef struct { float re, im; } qgtl_c64_t;
 
@@ -47,6 +50,15 @@ typedef struct {
     int8_t perm_a[QGTL_PERM_MAX];
     int8_t perm_b[QGTL_PERM_MAX];
     qgtl_c64_t mat[QGTL_TILE_BITS_MAX][4];     /* row-major 2x2 */
+    /* Register-phase plan (full 2^13 tiles): 32 amplitudes per thread; in
+     * phase p the tile bits ph_rbits[p][0..4] index a thread's registers and
+     * the gates ph_first[p] .. ph_first[p+1]-1 (in mat order) act on register
+     * bit ph_greg[g].  Phase 0 is the coalesced layout {0, 7, 8, 9, 10}.
+     * nphase = 0: no plan (the LDS-only kernel runs the pass). */
+    int32_t nphase;
+    int8_t ph_rbits[QGTL_MAX_PHASES][5];
+    int8_t ph_first[QGTL_MAX_PHASES + 1];
+    int8_t ph_greg[QGTL_TILE_BITS_MAX];
 } qgtl_pass_desc_t;
 
 /* Derivative-state injection in the frame after a block:
@@ -89,6 +101,13 @@ int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, doubl
 int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t);
 int qgtl_hip_swap_c64(void *base, size_t stride, int nstates, int n, int a, int b);
 int qgtl_hip_c64_to_c128(const void *src, void *dst, size_t count);
+/* out = H psi for a Pauli sum (terms in device memory) */
+int qgtl_hip_pauli_apply_c64(const void *psi, void *out, int n, int nterms, const void *terms_dev);
+/* per (state, tile) FP64 sums: <D_k|v>, |D_k|^2, |v|^2 (same layout as inject partials) */
+int qgtl_hip_dot_states_c64(const void *D, size_t stride, int K, const void *v, int n, double *partials,
+                            int *tiles_out);
+/* out[0] = energy, out[1 + k] = gradient (from reduced dot sums) */
+int qgtl_hip_energy_gradient_finalize(const double *dk_h, const double *psi_h, int P, double *out);
 
 /* ---- Gram (mixed): C = Z Z^T with Z rows (Re d_s, Im d_s), split-BF16x3 on MFMA ---- */
 int qgtl_hip_gram_tiles(int P);                                          /* upper 128x128 tiles */

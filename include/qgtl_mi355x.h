@@ -99,6 +99,25 @@ void qgtl_plan_destroy(qgtl_plan_t *plan);
 /* Release every cached device buffer of the calling process. */
 void qgtl_release_cache(void);
 
+/* ---- Hamiltonians: sums of Pauli strings ----
+ * term = coeff * i^{ny} X^xmask Z^zmask with ny = popcount(xmask & zmask)
+ * (a Y on qubit q sets bit q in both masks). */
+typedef struct {
+    uint32_t xmask;
+    uint32_t zmask;
+    double coeff_re;
+    double coeff_im;
+} qgtl_pauli_term_t;
+/* QGTL's Pauli-string convention: character 0 acts on the highest qubit
+ * (src/quantum_geometric/hybrid/quantum_classical_algorithms.c:339). */
+int qgtl_pauli_term_from_string(const char *pauli, double coeff, qgtl_pauli_term_t *out);
+/* Transverse-field Ising chain H = -J sum_{i<n-1} Z_i Z_{i+1} - h sum_i X_i (2n-1 terms). */
+int qgtl_tfim_terms(int n, double J, double h, qgtl_pauli_term_t *terms);
+int qgtl_plan_set_hamiltonian(qgtl_plan_t *plan, int nterms, const qgtl_pauli_term_t *terms);
+/* After qgtl_plan_execute (mixed mode): E = <psi|H|psi> and
+ * grad_k = 2 Re <d_k psi|H|psi> from the derivative states left on the GPU. */
+int qgtl_plan_energy_gradient(qgtl_plan_t *plan, double *energy, double *grad);
+
 /* FP64 copy (P x P complex128) of the QGT that the drop-in API caches on a
  * network created by create_quantum_geometric_tensor_network. */
 struct quantum_geometric_tensor_network;

@@ -159,6 +159,22 @@ class Plan:
         _check(lib().qgtl_plan_execute(self._h, _p(Q) if want_q else None, ctypes.byref(st)))
         return Q, st.as_dict()
 
+    def set_hamiltonian(self, terms):
+        """terms: array of PAULI_DTYPE (see pauli_terms / tfim_terms)."""
+        terms = np.ascontiguousarray(terms, PAULI_DTYPE)
+        L = lib()
+        L.qgtl_plan_set_hamiltonian.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        _check(L.qgtl_plan_set_hamiltonian(self._h, len(terms), _p(terms)))
+
+    def energy_gradient(self):
+        """E and grad_k = 2 Re <d_k psi|H|psi> from the last execute()."""
+        e = np.zeros(1, np.float64)
+        g = np.zeros(self.P, np.float64)
+        L = lib()
+        L.qgtl_plan_energy_gradient.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _check(L.qgtl_plan_energy_gradient(self._h, _p(e), _p(g)))
+        return float(e[0]), g
+
     def close(self):
         if getattr(self, "_h", None):
             lib().qgtl_plan_destroy(self._h)
@@ -169,6 +185,31 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+PAULI_DTYPE = np.dtype([("xmask", np.uint32), ("zmask", np.uint32), ("coeff_re", np.float64),
+                        ("coeff_im", np.float64)], align=True)
+
+
+def pauli_terms(strings, coeffs):
+    """QGTL convention: character 0 of each string acts on the highest qubit."""
+    out = np.zeros(len(strings), PAULI_DTYPE)
+    L = lib()
+    L.qgtl_pauli_term_from_string.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_void_p]
+    for i, (s, c) in enumerate(zip(strings, coeffs)):
+        _check(L.qgtl_pauli_term_from_string(s.encode(), float(c), out[i:i + 1].ctypes.data))
+    return out
+
+
+def tfim_terms(n, J, h):
+    """H = -J sum_{i<n-1} Z_i Z_{i+1} - h sum_i X_i."""
+    out = np.zeros(2 * n - 1, PAULI_DTYPE)
+    L = lib()
+    L.qgtl_tfim_terms.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    k = L.qgtl_tfim_terms(n, J, h, _p(out))
+    if k < 0:
+        _check(k)
+    return out
 
 
 SOLVE_CHOLESKY, SOLVE_CG, SOLVE_JACOBI_PCG, SOLVE_PCG = 0, 1, 2, 3

@@ -177,6 +177,191 @@ __global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc
     }
 }
 
+// ---------------------------------------------------------------------------
+// Register-phase pass for full 2^13 tiles.  Each of the 256 threads owns 32
+// amplitudes: in a layout, 5 tile bits select the register and the other 8
+// tile bits (ascending) are the thread index.  Gates on register bits run
+// with compile-time register pairing; a layout change is one LDS transpose
+// (write all, barrier, read all, barrier).  The load/store layout {0, 7..10}
+// gives each 64-lane load instruction 1 KB of contiguous amplitudes when the
+// tile's low 7 bits are contiguous.
+constexpr int kRT = 13;
+constexpr int kRegs = 32;
+
+__device__ __forceinline__ int rpad(uint32_t j) { return (int)(j + (j >> 5)); }
+
+// Tile index of register r for a thread whose other-bit deposit is tdep.
+struct RegLayout {
+    uint32_t tdep;
+    uint32_t rb[5];
+};
+
+__device__ __forceinline__ RegLayout make_layout(const int8_t *rbits, uint32_t t) {
+    RegLayout L;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        L.rb[i] = 1u << rbits[i];
+        mask |= L.rb[i];
+    }
+    uint32_t d = 0;
+    int k = 0;
+#pragma unroll
+    for (int b = 0; b < kRT; b++)
+        if (!((mask >> b) & 1u)) {
+            d |= ((t >> k) & 1u) << b;
+            k++;
+        }
+    L.tdep = d;
+    return L;
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t reg_index(const RegLayout &L) {
+    uint32_t j = L.tdep;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if ((R >> i) & 1) j |= L.rb[i];
+    return j;
+}
+
+template <int P>
+__device__ __forceinline__ void apply_reg(float2 (&v)[kRegs], const float2 *m) {
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!((r >> P) & 1)) apply2(v[r], v[r | (1 << P)], m);
+}
+
+template <int R>
+struct RegLoop {
+    __device__ __forceinline__ static void store_lds(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::store_lds(tile, L, v);
+        tile[rpad(reg_index<R - 1>(L))] = v[R - 1];
+    }
+    __device__ __forceinline__ static void store_lds_ring(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs],
+                                                         uint32_t par_r) {
+        RegLoop<R - 1>::store_lds_ring(tile, L, v, par_r);
+        uint32_t p = reg_index<R - 1>(L);
+        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
+        p &= (1u << kRT) - 1u;
+        const uint32_t y = p ^ (((p >> (kRT - 1)) & 1u) ^ par_r);
+        tile[rpad(y)] = v[R - 1];
+    }
+    __device__ __forceinline__ static void load_lds(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::load_lds(tile, L, v);
+        v[R - 1] = tile[rpad(reg_index<R - 1>(L))];
+    }
+    // global access in pairs (register bit 0 = tile bit 0)
+    __device__ __forceinline__ static void load_global(const float2 *src, uint32_t base, const uint32_t *hi_off,
+                                                       int nlow, uint32_t lowmask, const RegLayout &L, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global(src, base, hi_off, nlow, lowmask, L, v);
+        const uint32_t j = reg_index<R - 2>(L);
+        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
+        const float4 q = *reinterpret_cast<const float4 *>(src + g);
+        v[R - 2] = make_float2(q.x, q.y);
+        v[R - 1] = make_float2(q.z, q.w);
+    }
+    __device__ __forceinline__ static void store_global(float2 *dst, uint32_t base, const uint32_t *hi_off, int nlow,
+                                                        uint32_t lowmask, const RegLayout &L, const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global(dst, base, hi_off, nlow, lowmask, L, v);
+        const uint32_t j = reg_index<R - 2>(L);
+        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
+        *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
+    }
+    __device__ __forceinline__ static void store_global_ring(float2 *dst, uint32_t pre, uint32_t ones,
+                                                             const RegLayout &L, const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global_ring(dst, pre, ones, L, v);
+        const uint32_t y = reg_index<R - 2>(L);
+        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
+        const size_t g = ((size_t)yhi << kRT) | y;
+        *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
+    }
+};
+
+template <>
+struct RegLoop<0> {
+    __device__ __forceinline__ static void store_lds(float2 *, const RegLayout &, const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_lds_ring(float2 *, const RegLayout &, const float2 (&)[kRegs], uint32_t) {}
+    __device__ __forceinline__ static void load_lds(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_global(const float2 *, uint32_t, const uint32_t *, int, uint32_t,
+                                                       const RegLayout &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global(float2 *, uint32_t, const uint32_t *, int, uint32_t,
+                                                        const RegLayout &, const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global_ring(float2 *, uint32_t, uint32_t, const RegLayout &,
+                                                             const float2 (&)[kRegs]) {}
+};
+
+__global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
+                                                                 const float2 *__restrict__ src_all,
+                                                                 float2 *dst_all, size_t stride) {
+    __shared__ float2 tile[(1 << kRT) + (1 << kRT) / 32];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+
+    const int nlow = d.nlow, nhi = kRT - nlow;
+    const uint32_t r = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const float2 *src = src_all + (size_t)blockIdx.y * stride;
+    float2 *dst = dst_all + (size_t)blockIdx.y * stride;
+
+    uint32_t base = 0;
+    for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
+    for (int h = threadIdx.x; h < (1 << nhi); h += kThreads) {
+        uint32_t off = 0;
+        for (int i = 0; i < nhi; i++) off |= ((uint32_t)(h >> i) & 1u) << d.hi_pos[i];
+        hi_off[h] = off;
+    }
+    __syncthreads();
+    const uint32_t lowmask = (1u << nlow) - 1u;
+
+    float2 v[kRegs];
+    const RegLayout L0 = make_layout(d.ph_rbits[0], t);
+    RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
+
+    RegLayout Lc = L0;
+    for (int ph = 0; ph < d.nphase; ph++) {
+        if (ph > 0) {
+            const RegLayout Ln = make_layout(d.ph_rbits[ph], t);
+            RegLoop<kRegs>::store_lds(tile, Lc, v);
+            __syncthreads();
+            RegLoop<kRegs>::load_lds(tile, Ln, v);
+            __syncthreads();
+            Lc = Ln;
+        }
+        for (int g = d.ph_first[ph]; g < d.ph_first[ph + 1]; g++) {
+            float2 m[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) m[e] = make_float2(d.mat[g][e].re, d.mat[g][e].im);
+            switch (d.ph_greg[g]) {
+            case 0: apply_reg<0>(v, m); break;
+            case 1: apply_reg<1>(v, m); break;
+            case 2: apply_reg<2>(v, m); break;
+            case 3: apply_reg<3>(v, m); break;
+            default: apply_reg<4>(v, m); break;
+            }
+        }
+    }
+
+    if (d.store == QGTL_STORE_RING) {
+        const int nh = d.n - kRT;
+        uint32_t pre = r;
+        pre ^= pre << 1; pre ^= pre << 2; pre ^= pre << 4; pre ^= pre << 8; pre ^= pre << 16;
+        const uint32_t ones = (nh >= 32) ? 0xffffffffu : ((1u << nh) - 1u);
+        pre &= ones;
+        const uint32_t par_r = (uint32_t)__popc(r) & 1u;
+        RegLoop<kRegs>::store_lds_ring(tile, Lc, v, par_r);
+        __syncthreads();
+        RegLoop<kRegs>::load_lds(tile, L0, v);
+        RegLoop<kRegs>::store_global_ring(dst, pre, ones, L0, v);
+    } else {
+        if (d.nphase > 1) {
+            RegLoop<kRegs>::store_lds(tile, Lc, v);
+            __syncthreads();
+            RegLoop<kRegs>::load_lds(tile, L0, v);
+        }
+        RegLoop<kRegs>::store_global(dst, base, hi_off, nlow, lowmask, L0, v);
+    }
+}
+
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
 // FP64 per-tile sums of <d|psi>, |d|^2 and |psi|^2.
 constexpr int kInjectPerThread = 8;
@@ -249,6 +434,81 @@ __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double 
     if (threadIdx.x < 4) out[(size_t)k * 4 + threadIdx.x] = red[0][threadIdx.x];
 }
 
+// H psi for H = sum_t c_t P_t, P_t = i^{ny} X^x Z^z (ny = popcount(x & z)):
+// (P psi)[y] = i^{ny} (-1)^{|(y ^ x) & z|} psi[y ^ x].  QGTL's Pauli-string
+// Hamiltonians (src/quantum_geometric/hybrid/quantum_classical_algorithms.c:320-380).
+__global__ __launch_bounds__(kThreads) void pauli_apply_c64_kernel(const float2 *__restrict__ psi, float2 *out,
+                                                                   uint32_t N, int nterms,
+                                                                   const qgtl_pauli_term_t *__restrict__ terms) {
+    for (uint32_t y = blockIdx.x * kThreads + threadIdx.x; y < N; y += gridDim.x * kThreads) {
+        double ar = 0.0, ai = 0.0;
+        for (int t = 0; t < nterms; t++) {
+            const uint32_t x = terms[t].xmask, z = terms[t].zmask;
+            const uint32_t src = y ^ x;
+            const float2 v = psi[src];
+            const int ny = __popc(x & z) & 3;
+            const int neg = __popc(src & z) & 1;
+            // phase = i^ny * (-1)^neg applied to c_t * v
+            double pr = terms[t].coeff_re * v.x - terms[t].coeff_im * v.y;
+            double pi = terms[t].coeff_re * v.y + terms[t].coeff_im * v.x;
+            double qr, qi;
+            switch (ny) {
+            case 0: qr = pr; qi = pi; break;
+            case 1: qr = -pi; qi = pr; break;
+            case 2: qr = -pr; qi = -pi; break;
+            default: qr = pi; qi = -pr; break;
+            }
+            if (neg) { qr = -qr; qi = -qi; }
+            ar += qr;
+            ai += qi;
+        }
+        out[y] = make_float2((float)ar, (float)ai);
+    }
+}
+
+// FP64 sums per (state k, tile): <D_k|v> (re, im), |D_k|^2, |v|^2.
+__global__ __launch_bounds__(kThreads) void dot_states_c64_kernel(const float2 *__restrict__ D, size_t stride,
+                                                                  const float2 *__restrict__ v, uint32_t N,
+                                                                  double *partials) {
+    const int k = blockIdx.y;
+    const float2 *d = D + (size_t)k * stride;
+    double sr = 0.0, si = 0.0, sd = 0.0, sv = 0.0;
+#pragma unroll
+    for (int i = 0; i < kInjectPerThread; i++) {
+        const uint32_t y = blockIdx.x * kInjectTile + i * kThreads + threadIdx.x;
+        if (y < N) {
+            const float2 a = d[y], b = v[y];
+            const double ar = a.x, aim = a.y, br = b.x, bi = b.y;
+            sr += ar * br + aim * bi;
+            si += ar * bi - aim * br;
+            sd += ar * ar + aim * aim;
+            sv += br * br + bi * bi;
+        }
+    }
+    __shared__ double red[kThreads / 64][4];
+    sr = wave_sum(sr); si = wave_sum(si); sd = wave_sum(sd); sv = wave_sum(sv);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = sr; red[wave][1] = si; red[wave][2] = sd; red[wave][3] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int wv = 0; wv < kThreads / 64; wv++) s += red[wv][threadIdx.x];
+        partials[((size_t)k * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = s;
+    }
+}
+
+// energy = <psi|H psi> / |psi|^2; grad_k = 2 Re <d_k|H psi> / (2 |d_k| |psi|)
+// (normalised with the exact |d_k| = 1/2, |psi| = 1 as the QGT is).
+__global__ void energy_gradient_finalize_kernel(const double *__restrict__ dk_h, const double *__restrict__ psi_h,
+                                                int P, double *out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const double npsi = psi_h[2];
+    if (k == 0) out[0] = psi_h[0] / npsi;
+    if (k < P) out[1 + k] = 2.0 * dk_h[4 * k] * 0.5 / sqrt(dk_h[4 * k + 2] * npsi);
+}
+
 __global__ void fill_zero_state_c64_kernel(float2 *psi, uint32_t N) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
         psi[i] = make_float2(i == 0 ? 1.f : 0.f, 0.f);
@@ -316,8 +576,14 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
     if ((desc->store == QGTL_STORE_RING || desc->store == QGTL_STORE_PERM) && src == dst && desc->store == QGTL_STORE_RING)
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: ring store cannot run in place");
     const dim3 grid(1u << desc->nrest, (unsigned)nstates);
-    hipLaunchKernelGGL(pass_c64_kernel<QGTL_TILE_BITS_MAX>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                       (const float2 *)src, (float2 *)dst, stride);
+    const bool regs = desc->m == kRT && desc->nphase >= 1 && desc->nphase <= QGTL_MAX_PHASES &&
+                      desc->store != QGTL_STORE_PERM && tile_variant != 1;
+    if (regs)
+        hipLaunchKernelGGL(pass_c64_regs_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)src, (float2 *)dst, stride);
+    else
+        hipLaunchKernelGGL(pass_c64_kernel<QGTL_TILE_BITS_MAX>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)src, (float2 *)dst, stride);
     QGTL_LAUNCH_CHECK("pass_c64");
     return 0;
 }
@@ -357,6 +623,32 @@ int qgtl_hip_swap_c64(void *base, size_t stride, int nstates, int n, int a, int 
     hipLaunchKernelGGL(swap_c64_kernel, dim3(grid_for(quarter), (unsigned)nstates), dim3(256), 0,
                        qgtl::stream(), (float2 *)base, stride, quarter, a, b);
     QGTL_LAUNCH_CHECK("swap_c64");
+    return 0;
+}
+
+int qgtl_hip_pauli_apply_c64(const void *psi, void *out, int n, int nterms, const void *terms_dev) {
+    const uint32_t N = 1u << n;
+    hipLaunchKernelGGL(pauli_apply_c64_kernel, dim3(grid_for(N)), dim3(kThreads), 0, qgtl::stream(),
+                       (const float2 *)psi, (float2 *)out, N, nterms, (const qgtl_pauli_term_t *)terms_dev);
+    QGTL_LAUNCH_CHECK("pauli_apply_c64");
+    return 0;
+}
+
+int qgtl_hip_dot_states_c64(const void *D, size_t stride, int K, const void *v, int n, double *partials,
+                            int *tiles_out) {
+    const uint32_t N = 1u << n;
+    const unsigned tiles = (N + kInjectTile - 1) / kInjectTile;
+    *tiles_out = (int)tiles;
+    hipLaunchKernelGGL(dot_states_c64_kernel, dim3(tiles, (unsigned)K), dim3(kThreads), 0, qgtl::stream(),
+                       (const float2 *)D, stride, (const float2 *)v, N, partials);
+    QGTL_LAUNCH_CHECK("dot_states_c64");
+    return 0;
+}
+
+int qgtl_hip_energy_gradient_finalize(const double *dk_h, const double *psi_h, int P, double *out) {
+    hipLaunchKernelGGL(energy_gradient_finalize_kernel, dim3((unsigned)((P + 255) / 256 + 1)), dim3(256), 0,
+                       qgtl::stream(), dk_h, psi_h, P, out);
+    QGTL_LAUNCH_CHECK("energy_gradient_finalize");
     return 0;
 }
 

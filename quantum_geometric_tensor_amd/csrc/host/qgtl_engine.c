@@ -221,6 +221,14 @@ static void plan_free_device(struct qgtl_plan *p)
     qgtl_hip_free(p->C);
     qgtl_hip_free(p->Qdev);
     qgtl_hip_free(p->plain_partials);
+    qgtl_hip_free(p->terms_dev);
+    qgtl_hip_free(p->hpsi);
+    qgtl_hip_free(p->eg_partials);
+    qgtl_hip_free(p->eg_sums);
+    qgtl_hip_free(p->eg_out);
+    p->terms_dev = p->hpsi = NULL;
+    p->eg_partials = p->eg_sums = p->eg_out = NULL;
+    p->psi_final = NULL;
     p->psi[0] = p->psi[1] = p->D = p->tmp = NULL;
     p->inj_partials = p->inj_sums = p->C = p->Qdev = p->plain_partials = NULL;
     p->gram_partials = NULL;
@@ -461,6 +469,7 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
         }
         slot += K;
     }
+    p->psi_final = cur;
     if (p->P == 0) return 0;
     LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p)));
     LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, p->P, p->gram_chunks, p->C));

@@ -75,6 +75,14 @@ struct qgtl_plan {
     double *plain_partials;
     size_t plain_partials_len;
     uint64_t structure_hash;
+    void *psi_final;                     /* primal after the last execute (mixed) */
+    /* Hamiltonian for energy / gradient */
+    int nterms;
+    void *terms_dev;
+    void *hpsi;
+    double *eg_partials;
+    double *eg_sums;                     /* (P + 1) x 4 */
+    double *eg_out;                      /* 1 + P */
 };
 
 /* plan construction (qgtl_plan.c) */

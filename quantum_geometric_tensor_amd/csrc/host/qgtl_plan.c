@@ -146,6 +146,60 @@ static void to_c64(const qgtl_mat2 *m, qgtl_c64_t out[4])
     }
 }
 
+/* Register-phase plan for a full 2^13 tile (see qgtl_pass_desc_t): phase 0
+ * holds tile bits {0, 7, 8, 9, 10}; the remaining gate bits go five per phase
+ * (topped up with other tile bits).  Gates are reordered by phase (they
+ * commute: one matrix per qubit). */
+static void plan_phases(qgtl_pass_desc_t *p)
+{
+    static const int8_t ph0[5] = {0, 7, 8, 9, 10};
+    p->nphase = 0;
+    if (p->m != QGTL_TILE_BITS_MAX || p->store == QGTL_STORE_PERM) return;
+    int in0[QGTL_TILE_BITS_MAX] = {0};
+    for (int i = 0; i < 5; i++) in0[ph0[i]] = 1;
+    int order[QGTL_TILE_BITS_MAX], cnt = 0;
+    int rest[QGTL_TILE_BITS_MAX], nrest = 0;
+    memcpy(p->ph_rbits[0], ph0, 5);
+    p->ph_first[0] = 0;
+    for (int g = 0; g < p->ngates; g++)
+        if (in0[p->gate_bit[g]]) {
+            for (int i = 0; i < 5; i++)
+                if (ph0[i] == p->gate_bit[g]) p->ph_greg[cnt] = (int8_t)i;
+            order[cnt++] = g;
+        } else {
+            rest[nrest++] = g;
+        }
+    int nph = 1;
+    for (int r0 = 0; r0 < nrest; r0 += 5, nph++) {
+        p->ph_first[nph] = (int8_t)cnt;
+        const int len = (nrest - r0 < 5) ? nrest - r0 : 5;
+        int used[QGTL_TILE_BITS_MAX] = {0};
+        for (int i = 0; i < len; i++) {
+            const int g = rest[r0 + i];
+            p->ph_rbits[nph][i] = p->gate_bit[g];
+            used[p->gate_bit[g]] = 1;
+            p->ph_greg[cnt] = (int8_t)i;
+            order[cnt++] = g;
+        }
+        for (int i = len, b = 0; i < 5; i++) {
+            while (used[b]) b++;
+            p->ph_rbits[nph][i] = (int8_t)b;
+            used[b] = 1;
+        }
+    }
+    p->ph_first[nph] = (int8_t)cnt;
+    p->nphase = nph;
+    /* reorder matrices and their tile bits into phase order */
+    qgtl_c64_t mats[QGTL_TILE_BITS_MAX][4];
+    int8_t bits[QGTL_TILE_BITS_MAX];
+    for (int i = 0; i < cnt; i++) {
+        memcpy(mats[i], p->mat[order[i]], sizeof mats[i]);
+        bits[i] = p->gate_bit[order[i]];
+    }
+    memcpy(p->mat, mats, sizeof(qgtl_c64_t) * 4 * (size_t)cnt);
+    memcpy(p->gate_bit, bits, (size_t)cnt);
+}
+
 static void build_passes(qgtl_block_t *b, int n)
 {
     const int TB = QGTL_TILE_BITS_MAX;
@@ -221,6 +275,7 @@ static void build_passes(qgtl_block_t *b, int n)
         b->pass_flips[b->npasses] = b->is_ring;
         b->npasses++;
     }
+    for (int i = 0; i < b->npasses; i++) plan_phases(&b->pass[i]);
 }
 
 static void detect_ring(qgtl_block_t *b, int n)

@@ -64,3 +64,40 @@ def test_qgt_mixed_deterministic():
     a = Q.qgt(14, ops, Q.MODE_MIXED)
     b = Q.qgt(14, ops, Q.MODE_MIXED)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,L", [(6, 3), (12, 3), (16, 2)])
+def test_energy_gradient_tfim(n, L):
+    J, h = 1.0, 0.83
+    th, ops, oops = hea(n, L, seed=500 + n)
+    e_ref, g_ref = O.tfim_gradient(n, oops, J, h)
+    p = Q.Plan(n, ops, Q.MODE_MIXED)
+    p.set_hamiltonian(Q.tfim_terms(n, J, h))
+    p.execute()
+    e, g = p.energy_gradient()
+    p.close()
+    assert abs(e - e_ref) <= 1e-6 * max(1.0, abs(e_ref))
+    assert np.max(np.abs(g - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
+
+
+def test_pauli_strings_qgtl_convention():
+    # a general Pauli sum with Y terms, QGTL string convention (char 0 = highest qubit)
+    n, L = 5, 2
+    th, ops, oops = hea(n, L, seed=77)
+    strings, coeffs = ["XYZIZ", "ZZIII", "IYIXI", "YIIIY"], [0.7, -1.1, 0.4, 0.25]
+    psi = O.run(n, oops)
+    e_ref = sum(c * O.pauli_expectation(n, psi, s) for s, c in zip(strings, coeffs))
+    p = Q.Plan(n, ops, Q.MODE_MIXED)
+    p.set_hamiltonian(Q.pauli_terms(strings, coeffs))
+    p.execute()
+    e, g = p.energy_gradient()
+    p.close()
+    assert abs(e - e_ref) < 1e-6
+    # gradient by central differences of the oracle energy
+    for k in [0, 7, 19]:
+        tp, tm = th.copy(), th.copy()
+        tp[k] += 1e-5
+        tm[k] -= 1e-5
+        ep = sum(c * O.pauli_expectation(n, O.run(n, O.hea_ops(n, L, tp)), s) for s, c in zip(strings, coeffs))
+        em = sum(c * O.pauli_expectation(n, O.run(n, O.hea_ops(n, L, tm)), s) for s, c in zip(strings, coeffs))
+        assert abs(g[k] - (ep - em) / 2e-5) < 1e-6
