@@ -80,7 +80,6 @@ def main():
     _, st = plan.execute()
     _, st = plan.execute()
     Q.set_profiling(False)
-    terms = 4 if n >= 14 else 6
     sweep_avg_ms = st["sweep_ms"] / max(1, st["sweep_launches"])
     sweep_bytes_per_launch = st["sweep_bytes"] / max(1, st["sweep_launches"])
     achieved = sweep_bytes_per_launch / (sweep_avg_ms * 1e-3) / 1e9
@@ -129,7 +128,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"c64 sweeps + bf16x{terms} MFMA (fp32 acc) + f64 reductions",
+        "dtype": "c64 sweeps + split-fp16x3 MFMA Gram (fp32 acc) + f64 reductions/correction",
         "data": "synthetic (theta ~ U[0,2pi), splitmix64 seed 0)",
         "config": {"workload": "hardware-efficient ansatz QGT, n=20 qubits, 6 layers, P=240", "n": n,
                    "layers": L, "P": P, "amplitudes": N, "mode": "mixed"},

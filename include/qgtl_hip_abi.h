@@ -71,6 +71,90 @@ typedef struct {
     qgtl_c64_t A[QGTL_INJECT_MAX][4];
 } qgtl_inject_desc_t;
 
+/* ---- grouped launches ----
+ * One launch runs the same kind of work for many independent instances.
+ * Work item i covers the units [unit_begin_i, unit_begin_{i+1}) of the
+ * launch (the last item ends at the launch's unit count); a workgroup finds
+ * its item by binary search over unit_begin and loops over units with a
+ * grid stride, so the grid can be sized to the machine (persistent) rather
+ * than to the work.  All addresses are device addresses. */
+typedef struct {
+    uint64_t desc;          /* qgtl_pass_desc_t */
+    uint64_t src, dst;      /* state 0; state s at + s * stride */
+    uint64_t stride;        /* complex64 elements between states */
+    uint32_t ntiles;        /* tiles per state, 2^(n - m) */
+    uint32_t nstates;
+    uint32_t unit_begin;    /* units: ntiles * nstates, tile-major within a state */
+    uint32_t variant;       /* 0: LDS gate groups; 1: register phases (m = 13, nphase >= 1, no PERM store) */
+} qgtl_pass_item_t;
+
+typedef struct {
+    uint64_t desc;          /* qgtl_inject_desc_t */
+    uint64_t psi, dst;      /* state k of the item written at dst + k * stride */
+    uint64_t partials;      /* [K][tiles][4] FP64 sums */
+    uint64_t stride;
+    uint32_t tiles;         /* ceil(2^n / 2048) */
+    uint32_t K;
+    uint32_t unit_begin;    /* units: tiles * K */
+    uint32_t pad;
+} qgtl_inject_item_t;
+
+typedef struct {
+    uint64_t partials;      /* [K][tiles][4] */
+    uint64_t out;           /* [K][4] */
+    uint32_t tiles, K;
+    uint32_t unit_begin;    /* units: K */
+    uint32_t pad;
+} qgtl_reduce_item_t;
+
+typedef struct {
+    uint64_t D;             /* P states of N complex64 */
+    uint64_t partials;      /* [nchunks][T][128 * 128] fp32, T = nt (nt + 1) / 2 */
+    uint64_t N;
+    uint64_t chunk_len;     /* amplitudes per chunk, multiple of 32 */
+    int32_t P, nt, nchunks; /* nchunks: multiple of 8 */
+    uint32_t unit_begin;    /* multiple of 8; units: T * nchunks */
+} qgtl_gram_item_t;
+
+typedef struct {
+    uint64_t partials;
+    uint64_t C;             /* (nt * 128)^2 FP64 */
+    int32_t nt, nchunks;
+    uint32_t unit_begin;    /* units: T * 64 (256 elements each) */
+    uint32_t pad;
+} qgtl_gram_reduce_item_t;
+
+typedef struct {
+    uint64_t C, inj, Q;     /* Q: P x P complex FP64 in D-slot order */
+    int32_t R, P;
+    uint32_t unit_begin;    /* units: ceil(P * P / 256) */
+    uint32_t pad;
+} qgtl_finalize_item_t;
+
+typedef struct {
+    uint64_t psi;           /* set to |0...0> */
+    uint32_t N;
+    uint32_t unit_begin;    /* units: ceil(N / 2048) */
+} qgtl_fill_item_t;
+
+/* grid <= 0: one workgroup per unit (capped at 2^31 - 1) */
+int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t units, int grid);
+int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid);
+int qgtl_hip_inject_grouped(const qgtl_inject_item_t *items, int nitems, uint32_t units, int grid);
+int qgtl_hip_reduce_grouped(const qgtl_reduce_item_t *items, int nitems, uint32_t units, int grid);
+/* grid: multiple of 8 (XCD affinity of a chunk's tiles) */
+int qgtl_hip_gram_grouped(const qgtl_gram_item_t *items, int nitems, uint32_t units, int grid, int terms);
+int qgtl_hip_gram_reduce_grouped(const qgtl_gram_reduce_item_t *items, int nitems, uint32_t units, int grid);
+int qgtl_hip_finalize_grouped(const qgtl_finalize_item_t *items, int nitems, uint32_t units, int grid);
+
+/* ---- graphs: capture the library stream's launches and replay them ---- */
+typedef struct qgtl_hip_graph_s *qgtl_hip_graph_t;
+int qgtl_hip_capture_begin(void);
+int qgtl_hip_capture_end(qgtl_hip_graph_t *graph);       /* instantiates */
+int qgtl_hip_graph_launch(qgtl_hip_graph_t graph);
+void qgtl_hip_graph_destroy(qgtl_hip_graph_t graph);
+int qgtl_hip_cu_count(void);
+
 /* ---- runtime ---- */
 int qgtl_hip_init(int device);
 int qgtl_hip_device_count(void);
@@ -109,9 +193,13 @@ int qgtl_hip_dot_states_c64(const void *D, size_t stride, int K, const void *v, 
 /* out[0] = energy, out[1 + k] = gradient (from reduced dot sums) */
 int qgtl_hip_energy_gradient_finalize(const double *dk_h, const double *psi_h, int P, double *out);
 
-/* ---- Gram (mixed): C = Z Z^T with Z rows (Re d_s, Im d_s), split-BF16x3 on MFMA ---- */
+/* ---- Gram (mixed): C = Z Z^T with Z rows (Re d_s, Im d_s), split products on MFMA ---- */
 int qgtl_hip_gram_tiles(int P);                                          /* upper 128x128 tiles */
-/* terms = 3: split-BF16x3; terms = 6: exact three-piece split, six products */
+/* terms = 3 / 4: two bf16 pieces, 3 or 4 products; 6: exact three-piece bf16
+ * split, six products; QGTL_GRAM_F16X3 / X4: two fp16 pieces of 2^15 x
+ * (C is then scaled by 2^30), 3 or 4 products */
+#define QGTL_GRAM_F16X3 33
+#define QGTL_GRAM_F16X4 34
 int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms);
 int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C);   /* C: R x R tiles, FP64 */
 /* Q = normalised G - a a^H from C and injection sums (P x 4 doubles). */

@@ -64,7 +64,9 @@ typedef struct {
 typedef struct {
     int32_t chunk_states;     /* derivative states propagated together (0 = auto) */
     int32_t gram_chunks;      /* split-K factor of the Gram (0 = auto) */
-    int32_t gram_variant;     /* split-BF16 product terms: 3, 4 or 6 (exact 3-piece split); 0: by size */
+    int32_t gram_variant;     /* Gram operand split: 3 / 4 = two bf16 pieces with 3 / 4 products, 6 = exact
+                                 three-piece bf16 split, 33 / 34 = two fp16 pieces of 2^15 x with 3 / 4
+                                 products; 0 = default (33) */
     int32_t reserved[5];
 } qgtl_tune_t;
 
@@ -98,6 +100,65 @@ void qgtl_plan_destroy(qgtl_plan_t *plan);
 
 /* Release every cached device buffer of the calling process. */
 void qgtl_release_cache(void);
+
+/* ---- batched QGTs (mixed mode) ----
+ * Many independent circuits (uneven qubit counts and depths) run together:
+ * each step of the sweep (block b, pass j), the injections, the Gram, its
+ * split-K reduction and the FP64 correction is ONE grouped launch over every
+ * instance, with a persistent grid that loops over the work units; the whole
+ * sequence is captured once in a HIP graph and replayed per execution.
+ * QGTL evaluates one circuit and one matrix element at a time
+ * (src/quantum_geometric/core/quantum_geometric_curvature.c:289-320); the
+ * batch has no counterpart there.
+ *
+ * ops: the instances' gate lists one after another (nops[i] each); every
+ * instance needs n in 1..30, at least one parameter, and M-stages that are
+ * CNOT rings when n > 13 (any CNOT/SWAP list when n <= 13).  Q output: the
+ * instances' P_i x P_i complex128 matrices one after another (parameter
+ * order), at complex offset qgtl_batch_q_offset(b, i). */
+typedef struct qgtl_batch qgtl_batch_t;
+
+typedef struct {
+    int32_t pass_grid;        /* workgroups of the sweep launches (0: 2 per CU; -1: one per unit) */
+    int32_t inject_grid;      /* injection launches (0: 8 per CU; -1: one per unit) */
+    int32_t gram_grid;        /* Gram launches (0: 2 per CU; -1: one per unit); rounded to a multiple of 8 */
+    int32_t gram_chunk_len;   /* amplitudes per split-K chunk of a Gram (0: 16384) */
+    int32_t pass_variant;     /* 0: register phases where they apply; 1: LDS gate groups only */
+    int32_t no_graph;         /* 1: issue the launches directly instead of replaying the graph */
+    int32_t reserved[2];
+} qgtl_batch_tune_t;
+
+typedef struct {
+    double wall_ms;           /* host wall time of qgtl_batch_execute */
+    double device_ms;         /* HIP-event time of the launch sequence (graph replay) */
+    double upload_ms;         /* angle-dependent descriptors to the GPU (host wall) */
+    double download_ms;       /* Q to the host and reordering (host wall) */
+    int64_t launches;         /* kernel launches in the sequence */
+    int64_t units;            /* work units over all launches */
+    double sweep_bytes;       /* algorithmic sweep bytes (16 B per amplitude per state-pass) */
+    double gram_flops;        /* sum of 4 N P (P + 1) */
+    int32_t instances;
+    int32_t graph;            /* 1 when the graph was replayed */
+} qgtl_batch_stats_t;
+
+qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const qgtl_op_t *ops,
+                                const qgtl_batch_tune_t *tune);
+int qgtl_batch_count(const qgtl_batch_t *b);
+int qgtl_batch_num_params(const qgtl_batch_t *b, int i);
+size_t qgtl_batch_q_offset(const qgtl_batch_t *b, int i);   /* complex elements before instance i */
+size_t qgtl_batch_q_size(const qgtl_batch_t *b);            /* sum of P_i^2 */
+size_t qgtl_batch_device_bytes(const qgtl_batch_t *b);
+/* angles: one per op, same layout as the creation ops (non-rotation entries ignored) */
+int qgtl_batch_set_angles(qgtl_batch_t *b, const double *angles);
+int qgtl_batch_execute(qgtl_batch_t *b, double *Q, qgtl_batch_stats_t *stats);
+int qgtl_batch_set_tune(qgtl_batch_t *b, const qgtl_batch_tune_t *tune);
+int qgtl_batch_get_tune(const qgtl_batch_t *b, qgtl_batch_tune_t *tune);
+/* Autotuner: times candidate launch shapes on this batch (coordinate
+ * descent, median of `reps` replays each), keeps the fastest in the batch
+ * and, when json_path is given, writes it there as a JSON object. */
+int qgtl_batch_autotune(qgtl_batch_t *b, int reps, const char *json_path, qgtl_batch_tune_t *best);
+int qgtl_batch_tune_load(const char *json_path, qgtl_batch_tune_t *tune);
+void qgtl_batch_destroy(qgtl_batch_t *b);
 
 /* ---- Hamiltonians: sums of Pauli strings ----
  * term = coeff * i^{ny} X^xmask Z^zmask with ny = popcount(xmask & zmask)

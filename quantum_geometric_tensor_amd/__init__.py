@@ -259,3 +259,116 @@ def hermitian_eig(A):
 def rel_err(Q, Qref):
     """Normwise relative error max|Q - Qref| / max|Qref|."""
     return float(np.max(np.abs(Q - Qref)) / np.max(np.abs(Qref)))
+
+
+# ---------------------------------------------------------------- batches
+class BatchTune(ctypes.Structure):
+    _fields_ = [("pass_grid", ctypes.c_int32), ("inject_grid", ctypes.c_int32), ("gram_grid", ctypes.c_int32),
+                ("gram_chunk_len", ctypes.c_int32), ("pass_variant", ctypes.c_int32), ("no_graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("wall_ms", ctypes.c_double), ("device_ms", ctypes.c_double), ("upload_ms", ctypes.c_double),
+                ("download_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("units", ctypes.c_int64),
+                ("sweep_bytes", ctypes.c_double), ("gram_flops", ctypes.c_double), ("instances", ctypes.c_int32),
+                ("graph", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def _batch_lib():
+    L = lib()
+    if not getattr(L, "_batch_bound", False):
+        vp, ip = ctypes.c_void_p, ctypes.c_int
+        L.qgtl_batch_create.argtypes = [ip, vp, vp, vp, ctypes.POINTER(BatchTune)]
+        L.qgtl_batch_create.restype = vp
+        L.qgtl_batch_num_params.argtypes = [vp, ip]
+        L.qgtl_batch_q_offset.argtypes = [vp, ip]
+        L.qgtl_batch_q_offset.restype = ctypes.c_size_t
+        L.qgtl_batch_q_size.argtypes = [vp]
+        L.qgtl_batch_q_size.restype = ctypes.c_size_t
+        L.qgtl_batch_device_bytes.argtypes = [vp]
+        L.qgtl_batch_device_bytes.restype = ctypes.c_size_t
+        L.qgtl_batch_set_angles.argtypes = [vp, vp]
+        L.qgtl_batch_execute.argtypes = [vp, vp, ctypes.POINTER(BatchStats)]
+        L.qgtl_batch_set_tune.argtypes = [vp, ctypes.POINTER(BatchTune)]
+        L.qgtl_batch_get_tune.argtypes = [vp, ctypes.POINTER(BatchTune)]
+        L.qgtl_batch_autotune.argtypes = [vp, ip, ctypes.c_char_p, ctypes.POINTER(BatchTune)]
+        L.qgtl_batch_tune_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(BatchTune)]
+        L.qgtl_batch_destroy.argtypes = [vp]
+        L._batch_bound = True
+    return L
+
+
+class Batch:
+    """Many independent circuits, executed together (grouped launches in one HIP graph).
+
+    circuits: list of (n, ops) with ops from hea_ops / make_ops.
+    """
+
+    def __init__(self, circuits, tune=None):
+        L = _batch_lib()
+        self.n = np.array([c[0] for c in circuits], np.int32)
+        self.nops = np.array([len(c[1]) for c in circuits], np.int32)
+        self.ops = np.ascontiguousarray(np.concatenate([np.asarray(c[1], OP_DTYPE) for c in circuits]), OP_DTYPE)
+        t = tune if tune is not None else BatchTune()
+        h = L.qgtl_batch_create(len(circuits), _p(self.n), _p(self.nops), _p(self.ops), ctypes.byref(t))
+        if not h:
+            raise QGTLError(-1, L.qgtl_last_error().decode(errors="replace"))
+        self._h = h
+        self.count = len(circuits)
+        self.P = [L.qgtl_batch_num_params(h, i) for i in range(self.count)]
+        self.q_off = [L.qgtl_batch_q_offset(h, i) for i in range(self.count)]
+        self.q_size = L.qgtl_batch_q_size(h)
+        self.device_bytes = L.qgtl_batch_device_bytes(h)
+
+    def set_angles(self, per_op_angles):
+        """per_op_angles: one array per instance, one angle per op (as in the ops' angle field)."""
+        a = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in per_op_angles]))
+        if a.size != self.ops.size:
+            raise ValueError("one angle per op of every instance")
+        _check(_batch_lib().qgtl_batch_set_angles(self._h, _p(a)))
+
+    def execute(self, want_q=True):
+        st = BatchStats()
+        buf = np.zeros(self.q_size, np.complex128) if want_q else None
+        _check(_batch_lib().qgtl_batch_execute(self._h, _p(buf) if want_q else None, ctypes.byref(st)))
+        Qs = None
+        if want_q:
+            Qs = [buf[o:o + P * P].reshape(P, P) for o, P in zip(self.q_off, self.P)]
+        return Qs, st.as_dict()
+
+    def autotune(self, reps=3, path=None):
+        best = BatchTune()
+        _check(_batch_lib().qgtl_batch_autotune(self._h, reps, path.encode() if path else None, ctypes.byref(best)))
+        return best
+
+    def set_tune(self, tune):
+        _check(_batch_lib().qgtl_batch_set_tune(self._h, ctypes.byref(tune)))
+
+    def get_tune(self):
+        t = BatchTune()
+        _check(_batch_lib().qgtl_batch_get_tune(self._h, ctypes.byref(t)))
+        return t
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _batch_lib().qgtl_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def load_batch_tune(path):
+    t = BatchTune()
+    _check(_batch_lib().qgtl_batch_tune_load(path.encode(), ctypes.byref(t)))
+    return t

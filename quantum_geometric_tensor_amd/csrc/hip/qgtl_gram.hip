@@ -9,7 +9,14 @@
 //   TERMS = 3: x = h + l (l = rn(x - h)), keep hh + hl + lh;
 //   TERMS = 4: the same split, keep hh + hl + lh + ll;
 //   TERMS = 6: x = h + m + l exactly (3 x 8 = 24 significand bits), keep
-//              hh + hm + mh + hl + lh + mm (fp32-level products).
+//              hh + hm + mh + hl + lh + mm (fp32-level products);
+//   TERMS = 33 / 34: fp16 pieces of 2^15 x (11 significand bits each, so
+//              h + l carries 22 bits; |x| <= 1 keeps 2^15 x below the fp16
+//              maximum), keep hh + hl + lh (+ ll), v_mfma_f32_32x32x16_f16.
+//              The representation error is <= 2^-22 |x| per value, so a
+//              product sum is off by <= 2^-21 sum |x y| even when the errors
+//              are fully correlated; the 2^30 scale of C cancels in the
+//              normalisation of the finalize step.
 // Accumulation is two-level: the MFMA accumulators cover kInner amplitudes
 // and are then added into an fp32 outer sum, so rounding does not grow with
 // the chunk length.  K (the amplitude index) is split into chunks; each
@@ -25,6 +32,7 @@
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
@@ -45,6 +53,35 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t &hi, uint32_t 
     hi = ha | (hb << 16);
     lo = bf16_rn_bits(ra) | (bf16_rn_bits(rb) << 16);
 }
+
+constexpr float kF16Scale = 32768.0f;
+
+__device__ __forceinline__ uint32_t f16_bits(_Float16 h) { return (uint32_t)__builtin_bit_cast(uint16_t, h); }
+
+__device__ __forceinline__ void split2_f16(float a, float b, uint32_t &hi, uint32_t &lo) {
+    const float sa = a * kF16Scale, sb = b * kF16Scale;
+    const _Float16 ha = (_Float16)sa, hb = (_Float16)sb;
+    const _Float16 la = (_Float16)(sa - (float)ha), lb = (_Float16)(sb - (float)hb);
+    hi = f16_bits(ha) | (f16_bits(hb) << 16);
+    lo = f16_bits(la) | (f16_bits(lb) << 16);
+}
+
+constexpr bool is_f16(int terms) { return terms >= 30; }
+constexpr int product_terms(int terms) { return terms >= 30 ? terms - 30 : terms; }
+
+template <bool F16> struct Frag;
+template <> struct Frag<false> {
+    typedef bf16x8 type;
+    static __device__ __forceinline__ f32x16 mma(type a, type b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Frag<true> {
+    typedef f16x8 type;
+    static __device__ __forceinline__ f32x16 mma(type a, type b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
 
 __device__ __forceinline__ void split3(float a, float b, uint32_t &hi, uint32_t &mid, uint32_t &lo) {
     const uint32_t ha = bf16_rn_bits(a), hb = bf16_rn_bits(b);
@@ -67,7 +104,7 @@ __device__ __forceinline__ void tile_coords(int t, int nt, int &ti, int &tj) {
 
 // Load 64 states x kKS amplitudes of D starting at state s0, amplitude k0,
 // into LDS rows (2s, 2s+1) = (Re, Im) as S bf16 pieces (S = 2 or 3 arrays).
-template <int S>
+template <int S, bool F16>
 __device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, size_t N, int s0, size_t k0,
                                            uint32_t *const *rows) {
 #pragma unroll
@@ -80,8 +117,13 @@ __device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, s
         const int wr = (2 * sl) * (kRow / 2) + pair, wi = (2 * sl + 1) * (kRow / 2) + pair;
         if (S == 2) {
             uint32_t hr, lr, hm, lm;
-            split2(v.x, v.z, hr, lr);  // Re of amplitudes 2pair, 2pair+1
-            split2(v.y, v.w, hm, lm);  // Im
+            if (F16) {
+                split2_f16(v.x, v.z, hr, lr);
+                split2_f16(v.y, v.w, hm, lm);
+            } else {
+                split2(v.x, v.z, hr, lr);  // Re of amplitudes 2pair, 2pair+1
+                split2(v.y, v.w, hm, lm);  // Im
+            }
             rows[0][wr] = hr; rows[1][wr] = lr;
             rows[0][wi] = hm; rows[1][wi] = lm;
         } else {
@@ -95,14 +137,23 @@ __device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, s
 }
 
 template <int TERMS>
-__global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
-                                                              int nt, int nchunks, size_t chunk_len,
-                                                              float *__restrict__ partials) {
-    constexpr int S = (TERMS == 6) ? 3 : 2;            // bf16 pieces per value
-    __shared__ uint32_t lds[2 * S][kTile * kRow / 2];   // A pieces, then B pieces
+struct GramLds {
+    static constexpr int S = (TERMS == 6) ? 3 : 2;     // bf16 pieces per value
+    uint32_t rows[2 * S][kTile * kRow / 2];            // A pieces, then B pieces
+};
+
+// Unit `bid` of one Gram: chunk c's tiles are the units with bid % 8 == c % 8.
+template <int TERMS>
+__device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, size_t N, int nt, int nchunks,
+                                          size_t chunk_len, float *__restrict__ partials, int bid,
+                                          GramLds<TERMS> &sm) {
+    constexpr int S = GramLds<TERMS>::S;
+    constexpr bool F16 = is_f16(TERMS);
+    constexpr int PT = product_terms(TERMS);
+    typedef typename Frag<F16>::type frag_t;
+    auto &lds = sm.rows;
 
     const int T = upper_tiles(nt);
-    const int bid = blockIdx.x;
     const int xcd = bid % kXcd, slot = bid / kXcd;
     const int tile = slot % T, chunk = (slot / T) * kXcd + xcd;
     if (chunk >= nchunks) return;
@@ -133,37 +184,37 @@ __global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__res
     for (size_t kb = kbeg; kb < kend; kb += kInner) {
         const size_t ke = (kb + kInner < kend) ? kb + kInner : kend;
         for (size_t k0 = kb; k0 < ke; k0 += kKS) {
-            stage_rows<S>(D, P, N, ti * (kTile / 2), k0, arows);
-            if (!diag) stage_rows<S>(D, P, N, tj * (kTile / 2), k0, brows);
+            stage_rows<S, F16>(D, P, N, ti * (kTile / 2), k0, arows);
+            if (!diag) stage_rows<S, F16>(D, P, N, tj * (kTile / 2), k0, brows);
             __syncthreads();
 #pragma unroll 1
             for (int kk = 0; kk < kKS / 16; kk++) {
                 const int koff = kk * 16 + 8 * (lane >> 5);
-                bf16x8 fa[S][2], fb[S][2];
+                frag_t fa[S][2], fb[S][2];
 #pragma unroll
                 for (int p = 0; p < S; p++)
 #pragma unroll
                     for (int b = 0; b < 2; b++) {
                         const int ra = wm * 64 + b * 32 + (lane & 31);
                         const int rb = wn * 64 + b * 32 + (lane & 31);
-                        fa[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(arows[p]) + ra * kRow + koff);
-                        fb[p][b] = *reinterpret_cast<const bf16x8 *>(reinterpret_cast<const __bf16 *>(brows[p]) + rb * kRow + koff);
+                        fa[p][b] = *reinterpret_cast<const frag_t *>(reinterpret_cast<const uint16_t *>(arows[p]) + ra * kRow + koff);
+                        fb[p][b] = *reinterpret_cast<const frag_t *>(reinterpret_cast<const uint16_t *>(brows[p]) + rb * kRow + koff);
                     }
 #pragma unroll
                 for (int a = 0; a < 2; a++)
 #pragma unroll
                     for (int b = 0; b < 2; b++) {
                         // smallest terms first
-                        if (TERMS == 6) {
-                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
-                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][a], fb[0][b], acc[a][b], 0, 0, 0);
-                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[2][b], acc[a][b], 0, 0, 0);
+                        if (PT == 6) {
+                            acc[a][b] = Frag<F16>::mma(fa[1][a], fb[1][b], acc[a][b]);
+                            acc[a][b] = Frag<F16>::mma(fa[2 % S][a], fb[0][b], acc[a][b]);
+                            acc[a][b] = Frag<F16>::mma(fa[0][a], fb[2 % S][b], acc[a][b]);
                         }
-                        if (TERMS == 4)
-                            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[1][b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][a], fb[0][b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[1][b], acc[a][b], 0, 0, 0);
-                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][a], fb[0][b], acc[a][b], 0, 0, 0);
+                        if (PT == 4)
+                            acc[a][b] = Frag<F16>::mma(fa[1][a], fb[1][b], acc[a][b]);
+                        acc[a][b] = Frag<F16>::mma(fa[1][a], fb[0][b], acc[a][b]);
+                        acc[a][b] = Frag<F16>::mma(fa[0][a], fb[1][b], acc[a][b]);
+                        acc[a][b] = Frag<F16>::mma(fa[0][a], fb[0][b], acc[a][b]);
                     }
             }
             __syncthreads();
@@ -192,34 +243,69 @@ __global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__res
         }
 }
 
+template <int TERMS>
+__global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
+                                                              int nt, int nchunks, size_t chunk_len,
+                                                              float *__restrict__ partials) {
+    __shared__ GramLds<TERMS> sm;
+    gram_unit<TERMS>(D, P, N, nt, nchunks, chunk_len, partials, blockIdx.x, sm);
+}
+
+// Grouped: grid and every unit_begin are multiples of 8, so u % 8 == local % 8
+// and a chunk's tiles still share one XCD.
+template <int TERMS>
+__global__ __launch_bounds__(kThreads, (TERMS == 6) ? 1 : 2) void gram_grouped_kernel(const qgtl_gram_item_t *__restrict__ items,
+                                                                int nitems, uint32_t units) {
+    __shared__ GramLds<TERMS> sm;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_gram_item_t it = items[qgtl::find_item(items, nitems, u)];
+        __syncthreads();
+        gram_unit<TERMS>(reinterpret_cast<const float *>(it.D), it.P, (size_t)it.N, it.nt, it.nchunks,
+                         (size_t)it.chunk_len, reinterpret_cast<float *>(it.partials), (int)(u - it.unit_begin), sm);
+    }
+}
+
 // C[ti*128 + r][tj*128 + c] = sum over chunks (fixed order, FP64); C is R x R.
-__global__ __launch_bounds__(kThreads) void gram_reduce_kernel(const float *__restrict__ partials, int nt,
-                                                               int nchunks, double *__restrict__ C) {
+__device__ __forceinline__ void gram_reduce_elem(const float *__restrict__ partials, int nt, int nchunks,
+                                                 double *__restrict__ C, size_t idx) {
     const int T = upper_tiles(nt);
     const int R = nt * kTile;
-    const size_t total = (size_t)T * kTile * kTile;
+    const int tile = (int)(idx / (kTile * kTile));
+    const int e = (int)(idx % (kTile * kTile));
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s += (double)partials[((size_t)c * T + tile) * (kTile * kTile) + e];
+    int ti, tj;
+    tile_coords(tile, nt, ti, tj);
+    C[(size_t)(ti * kTile + e / kTile) * R + tj * kTile + (e % kTile)] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void gram_reduce_kernel(const float *__restrict__ partials, int nt,
+                                                               int nchunks, double *__restrict__ C) {
+    const size_t total = (size_t)upper_tiles(nt) * kTile * kTile;
     for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * kThreads) {
-        const int tile = (int)(idx / (kTile * kTile));
-        const int e = (int)(idx % (kTile * kTile));
-        double s = 0.0;
-        for (int c = 0; c < nchunks; c++) s += (double)partials[((size_t)c * T + tile) * (kTile * kTile) + e];
-        int ti, tj;
-        tile_coords(tile, nt, ti, tj);
-        C[(size_t)(ti * kTile + e / kTile) * R + tj * kTile + (e % kTile)] = s;
+         idx += (size_t)gridDim.x * kThreads)
+        gram_reduce_elem(partials, nt, nchunks, C, idx);
+}
+
+__global__ __launch_bounds__(kThreads) void gram_reduce_grouped_kernel(const qgtl_gram_reduce_item_t *__restrict__ items,
+                                                                       int nitems, uint32_t units) {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_gram_reduce_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const size_t idx = (size_t)(u - it.unit_begin) * kThreads + threadIdx.x;
+        gram_reduce_elem(reinterpret_cast<const float *>(it.partials), it.nt, it.nchunks,
+                         reinterpret_cast<double *>(it.C), idx);
     }
 }
 
 // Q_kl = G_kl / (4 sqrt(G_kk G_ll)) - a_k conj(a_l), a_k = <d_k|psi> / (2 sqrt(|d_k|^2 |psi|^2)).
 // The normalisation applies the exact identities ||d_k psi|| = 1/2 and
 // ||psi|| = 1 to remove the norm drift of the complex64 sweeps.
-__global__ __launch_bounds__(kThreads) void qgt_finalize_kernel(const double *__restrict__ C, int R,
-                                                                const double *__restrict__ inj, int P,
-                                                                double *__restrict__ Q) {
-    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < (size_t)P * P;
-         idx += (size_t)gridDim.x * kThreads) {
+__device__ __forceinline__ void qgt_finalize_elem(const double *__restrict__ C, int R,
+                                                  const double *__restrict__ inj, int P,
+                                                  double *__restrict__ Q, size_t idx) {
+    {
         const int k = (int)(idx / P), l = (int)(idx % P);
-        if (k > l) continue;
+        if (k > l) return;
         const double gre = C[(size_t)(2 * k) * R + 2 * l] + C[(size_t)(2 * k + 1) * R + 2 * l + 1];
         const double gim = C[(size_t)(2 * k) * R + 2 * l + 1] - C[(size_t)(2 * k + 1) * R + 2 * l];
         const double gkk = C[(size_t)(2 * k) * R + 2 * k] + C[(size_t)(2 * k + 1) * R + 2 * k + 1];
@@ -238,6 +324,25 @@ __global__ __launch_bounds__(kThreads) void qgt_finalize_kernel(const double *__
             Q[2 * ((size_t)l * P + k)] = qr;
             Q[2 * ((size_t)l * P + k) + 1] = -qi;
         }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void qgt_finalize_kernel(const double *__restrict__ C, int R,
+                                                                const double *__restrict__ inj, int P,
+                                                                double *__restrict__ Q) {
+    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < (size_t)P * P;
+         idx += (size_t)gridDim.x * kThreads)
+        qgt_finalize_elem(C, R, inj, P, Q, idx);
+}
+
+__global__ __launch_bounds__(kThreads) void finalize_grouped_kernel(const qgtl_finalize_item_t *__restrict__ items,
+                                                                    int nitems, uint32_t units) {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_finalize_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const size_t idx = (size_t)(u - it.unit_begin) * kThreads + threadIdx.x;
+        if (idx < (size_t)it.P * it.P)
+            qgt_finalize_elem(reinterpret_cast<const double *>(it.C), it.R, reinterpret_cast<const double *>(it.inj),
+                              it.P, reinterpret_cast<double *>(it.Q), idx);
     }
 }
 
@@ -283,6 +388,10 @@ int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nc
     const dim3 grid((unsigned)(T * nchunks)), block(kThreads);
     if (terms == 6)
         hipLaunchKernelGGL(gram_split_kernel<6>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+    else if (terms == QGTL_GRAM_F16X3)
+        hipLaunchKernelGGL(gram_split_kernel<QGTL_GRAM_F16X3>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
+    else if (terms == QGTL_GRAM_F16X4)
+        hipLaunchKernelGGL(gram_split_kernel<QGTL_GRAM_F16X4>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     else if (terms == 4)
         hipLaunchKernelGGL(gram_split_kernel<4>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     else
@@ -307,6 +416,46 @@ int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q) 
     if (grid > 4096) grid = 4096;
     hipLaunchKernelGGL(qgt_finalize_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), C, nt * kTile, inj, P, Q);
     QGTL_LAUNCH_CHECK("qgt_finalize");
+    return 0;
+}
+
+static unsigned grouped_grid(uint32_t units, int grid) {
+    if (grid <= 0 || (uint32_t)grid > units) return units;
+    return (unsigned)grid;
+}
+
+int qgtl_hip_gram_grouped(const qgtl_gram_item_t *items, int nitems, uint32_t units, int grid, int terms) {
+    if (!items || nitems < 1 || units < 1 || units % kXcd != 0)
+        return qgtl::set_error_msg(-118, "qgtl_hip_gram_grouped: empty items or units not a multiple of 8");
+    unsigned g = grouped_grid(units, grid);
+    g = (g + kXcd - 1) / kXcd * kXcd;
+    if (terms == 6)
+        hipLaunchKernelGGL(gram_grouped_kernel<6>, dim3(g), dim3(kThreads), 0, qgtl::stream(), items, nitems, units);
+    else if (terms == QGTL_GRAM_F16X3)
+        hipLaunchKernelGGL(gram_grouped_kernel<QGTL_GRAM_F16X3>, dim3(g), dim3(kThreads), 0, qgtl::stream(), items, nitems, units);
+    else if (terms == QGTL_GRAM_F16X4)
+        hipLaunchKernelGGL(gram_grouped_kernel<QGTL_GRAM_F16X4>, dim3(g), dim3(kThreads), 0, qgtl::stream(), items, nitems, units);
+    else if (terms == 4)
+        hipLaunchKernelGGL(gram_grouped_kernel<4>, dim3(g), dim3(kThreads), 0, qgtl::stream(), items, nitems, units);
+    else
+        hipLaunchKernelGGL(gram_grouped_kernel<3>, dim3(g), dim3(kThreads), 0, qgtl::stream(), items, nitems, units);
+    QGTL_LAUNCH_CHECK("gram_grouped");
+    return 0;
+}
+
+int qgtl_hip_gram_reduce_grouped(const qgtl_gram_reduce_item_t *items, int nitems, uint32_t units, int grid) {
+    if (!items || nitems < 1 || units < 1) return qgtl::set_error_msg(-118, "gram_reduce_grouped: empty item list");
+    hipLaunchKernelGGL(gram_reduce_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("gram_reduce_grouped");
+    return 0;
+}
+
+int qgtl_hip_finalize_grouped(const qgtl_finalize_item_t *items, int nitems, uint32_t units, int grid) {
+    if (!items || nitems < 1 || units < 1) return qgtl::set_error_msg(-118, "finalize_grouped: empty item list");
+    hipLaunchKernelGGL(finalize_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("finalize_grouped");
     return 0;
 }
 

@@ -10,6 +10,19 @@ hipStream_t stream();
 int set_error(hipError_t e, const char *what);
 int set_error_msg(int code, const char *msg);
 
+// Grouped launches: the item whose unit range holds u, i.e. the largest i
+// with items[i].unit_begin <= u (items sorted, items[0].unit_begin = 0).
+template <typename Item>
+__device__ __forceinline__ int find_item(const Item *__restrict__ items, int nitems, uint32_t u) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].unit_begin <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 }  // namespace qgtl
 
 // Map a HIP status to the library's error codes and remember the message.

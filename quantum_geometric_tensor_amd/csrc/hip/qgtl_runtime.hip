@@ -122,4 +122,47 @@ int qgtl_hip_event_destroy(qgtl_hip_event_t ev) {
     return 0;
 }
 
+struct qgtl_hip_graph_s {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+int qgtl_hip_capture_begin(void) {
+    QGTL_HIP_CHECK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+
+int qgtl_hip_capture_end(qgtl_hip_graph_t *out) {
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    QGTL_HIP_CHECK(hipStreamEndCapture(g_stream, &g));
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return qgtl::set_error(e, "hipGraphInstantiate");
+    }
+    *out = new qgtl_hip_graph_s{g, x};
+    return 0;
+}
+
+int qgtl_hip_graph_launch(qgtl_hip_graph_t graph) {
+    QGTL_HIP_CHECK(hipGraphLaunch(graph->exec, g_stream));
+    return 0;
+}
+
+void qgtl_hip_graph_destroy(qgtl_hip_graph_t graph) {
+    if (!graph) return;
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
+    delete graph;
+}
+
+int qgtl_hip_cu_count(void) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return cus;
+}
+
 }  // extern "C"

@@ -87,17 +87,12 @@ __device__ __forceinline__ uint32_t ring_local_inverse(uint32_t y, int m, uint32
     return rest | (x1 << 1) | x0;
 }
 
-template <int TB>
-__global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc_t d,
-                                                            const float2 *__restrict__ src_all,
-                                                            float2 *dst_all, size_t stride) {
-    __shared__ float2 tile[(1 << TB) + (1 << TB) / 8];
-    __shared__ uint32_t hi_off[1 << kHiTableBits];
+constexpr int kLdsTileElems = (1 << QGTL_TILE_BITS_MAX) + (1 << QGTL_TILE_BITS_MAX) / 8;
 
+// One tile r of one state: load, gate groups through LDS, store.
+__device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
+                                              float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off) {
     const int m = d.m, nlow = d.nlow, nhi = m - nlow;
-    const uint32_t r = blockIdx.x;
-    const float2 *src = src_all + (size_t)blockIdx.y * stride;
-    float2 *dst = dst_all + (size_t)blockIdx.y * stride;
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -175,6 +170,15 @@ __global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc
             *reinterpret_cast<float4 *>(dst + 2 * p) = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
         }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc_t d,
+                                                            const float2 *__restrict__ src_all,
+                                                            float2 *dst_all, size_t stride) {
+    __shared__ float2 tile[kLdsTileElems];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+    pass_lds_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
+                  tile, hi_off);
 }
 
 // ---------------------------------------------------------------------------
@@ -291,17 +295,12 @@ struct RegLoop<0> {
                                                              const float2 (&)[kRegs]) {}
 };
 
-__global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
-                                                                 const float2 *__restrict__ src_all,
-                                                                 float2 *dst_all, size_t stride) {
-    __shared__ float2 tile[(1 << kRT) + (1 << kRT) / 32];
-    __shared__ uint32_t hi_off[1 << kHiTableBits];
+constexpr int kRegTileElems = (1 << kRT) + (1 << kRT) / 32;
 
+__device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
+                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off) {
     const int nlow = d.nlow, nhi = kRT - nlow;
-    const uint32_t r = blockIdx.x;
     const uint32_t t = threadIdx.x;
-    const float2 *src = src_all + (size_t)blockIdx.y * stride;
-    float2 *dst = dst_all + (size_t)blockIdx.y * stride;
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -362,6 +361,15 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
     }
 }
 
+__global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
+                                                                 const float2 *__restrict__ src_all,
+                                                                 float2 *dst_all, size_t stride) {
+    __shared__ float2 tile[kRegTileElems];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+    pass_regs_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
+                   tile, hi_off);
+}
+
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
 // FP64 per-tile sums of <d|psi>, |d|^2 and |psi|^2.
 constexpr int kInjectPerThread = 8;
@@ -373,20 +381,18 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_desc_t d,
-                                                              const float2 *__restrict__ psi,
-                                                              float2 *dst_all, size_t stride,
-                                                              double *partials) {
-    const int k = blockIdx.y;
+// Tile `tile` of derivative state k; its four sums go to out[0..3].
+__device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const float2 *__restrict__ psi,
+                                            float2 *dst, int k, uint32_t tile, double *out,
+                                            double (*red)[4]) {
     const uint32_t N = 1u << d.n;
     const uint32_t v = d.v[k], w = d.w[k];
     const float2 a00 = make_float2(d.A[k][0].re, d.A[k][0].im), a01 = make_float2(d.A[k][1].re, d.A[k][1].im);
     const float2 a10 = make_float2(d.A[k][2].re, d.A[k][2].im), a11 = make_float2(d.A[k][3].re, d.A[k][3].im);
-    float2 *dst = dst_all + (size_t)k * stride;
     double sr = 0.0, si = 0.0, sd = 0.0, sp = 0.0;
 #pragma unroll
     for (int i = 0; i < kInjectPerThread; i++) {
-        const uint32_t y = blockIdx.x * kInjectTile + i * kThreads + threadIdx.x;
+        const uint32_t y = tile * kInjectTile + i * kThreads + threadIdx.x;
         if (y < N) {
             const float2 a = psi[y], b = psi[y ^ v];
             const bool s = (__popc(y & w) & 1) != 0;
@@ -400,7 +406,6 @@ __global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_
             sp += pr * pr + pi * pi;
         }
     }
-    __shared__ double red[kThreads / 64][4];
     sr = wave_sum(sr); si = wave_sum(si); sd = wave_sum(sd); sp = wave_sum(sp);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -410,18 +415,27 @@ __global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_
     if (threadIdx.x < 4) {
         double s = 0.0;
         for (int wv = 0; wv < kThreads / 64; wv++) s += red[wv][threadIdx.x];
-        partials[((size_t)k * gridDim.x + blockIdx.x) * 4 + threadIdx.x] = s;
+        out[threadIdx.x] = s;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double *__restrict__ partials,
-                                                                   int tiles, double *out) {
-    const int k = blockIdx.x;
-    __shared__ double red[kThreads][4];
+__global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_desc_t d,
+                                                              const float2 *__restrict__ psi,
+                                                              float2 *dst_all, size_t stride,
+                                                              double *partials) {
+    __shared__ double red[kThreads / 64][4];
+    const int k = blockIdx.y;
+    inject_body(d, psi, dst_all + (size_t)k * stride, k, blockIdx.x,
+                partials + ((size_t)k * gridDim.x + blockIdx.x) * 4, red);
+}
+
+// out[c] = sum over t of pk[t * 4 + c] (fixed tree order, FP64).
+__device__ __forceinline__ void reduce_body(const double *__restrict__ pk, int tiles, double *out,
+                                            double (*red)[4]) {
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     for (int t = threadIdx.x; t < tiles; t += kThreads)
 #pragma unroll
-        for (int c = 0; c < 4; c++) s[c] += partials[((size_t)k * tiles + t) * 4 + c];
+        for (int c = 0; c < 4; c++) s[c] += pk[(size_t)t * 4 + c];
 #pragma unroll
     for (int c = 0; c < 4; c++) red[threadIdx.x][c] = s[c];
     __syncthreads();
@@ -431,7 +445,14 @@ __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double 
             for (int c = 0; c < 4; c++) red[threadIdx.x][c] += red[threadIdx.x + h][c];
         __syncthreads();
     }
-    if (threadIdx.x < 4) out[(size_t)k * 4 + threadIdx.x] = red[0][threadIdx.x];
+    if (threadIdx.x < 4) out[threadIdx.x] = red[0][threadIdx.x];
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double *__restrict__ partials,
+                                                                   int tiles, double *out) {
+    __shared__ double red[kThreads][4];
+    const int k = blockIdx.x;
+    reduce_body(partials + (size_t)k * tiles * 4, tiles, out + (size_t)k * 4, red);
 }
 
 // H psi for H = sum_t c_t P_t, P_t = i^{ny} X^x Z^z (ny = popcount(x & z)):
@@ -543,6 +564,78 @@ __global__ void c64_to_c128_kernel(const float2 *src, double2 *dst, size_t count
         dst[i] = make_double2(src[i].x, src[i].y);
 }
 
+
+// ---------------------------------------------------------------------------
+// Grouped launches: the same work for many instances in one launch, with a
+// grid-stride loop over units (see qgtl_pass_item_t and friends).
+__global__ __launch_bounds__(kThreads) void fill_grouped_kernel(const qgtl_fill_item_t *__restrict__ items,
+                                                                int nitems, uint32_t units) {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_fill_item_t it = items[qgtl::find_item(items, nitems, u)];
+        float2 *psi = reinterpret_cast<float2 *>(it.psi);
+        const uint32_t base = (u - it.unit_begin) * kInjectTile;
+        for (uint32_t j = threadIdx.x; j < (uint32_t)kInjectTile; j += kThreads) {
+            const uint32_t y = base + j;
+            if (y < it.N) psi[y] = make_float2(y == 0 ? 1.f : 0.f, 0.f);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void pass_grouped_kernel(const qgtl_pass_item_t *__restrict__ items,
+                                                                int nitems, uint32_t units) {
+    __shared__ float2 tile[kLdsTileElems];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+    __shared__ qgtl_pass_desc_t sd;
+    static_assert(kLdsTileElems >= kRegTileElems, "LDS tile too small for the register pass");
+    static_assert(sizeof(qgtl_pass_desc_t) % 4 == 0, "descriptor copy works in words");
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_pass_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const uint32_t local = u - it.unit_begin;
+        const uint32_t r = local % it.ntiles, s = local / it.ntiles;
+        __syncthreads();   // the previous unit is done with LDS
+        const uint32_t *dw = reinterpret_cast<const uint32_t *>(it.desc);
+        uint32_t *sw = reinterpret_cast<uint32_t *>(&sd);
+        for (int w = threadIdx.x; w < (int)(sizeof(qgtl_pass_desc_t) / 4); w += kThreads) sw[w] = dw[w];
+        __syncthreads();
+        const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
+        float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
+        if (it.variant) pass_regs_body(sd, src, dst, r, tile, hi_off);
+        else pass_lds_body(sd, src, dst, r, tile, hi_off);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void inject_grouped_kernel(const qgtl_inject_item_t *__restrict__ items,
+                                                                  int nitems, uint32_t units) {
+    __shared__ double red[kThreads / 64][4];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_inject_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const uint32_t local = u - it.unit_begin;
+        const uint32_t t = local % it.tiles;
+        const int k = (int)(local / it.tiles);
+        __syncthreads();
+        const qgtl_inject_desc_t &d = *reinterpret_cast<const qgtl_inject_desc_t *>(it.desc);
+        inject_body(d, reinterpret_cast<const float2 *>(it.psi), reinterpret_cast<float2 *>(it.dst) + (size_t)k * it.stride,
+                    k, t, reinterpret_cast<double *>(it.partials) + ((size_t)k * it.tiles + t) * 4, red);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_grouped_kernel(const qgtl_reduce_item_t *__restrict__ items,
+                                                                  int nitems, uint32_t units) {
+    __shared__ double red[kThreads][4];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_reduce_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const uint32_t k = u - it.unit_begin;
+        __syncthreads();
+        reduce_body(reinterpret_cast<const double *>(it.partials) + (size_t)k * it.tiles * 4, (int)it.tiles,
+                    reinterpret_cast<double *>(it.out) + (size_t)k * 4, red);
+    }
+}
+
+unsigned grouped_grid(uint32_t units, int grid) {
+    if (grid <= 0 || (uint32_t)grid > units) return units;
+    return (unsigned)grid;
+}
+
 unsigned grid_for(size_t work, unsigned threads = 256, unsigned cap = 8192) {
     size_t g = (work + threads - 1) / threads;
     if (g < 1) g = 1;
@@ -553,6 +646,44 @@ unsigned grid_for(size_t work, unsigned threads = 256, unsigned cap = 8192) {
 }  // namespace
 
 extern "C" {
+
+#define QGTL_GROUPED_ARGS_OK(items, nitems, units)                                       \
+    do {                                                                                  \
+        if (!(items) || (nitems) < 1 || (units) < 1)                                      \
+            return qgtl::set_error_msg(-118, "grouped launch: empty item list");          \
+    } while (0)
+
+int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t units, int grid) {
+    QGTL_GROUPED_ARGS_OK(items, nitems, units);
+    hipLaunchKernelGGL(fill_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("fill_grouped");
+    return 0;
+}
+
+int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid) {
+    QGTL_GROUPED_ARGS_OK(items, nitems, units);
+    hipLaunchKernelGGL(pass_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("pass_grouped");
+    return 0;
+}
+
+int qgtl_hip_inject_grouped(const qgtl_inject_item_t *items, int nitems, uint32_t units, int grid) {
+    QGTL_GROUPED_ARGS_OK(items, nitems, units);
+    hipLaunchKernelGGL(inject_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("inject_grouped");
+    return 0;
+}
+
+int qgtl_hip_reduce_grouped(const qgtl_reduce_item_t *items, int nitems, uint32_t units, int grid) {
+    QGTL_GROUPED_ARGS_OK(items, nitems, units);
+    hipLaunchKernelGGL(reduce_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("reduce_grouped");
+    return 0;
+}
 
 int qgtl_hip_fill_zero_state_c64(void *psi, int n) {
     const uint32_t N = 1u << n;
@@ -582,7 +713,7 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
         hipLaunchKernelGGL(pass_c64_regs_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride);
     else
-        hipLaunchKernelGGL(pass_c64_kernel<QGTL_TILE_BITS_MAX>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+        hipLaunchKernelGGL(pass_c64_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride);
     QGTL_LAUNCH_CHECK("pass_c64");
     return 0;

@@ -311,14 +311,13 @@ static int gram_chunks(const struct qgtl_plan *p)
     return want;
 }
 
-/* Split-BF16x3 leaves a ~2.7e-6 bias (the dropped lo*lo product); x4 keeps
- * it and its remaining error is the random 2^-16 representation error of
- * the lo piece, which averages out over long states; short states use the
- * exact three-piece split (x6). */
+/* Split-BF16x3 leaves a ~2.7e-6 bias (the dropped lo*lo product) and x4
+ * a 2^-17 representation error that averages out only over unstructured
+ * states; the default is the two-piece fp16 split (see qgtl_internal.h). */
 static int gram_terms(const struct qgtl_plan *p)
 {
-    if (p->tune.gram_variant == 3 || p->tune.gram_variant == 4 || p->tune.gram_variant == 6) return p->tune.gram_variant;
-    return p->n >= QGTL_GRAM_X4_MIN_QUBITS ? 4 : 6;
+    if (qgtl_valid_gram_terms(p->tune.gram_variant)) return p->tune.gram_variant;
+    return qgtl_default_gram_terms(p->n);
 }
 
 static int alloc_mixed(struct qgtl_plan *p)

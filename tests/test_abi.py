@@ -86,3 +86,15 @@ def test_hea_builder_matches_oracle():
         a = Q.hea_ops(n, L, th)
         b = O.hea_ops(n, L, th)
         assert a.tobytes() == b.tobytes()
+
+
+def test_batch_validation_errors():
+    """Batch creation checks every instance on the host before touching the GPU."""
+    ring_free = Q.make_ops([("RY", 0, -1, 0, 0.3), ("CNOT", 0, 1, -1, 0.0), ("RY", 1, -1, 1, 0.2)])
+    with pytest.raises(Q.QGTLError, match="ring"):
+        Q.Batch([(3, Q.hea_ops(3, 1, np.ones(6))), (14, ring_free)])
+    no_params = Q.make_ops([("H", 0, -1, -1, 0.0)])
+    with pytest.raises(Q.QGTLError, match="no parameters"):
+        Q.Batch([(2, no_params)])
+    with pytest.raises(Q.QGTLError, match="instance 1"):
+        Q.Batch([(2, Q.hea_ops(2, 1, np.ones(4))), (2, Q.make_ops([("RY", 4, -1, 0, 0.1)]))])
