@@ -43,6 +43,56 @@ def splitmix_angles(seed, count):
     return out
 
 
+TUNE_FILE = os.path.join(ROOT, "quantum_geometric_tensor_amd", "tune", "batch_hea_n12-18_L2-8.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_pass_regs_hea_n20_L6.txt")
+
+
+def read_traffic(kernel_prefix):
+    """HBM bytes per dispatch of the dominant kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated with a plain copy), or None."""
+    try:
+        vals = {}
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        if not vals.get("kernel", "").startswith(kernel_prefix):
+            return None
+        return float(vals["hbm_bytes_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def batched_leg(Q, reps=10, warmup=2):
+    """512 independent HEA instances (n ~ U{12..18}, L ~ U{2..8}, then theta, one
+    splitmix64 seed-0 stream) in one captured launch sequence; each timed run
+    re-uploads the angles and downloads every Q."""
+    from quantum_geometric_tensor_amd.workloads import hea_batch
+    work = hea_batch(512, seed=0)
+    tune = Q.load_batch_tune(TUNE_FILE) if os.path.exists(TUNE_FILE) else None
+    b = Q.Batch([(n, Q.hea_ops(n, L, th)) for n, L, th in work], tune)
+    angles = [Q.hea_ops(n, L, th)["angle"] for n, L, th in work]
+    for _ in range(warmup):
+        b.set_angles(angles)
+        b.execute()
+    walls, devs = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.set_angles(angles)
+        _, st = b.execute()
+        walls.append(time.perf_counter() - t0)
+        devs.append(st["device_ms"])
+    wall = float(np.median(walls))
+    out = {"workload": "512 HEA instances, n~U{12..18}, L~U{2..8}, seed 0", "value": 512 / wall, "unit": "QGT/s",
+           "ms_per_batch": wall * 1e3, "device_ms_median": float(np.median(devs)), "launches": st["launches"],
+           "graph": bool(st["graph"]), "sweep_bytes": st["sweep_bytes"], "gram_flops": st["gram_flops"],
+           "device_GB": b.device_bytes / 1e9, "tune": b.get_tune().as_dict(),
+           "tune_file": os.path.relpath(TUNE_FILE, ROOT) if tune is not None else None}
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,6 +102,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed-precision path (for profiler runs): no FP64 legs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 512-instance batched leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (for profiler runs)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain copy kernel of this size (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the QGT path on a single MI355X "
@@ -59,6 +113,18 @@ def main():
         sys.exit(2)
 
     import quantum_geometric_tensor_amd as Q
+    if args.copy_calibration_mib:
+        import ctypes
+        ms = ctypes.c_double(0.0)
+        nbytes = args.copy_calibration_mib << 20
+        rc = Q.lib().qgtl_debug_copy_kernel(ctypes.c_size_t(nbytes), ctypes.byref(ms))
+        if rc:
+            raise Q.QGTLError(rc, Q.lib().qgtl_last_error().decode())
+        print(json.dumps({"copy_calibration_bytes": nbytes, "ms": ms.value,
+                          "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
+    if args.batched_only:
+        print(json.dumps(batched_leg(Q)), flush=True)
+        return
     n, L = N_QUBITS, LAYERS
     theta = splitmix_angles(0, 2 * n * L)
     ops = Q.hea_ops(n, L, theta)
@@ -136,9 +202,9 @@ def main():
         "speedup_vs_fp64_plain": plain_ms / ms_per_step,
         "rel_err_vs_fp64_plain_gpu": err_plain,
         "rel_err_vs_fp64_cpu_oracle": err_cpu,
-        "roofline": {"bound": "hbm", "kernel": "pass_c64_kernel (LDS-tiled sweep pass)",
+        "roofline": {"bound": "hbm", "kernel": "pass_c64_regs_kernel (register-phase sweep pass)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic("pass_c64_regs_kernel"),
                      "bytes_per_launch": sweep_bytes_per_launch, "avg_launch_ms": sweep_avg_ms},
         "breakdown_ms": {"sweep": st["sweep_ms"], "inject": st["inject_ms"], "gram": st["gram_ms"],
                          "finalize": st["finalize_ms"], "launches": st["kernel_launches"]},
@@ -146,6 +212,8 @@ def main():
                  "tflops_algorithmic": st["gram_flops"] / (st["gram_kernel_ms"] * 1e-3) / 1e12 if st["gram_kernel_ms"] else None},
         "cpu_baseline": cpu,
     }
+    if not args.no_batched:
+        line["batched"] = batched_leg(Q)
     print(json.dumps(line), flush=True)
 
 

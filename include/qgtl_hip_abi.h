@@ -161,6 +161,8 @@ int qgtl_hip_device_count(void);
 void *qgtl_hip_stream(void);                          /* the library's stream (default priority) */
 int qgtl_hip_malloc(void **ptr, size_t bytes);
 int qgtl_hip_free(void *ptr);
+int qgtl_hip_host_alloc(void **ptr, size_t bytes);     /* pinned host memory (hipHostMalloc) */
+int qgtl_hip_host_free(void *ptr);
 int qgtl_hip_h2d(void *dst, const void *src, size_t bytes);   /* async on the library stream */
 int qgtl_hip_d2h(void *dst, const void *src, size_t bytes);   /* async on the library stream */
 int qgtl_hip_d2d(void *dst, const void *src, size_t bytes);
@@ -185,6 +187,8 @@ int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, doubl
 int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t);
 int qgtl_hip_swap_c64(void *base, size_t stride, int nstates, int n, int a, int b);
 int qgtl_hip_c64_to_c128(const void *src, void *dst, size_t count);
+int qgtl_hip_fill_f32(void *dst, size_t count, float value);
+int qgtl_hip_copy_f32x4(void *dst, const void *src, size_t count16);    /* count16 units of 16 bytes */
 /* out = H psi for a Pauli sum (terms in device memory) */
 int qgtl_hip_pauli_apply_c64(const void *psi, void *out, int n, int nterms, const void *terms_dev);
 /* per (state, tile) FP64 sums: <D_k|v>, |D_k|^2, |v|^2 (same layout as inject partials) */

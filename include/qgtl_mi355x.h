@@ -208,6 +208,11 @@ int qgtl_hermitian_eig(int P, const double *A, double *evals, double *V);
  * product D[i][j] = (1 + i/16)(i%16 + 16 (j%8)) as the Gram kernel reads it. */
 int qgtl_debug_mfma_layout(float *host_out);
 
+/* Diagnostics: one plain copy kernel of `bytes` (multiple of 16) between two
+ * fresh device buffers; *ms = its HIP-event time.  Reference of known size
+ * for the HBM traffic counters (FETCH_SIZE / WRITE_SIZE). */
+int qgtl_debug_copy_kernel(size_t bytes, double *ms);
+
 /* Diagnostics: raw FP64 Gram C = Z Z^T (R x R) and injection sums (P x 4)
  * of the last mixed-mode execution of plan, in D-slot order. */
 int qgtl_plan_debug_buffers(qgtl_plan_t *plan, double *C_out, double *inj_out, int *R_out);

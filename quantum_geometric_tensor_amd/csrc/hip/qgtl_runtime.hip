@@ -76,6 +76,18 @@ int qgtl_hip_free(void *ptr) {
     return 0;
 }
 
+int qgtl_hip_host_alloc(void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    QGTL_HIP_CHECK(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int qgtl_hip_host_free(void *ptr) {
+    if (ptr) QGTL_HIP_CHECK(hipHostFree(ptr));
+    return 0;
+}
+
 int qgtl_hip_h2d(void *dst, const void *src, size_t bytes) {
     if (bytes) QGTL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
     return 0;

@@ -636,6 +636,16 @@ unsigned grouped_grid(uint32_t units, int grid) {
     return (unsigned)grid;
 }
 
+__global__ void fill_f32_kernel(float *dst, size_t count, float value) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = value;
+}
+
+__global__ void copy_f32x4_kernel(float4 *dst, const float4 *__restrict__ src, size_t count) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 unsigned grid_for(size_t work, unsigned threads = 256, unsigned cap = 8192) {
     size_t g = (work + threads - 1) / threads;
     if (g < 1) g = 1;
@@ -780,6 +790,19 @@ int qgtl_hip_energy_gradient_finalize(const double *dk_h, const double *psi_h, i
     hipLaunchKernelGGL(energy_gradient_finalize_kernel, dim3((unsigned)((P + 255) / 256 + 1)), dim3(256), 0,
                        qgtl::stream(), dk_h, psi_h, P, out);
     QGTL_LAUNCH_CHECK("energy_gradient_finalize");
+    return 0;
+}
+
+int qgtl_hip_fill_f32(void *dst, size_t count, float value) {
+    hipLaunchKernelGGL(fill_f32_kernel, dim3(grid_for(count)), dim3(256), 0, qgtl::stream(), (float *)dst, count, value);
+    QGTL_LAUNCH_CHECK("fill_f32");
+    return 0;
+}
+
+int qgtl_hip_copy_f32x4(void *dst, const void *src, size_t count16) {
+    hipLaunchKernelGGL(copy_f32x4_kernel, dim3(grid_for(count16, 256, 16384)), dim3(256), 0, qgtl::stream(),
+                       (float4 *)dst, (const float4 *)src, count16);
+    QGTL_LAUNCH_CHECK("copy_f32x4");
     return 0;
 }
 

@@ -515,7 +515,7 @@ void qgtl_batch_destroy(qgtl_batch_t *B)
     free(B->inj_h);
     free(B->items_h);
     free(B->launches);
-    free(B->qhost);
+    qgtl_hip_host_free(B->qhost);
     free(B);
 }
 
@@ -599,11 +599,11 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
     B->cus = qgtl_hip_cu_count();
     B->pass_h = (qgtl_pass_desc_t *)calloc(B->npass ? B->npass : 1, sizeof(qgtl_pass_desc_t));
     B->inj_h = (qgtl_inject_desc_t *)calloc(B->ninj ? B->ninj : 1, sizeof(qgtl_inject_desc_t));
-    B->qhost = (double *)malloc(qo * 2 * sizeof(double));
-    if (!B->pass_h || !B->inj_h || !B->qhost) {
+    if (!B->pass_h || !B->inj_h) {
         qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
         return fail_create(B);
     }
+    CTRY(qgtl_hip_host_alloc((void **)&B->qhost, qo * 2 * sizeof(double)));
     CTRY(qgtl_hip_malloc(&B->states, xo * 8));
     CTRY(qgtl_hip_malloc((void **)&B->inj_partials, po * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->inj_sums, so * sizeof(double)));

@@ -710,4 +710,33 @@ int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int 
     return 0;
 }
 
+/* Diagnostics: one plain copy of `bytes` (a multiple of 16) between two
+ * fresh buffers, the known-size reference for HBM traffic counters. */
+int qgtl_debug_copy_kernel(size_t bytes, double *ms)
+{
+    if (bytes == 0 || bytes % 16) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "copy size must be a multiple of 16");
+    int rc = ensure_init();
+    if (rc) return rc;
+    void *a = NULL, *b = NULL;
+    HIPCALL(qgtl_hip_malloc(&a, bytes));
+    rc = qgtl_hip_malloc(&b, bytes);
+    qgtl_hip_event_t e0 = NULL, e1 = NULL;
+    if (!rc) rc = qgtl_hip_event_create(&e0);
+    if (!rc) rc = qgtl_hip_event_create(&e1);
+    if (!rc) rc = qgtl_hip_fill_f32(a, bytes / 4, 1.0f);
+    if (!rc) rc = qgtl_hip_event_record(e0);
+    if (!rc) rc = qgtl_hip_copy_f32x4(b, a, bytes / 16);
+    if (!rc) rc = qgtl_hip_event_record(e1);
+    if (!rc) rc = qgtl_hip_sync();
+    float t = 0.f;
+    if (!rc) rc = qgtl_hip_event_elapsed_ms(e0, e1, &t);
+    qgtl_hip_event_destroy(e0);
+    qgtl_hip_event_destroy(e1);
+    qgtl_hip_free(a);
+    qgtl_hip_free(b);
+    if (rc) return hip_fail(rc, "qgtl_debug_copy_kernel");
+    if (ms) *ms = t;
+    return 0;
+}
+
 int qgtl_init_if_needed(void) { return ensure_init(); }
