@@ -223,6 +223,22 @@ int qgtl_hip_gram_f64(const void *D, int P, size_t N, double *partials, double *
 int qgtl_hip_dotc_c128(const void *D, int P, size_t N, const void *psi, double *partials, double *a);
 int qgtl_hip_qgt_finalize_plain(const double *G, const double *a, int P, double *Q);
 
+/* ---- learned preconditioner: dense layers (bf16 MFMA, fp32 accumulate) ---- */
+/* C (M x N fp32) = op(A) op(B) [+ bias] [+ C]; op(A)[m][k] = ta ? A[k][m] : A[m][k] (lda
+ * columns), op(B)[k][n] = tb ? B[n][k] : B[k][n] (ldb columns); *_bf16: operand stored as bf16 */
+int qgtl_hip_mlp_gemm(int M, int N, int K, const void *A, int a_bf16, int ta, int lda, const void *B, int b_bf16,
+                      int tb, int ldb, float *C, int ldc, const float *bias, int accumulate);
+int qgtl_hip_gelu_fwd(const float *z, float *a, size_t n);
+int qgtl_hip_gelu_bwd(const float *z, float *da, size_t n);           /* da *= g'(z) */
+int qgtl_hip_colsum(const float *X, int rows, int cols, float *out);
+int qgtl_hip_gather_rows(const float *src, const int *idx, int rows, int cols, float *dst);
+int qgtl_hip_adam(float *w, uint16_t *w_bf16, float *m, float *v, const float *g, size_t n, float lr, float b1,
+                  float b2, float eps, int step);
+int qgtl_hip_to_bf16(const float *w, uint16_t *out, size_t n);
+/* per sample b: loss[b] (FP64) and dY = scale * dL/dy for As_all[idx[b]] (P x P fp32) */
+int qgtl_hip_kaporin(int P, int r, int B, const float *Y, const float *As_all, const int *idx, double *loss,
+                     float *dY, float scale);
+
 /* ---- FP64 natural-gradient solves, one workgroup per system ---- */
 int qgtl_hip_lu_solve_c128(void *A, const void *b, void *x, int *piv, int P, int batch, int *status);
 int qgtl_hip_herm_eig_c128(void *A, void *V, double *ev, int P, int batch, int max_sweeps, int *sweeps);

@@ -204,6 +204,66 @@ int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double la
  * Hermitian complex128 matrix, parallel Jacobi on the GPU.  V may be NULL. */
 int qgtl_hermitian_eig(int P, const double *A, double *evals, double *V);
 
+/* ---- learned preconditioner for the metric solve ----
+ * A small MLP maps features x of one parameter point to the diagonal d and
+ * low-rank factor U of M^-1 = diag(d) + U U^T for the PCG solve of
+ * (Re Q + lambda I) x = grad E (QGTL_SOLVE_PCG).  hidden_layers dense layers
+ * of width `hidden` (tanh-form GELU), BF16 weights and MFMA operands with
+ * FP32 accumulation, FP32 Adam on Kaporin's condition number of the
+ * preconditioned, Jacobi-scaled matrix As = D^-1/2 A D^-1/2 (D = diag A):
+ * with y the network output, s_k = y[k(1+r)], u_kj = y[k(1+r)+1+j] / sqrt(P),
+ * d = e^s / D, U = D^-1/2 u (s = 0, u = 0 is Jacobi).  No QGTL counterpart;
+ * it preconditions compute_natural_gradient_cpu's CG
+ * (src/quantum_geometric/hybrid/classical_optimization_engine.c:543-610). */
+typedef struct qgtl_mlp qgtl_mlp_t;
+qgtl_mlp_t *qgtl_mlp_create(int in_dim, int hidden, int hidden_layers, int P, int rank, uint64_t seed);
+size_t qgtl_mlp_num_params(const qgtl_mlp_t *m);
+/* FP32 parameter vector: W_0 (in x out, row-major), b_0, W_1, b_1, ...; set resets Adam */
+int qgtl_mlp_get_params(const qgtl_mlp_t *m, float *params);
+int qgtl_mlp_set_params(qgtl_mlp_t *m, const float *params);
+/* training set on the GPU: S feature rows (S x in_dim) and S Jacobi-scaled matrices As (S x P x P) */
+int qgtl_mlp_set_data(qgtl_mlp_t *m, int S, const float *X, const float *As);
+/* mean loss over the samples idx[0..B) and its gradient (grad may be NULL) */
+int qgtl_mlp_loss_grad(qgtl_mlp_t *m, int B, const int *idx, double *loss, float *grad);
+/* one Adam step (beta 0.9 / 0.999) on the samples idx; *loss: batch mean before the step */
+int qgtl_mlp_train_step(qgtl_mlp_t *m, int B, const int *idx, float lr, double *loss);
+/* network output Y (B x P(1+r)) for B feature rows X */
+int qgtl_mlp_forward(qgtl_mlp_t *m, int B, const float *X, float *Y);
+/* features of one point: sin theta (P), cos theta (P), log D (P), field (in_dim = 3P + 1) */
+int qgtl_pc_features(int P, const double *theta, const double *D, double field, float *x);
+/* d (P) and U (P x rank) of M^-1 for one system from its features and the diagonal D of A */
+int qgtl_mlp_preconditioner(qgtl_mlp_t *m, const float *x, const double *D, double *d, double *U);
+void qgtl_mlp_destroy(qgtl_mlp_t *m);
+
+/* ---- natural-gradient VQE step (QGTL: run_vqe / update_natural_gradient,
+ * src/quantum_geometric/hybrid/quantum_classical_algorithms.c:854,
+ * src/quantum_geometric/hybrid/classical_optimization_engine.c:390-416) ----
+ * theta (P, parameter order) -> E, grad E and Q at theta on the GPU; solve
+ * (Re Q + lambda I) x = grad E with `method`; theta -= eta x.  The plan must be
+ * mixed-mode with a Hamiltonian set; QGTL_SOLVE_PCG uses the learned
+ * preconditioner `mlp` (features: theta, diag(Re Q) + lambda, `field`). */
+typedef struct {
+    double lambda;
+    double eta;
+    int32_t method;          /* QGTL_SOLVE_* */
+    int32_t maxit;
+    double tol;              /* relative residual of the iterative methods */
+} qgtl_ng_config_t;
+typedef struct {
+    double energy;           /* at the incoming theta */
+    double grad_norm;
+    double step_norm;
+    int32_t solve_iterations;
+    int32_t solve_status;
+    double solve_rel_residual;
+    double solve_device_ms;
+    double qgt_ms;           /* host wall: angles, QGT, energy and gradient */
+    double solve_ms;         /* host wall: preconditioner and solve */
+    double step_ms;          /* host wall of the whole step */
+} qgtl_ng_info_t;
+int qgtl_ng_step(qgtl_plan_t *plan, double *theta, const qgtl_ng_config_t *cfg, qgtl_mlp_t *mlp, double field,
+                 qgtl_ng_info_t *info);
+
 /* Diagnostics: v_mfma_f32_32x32x16_bf16 lane-layout probe; writes the 32 x 32
  * product D[i][j] = (1 + i/16)(i%16 + 16 (j%8)) as the Gram kernel reads it. */
 int qgtl_debug_mfma_layout(float *host_out);

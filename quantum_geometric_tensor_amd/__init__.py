@@ -372,3 +372,105 @@ def load_batch_tune(path):
     t = BatchTune()
     _check(_batch_lib().qgtl_batch_tune_load(path.encode(), ctypes.byref(t)))
     return t
+
+
+# ---------------------------------------------------------------- learned preconditioner
+def _mlp_lib():
+    L = lib()
+    if not getattr(L, "_mlp_bound", False):
+        vp, ip = ctypes.c_void_p, ctypes.c_int
+        L.qgtl_mlp_create.argtypes = [ip, ip, ip, ip, ip, ctypes.c_uint64]
+        L.qgtl_mlp_create.restype = vp
+        L.qgtl_mlp_num_params.argtypes = [vp]
+        L.qgtl_mlp_num_params.restype = ctypes.c_size_t
+        L.qgtl_mlp_get_params.argtypes = [vp, vp]
+        L.qgtl_mlp_set_params.argtypes = [vp, vp]
+        L.qgtl_mlp_set_data.argtypes = [vp, ip, vp, vp]
+        L.qgtl_mlp_loss_grad.argtypes = [vp, ip, vp, vp, vp]
+        L.qgtl_mlp_train_step.argtypes = [vp, ip, vp, ctypes.c_float, vp]
+        L.qgtl_mlp_forward.argtypes = [vp, ip, vp, vp]
+        L.qgtl_pc_features.argtypes = [ip, vp, vp, ctypes.c_double, vp]
+        L.qgtl_mlp_preconditioner.argtypes = [vp, vp, vp, vp, vp]
+        L.qgtl_mlp_destroy.argtypes = [vp]
+        L._mlp_bound = True
+    return L
+
+
+def pc_features(theta, D, field):
+    """Features of one parameter point: sin theta, cos theta, log diag(A), field (3P + 1 floats)."""
+    theta = np.ascontiguousarray(theta, np.float64)
+    D = np.ascontiguousarray(D, np.float64)
+    x = np.zeros(3 * theta.size + 1, np.float32)
+    _check(_mlp_lib().qgtl_pc_features(theta.size, _p(theta), _p(D), float(field), _p(x)))
+    return x
+
+
+class MLP:
+    """The learned preconditioner's network, trained and evaluated on the GPU."""
+
+    def __init__(self, in_dim, hidden, hidden_layers, P, rank, seed=0):
+        L = _mlp_lib()
+        h = L.qgtl_mlp_create(in_dim, hidden, hidden_layers, P, rank, seed)
+        if not h:
+            raise QGTLError(-1, L.qgtl_last_error().decode(errors="replace"))
+        self._h = h
+        self.in_dim, self.hidden, self.hidden_layers, self.P, self.rank = in_dim, hidden, hidden_layers, P, rank
+        self.out_dim = P * (1 + rank)
+        self.num_params = L.qgtl_mlp_num_params(h)
+
+    def get_params(self):
+        p = np.zeros(self.num_params, np.float32)
+        _check(_mlp_lib().qgtl_mlp_get_params(self._h, _p(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise ValueError("parameter count mismatch")
+        _check(_mlp_lib().qgtl_mlp_set_params(self._h, _p(p)))
+
+    def set_data(self, X, As):
+        X = np.ascontiguousarray(X, np.float32)
+        As = np.ascontiguousarray(As, np.float32)
+        if X.shape[1] != self.in_dim or As.shape[1:] != (self.P, self.P) or X.shape[0] != As.shape[0]:
+            raise ValueError("X must be S x in_dim and As S x P x P")
+        _check(_mlp_lib().qgtl_mlp_set_data(self._h, X.shape[0], _p(X), _p(As)))
+        self.S = X.shape[0]
+
+    def loss_grad(self, idx):
+        idx = np.ascontiguousarray(idx, np.int32)
+        loss = np.zeros(1)
+        g = np.zeros(self.num_params, np.float32)
+        _check(_mlp_lib().qgtl_mlp_loss_grad(self._h, idx.size, _p(idx), _p(loss), _p(g)))
+        return float(loss[0]), g
+
+    def train_step(self, idx, lr):
+        idx = np.ascontiguousarray(idx, np.int32)
+        loss = np.zeros(1)
+        _check(_mlp_lib().qgtl_mlp_train_step(self._h, idx.size, _p(idx), float(lr), _p(loss)))
+        return float(loss[0])
+
+    def forward(self, X):
+        X = np.ascontiguousarray(np.atleast_2d(X), np.float32)
+        Y = np.zeros((X.shape[0], self.out_dim), np.float32)
+        _check(_mlp_lib().qgtl_mlp_forward(self._h, X.shape[0], _p(X), _p(Y)))
+        return Y
+
+    def preconditioner(self, x, D):
+        x = np.ascontiguousarray(x, np.float32)
+        D = np.ascontiguousarray(D, np.float64)
+        d = np.zeros(self.P)
+        U = np.zeros((self.P, self.rank))
+        _check(_mlp_lib().qgtl_mlp_preconditioner(self._h, _p(x), _p(D), _p(d), _p(U)))
+        return d, U
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _mlp_lib().qgtl_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -106,4 +106,7 @@ int qgtl_set_error(int code, const char *fmt, ...);
 
 /* engine (qgtl_engine.c) */
 int qgtl_init_if_needed(void);
+
+/* learned preconditioner (qgtl_mlp.c) */
+void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank);
 #endif

@@ -52,6 +52,13 @@ def lib():
         L.oracle_hermitian_eig.argtypes = [ip, vp, vp, vp]
         L.oracle_regularized_natural_gradient.argtypes = [ip, vp, vp, vp, dp, dp, ip, ip, dp]
         L.oracle_regularized_natural_gradient.restype = dp
+        L.oracle_mlp_num_params.argtypes = [ip, ip, ip, ip]
+        L.oracle_mlp_num_params.restype = ctypes.c_size_t
+        L.oracle_mlp_forward.argtypes = [ip, ip, ip, ip, ip, vp, vp, vp]
+        L.oracle_kaporin.argtypes = [ip, ip, vp, vp, vp]
+        L.oracle_kaporin.restype = dp
+        L.oracle_mlp_loss_grad.argtypes = [ip, ip, ip, ip, ip, ip, vp, vp, vp, vp]
+        L.oracle_mlp_loss_grad.restype = dp
         _lib = L
     return _lib
 
@@ -202,3 +209,33 @@ def regularized_natural_gradient(G, g, lam=1e-4, cond=1e8, adaptive=True, pinv=T
     used = lib().oracle_regularized_natural_gradient(len(g), _p(G), _p(g), _p(x), lam, cond,
                                                      int(adaptive), int(pinv), cutoff)
     return x, used
+
+
+# ------------------------------------------------------------------ learned preconditioner
+def mlp_num_params(F, H, nh, O):
+    return int(lib().oracle_mlp_num_params(F, H, nh, O))
+
+
+def mlp_forward(F, H, nh, O, params, X):
+    X = np.ascontiguousarray(X, np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    Y = np.zeros((X.shape[0], O))
+    lib().oracle_mlp_forward(F, H, nh, O, X.shape[0], _p(params), _p(X), _p(Y))
+    return Y
+
+
+def kaporin(P, r, y, As, grad=True):
+    y = np.ascontiguousarray(y, np.float64)
+    As = np.ascontiguousarray(As, np.float64)
+    dy = np.zeros_like(y) if grad else None
+    loss = lib().oracle_kaporin(P, r, _p(y), _p(As), _p(dy) if grad else None)
+    return (loss, dy) if grad else loss
+
+
+def mlp_loss_grad(F, H, nh, P, r, params, X, As, grad=True):
+    X = np.ascontiguousarray(X, np.float64)
+    As = np.ascontiguousarray(As, np.float64)
+    params = np.ascontiguousarray(params, np.float64)
+    g = np.zeros_like(params) if grad else None
+    loss = lib().oracle_mlp_loss_grad(F, H, nh, P, r, X.shape[0], _p(params), _p(X), _p(As), _p(g) if grad else None)
+    return (loss, g) if grad else loss

@@ -1,0 +1,120 @@
+"""HIP training and inference kernels of the learned preconditioner against
+the FP64 CPU oracle (oracle/mlp_oracle.c).
+
+The GPU rounds weights and every GEMM operand (activations, back-propagated
+gradients) to bf16 and accumulates in fp32; the oracle gets the same
+bf16-rounded weights and inputs and computes everything else in FP64.  Bars:
+forward output normwise 2e-2, loss 1e-2 relative, parameter gradient
+normwise 5e-2 with cosine similarity >= 0.998, one Adam step to 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import quantum_geometric_tensor_amd as Q
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(x):
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return r.view(np.float32).astype(np.float64)
+
+
+def spd_unit_diag(P, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.normal(size=(P, P + 2))
+    A = B @ B.T / P + 3e-2 * np.eye(P)
+    d = 1 / np.sqrt(np.diag(A))
+    return A * d[:, None] * d[None, :]
+
+
+def oracle_params(m):
+    """GPU master parameters with the weights rounded to bf16 (biases stay fp32)."""
+    p = m.get_params().astype(np.float64)
+    out = p.copy()
+    o = 0
+    for i in range(m.hidden_layers + 1):
+        fin = m.in_dim if i == 0 else m.hidden
+        fout = m.out_dim if i == m.hidden_layers else m.hidden
+        out[o:o + fin * fout] = bf16(p[o:o + fin * fout])
+        o += fin * fout + fout
+    assert o == p.size
+    return out
+
+
+def make_net(seed=3, F=19, H=48, nh=3, P=6, r=2):
+    m = Q.MLP(F, H, nh, P, r, seed=seed)
+    # larger output weights than the Jacobi-like start, so the check sees a non-trivial preconditioner
+    p = m.get_params()
+    p *= 4.0
+    m.set_params(p)
+    return m
+
+
+def test_mlp_forward_matches_oracle():
+    m = make_net()
+    X = np.random.default_rng(0).normal(size=(37, m.in_dim)).astype(np.float32)
+    Y = m.forward(X).astype(np.float64)
+    Yo = O.mlp_forward(m.in_dim, m.hidden, m.hidden_layers, m.out_dim, oracle_params(m), bf16(X))
+    err = np.max(np.abs(Y - Yo)) / np.max(np.abs(Yo))
+    print(f"forward normwise err {err:.2e}")
+    assert err <= 2e-2
+    m.close()
+
+
+def test_mlp_loss_and_gradient_match_oracle():
+    m = make_net(seed=5)
+    S = 20
+    rng = np.random.default_rng(1)
+    X = rng.normal(size=(S, m.in_dim)).astype(np.float32)
+    As = np.stack([spd_unit_diag(m.P, 100 + s) for s in range(S)])
+    m.set_data(X, As.astype(np.float32))
+    idx = np.array([3, 7, 0, 19, 11, 5, 5, 12], np.int32)
+    loss, g = m.loss_grad(idx)
+    lo, go = O.mlp_loss_grad(m.in_dim, m.hidden, m.hidden_layers, m.P, m.rank, oracle_params(m), bf16(X[idx]),
+                             bf16(As[idx]))
+    gerr = np.max(np.abs(g - go)) / np.max(np.abs(go))
+    cos = float(np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)))
+    print(f"loss {loss:.6f} oracle {lo:.6f}; grad normwise err {gerr:.2e} cos {cos:.6f}")
+    assert abs(loss - lo) <= 1e-2 * max(1.0, abs(lo))
+    assert gerr <= 5e-2 and cos >= 0.998
+    m.close()
+
+
+def test_adam_step_matches_formula():
+    m = make_net(seed=7, F=11, H=32, P=5, r=1)
+    S = 6
+    rng = np.random.default_rng(2)
+    m.set_data(rng.normal(size=(S, m.in_dim)).astype(np.float32),
+               np.stack([spd_unit_diag(m.P, s) for s in range(S)]).astype(np.float32))
+    idx = np.arange(S, dtype=np.int32)
+    w0 = m.get_params().astype(np.float64)
+    _, g = m.loss_grad(idx)
+    lr = 1e-3
+    m.train_step(idx, lr)
+    w1 = m.get_params().astype(np.float64)
+    g = g.astype(np.float64)
+    mhat = (0.1 * g) / (1 - 0.9)
+    vhat = (0.001 * g * g) / (1 - 0.999)
+    expect = w0 - lr * mhat / (np.sqrt(vhat) + 1e-8)
+    assert np.max(np.abs(w1 - expect)) <= 1e-6
+    m.close()
+
+
+def test_mlp_training_reduces_loss():
+    m = Q.MLP(13, 64, 3, 8, 2, seed=11)
+    S = 64
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(S, m.in_dim)).astype(np.float32)
+    As = np.stack([spd_unit_diag(m.P, 500 + s) for s in range(S)]).astype(np.float32)
+    m.set_data(X, As)
+    idx = np.arange(S, dtype=np.int32)
+    first = m.loss_grad(idx)[0]
+    for it in range(150):
+        m.train_step(rng.choice(S, 16, replace=False).astype(np.int32), 2e-3)
+    last = m.loss_grad(idx)[0]
+    print(f"Kaporin loss {first:.3f} -> {last:.3f}")
+    assert last < first - 0.5
+    m.close()
