@@ -59,6 +59,17 @@ typedef struct {
     int8_t ph_rbits[QGTL_MAX_PHASES][5];
     int8_t ph_first[QGTL_MAX_PHASES + 1];
     int8_t ph_greg[QGTL_TILE_BITS_MAX];
+    /* Rotation form (form = 1): matrix g = diag(post[g]) R_g diag(pre[g]) with the
+     * real rotation R_g = [[c, -s], [s, c]], c = rot[g].re, s = rot[g].im;
+     * pre/post[g][b] is the unit phase on tile-bit value b.  A pass then costs
+     * 4 real FMA per amplitude per gate plus one complex product per phase
+     * layer (the product over the pass's gates, from two tables in LDS).
+     * has_pre = 0: every pre phase is 1.  form = 0: mat holds general matrices. */
+    int32_t form;
+    int32_t has_pre;
+    qgtl_c64_t rot[QGTL_TILE_BITS_MAX];
+    qgtl_c64_t pre[QGTL_TILE_BITS_MAX][2];
+    qgtl_c64_t post[QGTL_TILE_BITS_MAX][2];
 } qgtl_pass_desc_t;
 
 /* Derivative-state injection in the frame after a block:

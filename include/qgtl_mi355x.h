@@ -67,7 +67,8 @@ typedef struct {
     int32_t gram_variant;     /* Gram operand split: 3 / 4 = two bf16 pieces with 3 / 4 products, 6 = exact
                                  three-piece bf16 split, 33 / 34 = two fp16 pieces of 2^15 x with 3 / 4
                                  products; 0 = default (33) */
-    int32_t reserved[5];
+    int32_t sweep_form;       /* 0 = rotation form (real rotations + phase tables), 1 = general complex matrices */
+    int32_t reserved[4];
 } qgtl_tune_t;
 
 int qgtl_init(int device);
@@ -241,7 +242,8 @@ void qgtl_mlp_destroy(qgtl_mlp_t *m);
  * theta (P, parameter order) -> E, grad E and Q at theta on the GPU; solve
  * (Re Q + lambda I) x = grad E with `method`; theta -= eta x.  The plan must be
  * mixed-mode with a Hamiltonian set; QGTL_SOLVE_PCG uses the learned
- * preconditioner `mlp` (features: theta, diag(Re Q) + lambda, `field`). */
+ * preconditioner `mlp` (features: theta, diag(Re Q) + lambda, `field`) with
+ * Jacobi-PCG as the fallback when it misses the tolerance. */
 typedef struct {
     double lambda;
     double eta;
@@ -253,8 +255,10 @@ typedef struct {
     double energy;           /* at the incoming theta */
     double grad_norm;
     double step_norm;
-    int32_t solve_iterations;
+    int32_t solve_iterations;  /* including a fallback solve */
     int32_t solve_status;
+    int32_t fallback;          /* 1: the learned PCG missed the tolerance and Jacobi-PCG solved */
+    int32_t pad;
     double solve_rel_residual;
     double solve_device_ms;
     double qgt_ms;           /* host wall: angles, QGT, energy and gradient */
@@ -272,6 +276,11 @@ int qgtl_debug_mfma_layout(float *host_out);
  * fresh device buffers; *ms = its HIP-event time.  Reference of known size
  * for the HBM traffic counters (FETCH_SIZE / WRITE_SIZE). */
 int qgtl_debug_copy_kernel(size_t bytes, double *ms);
+
+/* Diagnostics: largest |diag(post) R diag(pre) - U| over the sweep passes'
+ * gates (the rotation form of the pass descriptors) and how many passes
+ * carry a pre-phase layer.  Host only. */
+int qgtl_plan_debug_form(const qgtl_plan_t *plan, double *max_err, int *passes_with_pre);
 
 /* Diagnostics: raw FP64 Gram C = Z Z^T (R x R) and injection sums (P x 4)
  * of the last mixed-mode execution of plan, in D-slot order. */

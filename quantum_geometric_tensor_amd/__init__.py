@@ -47,7 +47,7 @@ class Stats(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("chunk_states", ctypes.c_int32), ("gram_chunks", ctypes.c_int32),
-                ("gram_variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("gram_variant", ctypes.c_int32), ("sweep_form", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
 
 
 _lib = None
@@ -139,10 +139,10 @@ def statevector(n, ops, mode=MODE_MIXED):
 class Plan:
     """A compiled circuit with device buffers, re-run with new angles."""
 
-    def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0):
+    def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0, sweep_form=0):
         self.ops = np.ascontiguousarray(ops, OP_DTYPE).copy()
         self.n = n
-        tune = Tune(chunk_states, gram_chunks, gram_terms)
+        tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form)
         h = lib().qgtl_plan_create(n, len(self.ops), _p(self.ops), mode, ctypes.byref(tune))
         if not h:
             raise QGTLError(-1, lib().qgtl_last_error().decode(errors="replace"))
@@ -474,3 +474,35 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- natural-gradient VQE step
+class NGConfig(ctypes.Structure):
+    _fields_ = [("lam", ctypes.c_double), ("eta", ctypes.c_double), ("method", ctypes.c_int32),
+                ("maxit", ctypes.c_int32), ("tol", ctypes.c_double)]
+
+
+class NGInfo(ctypes.Structure):
+    _fields_ = [("energy", ctypes.c_double), ("grad_norm", ctypes.c_double), ("step_norm", ctypes.c_double),
+                ("solve_iterations", ctypes.c_int32), ("solve_status", ctypes.c_int32),
+                ("fallback", ctypes.c_int32), ("pad", ctypes.c_int32), ("solve_rel_residual", ctypes.c_double), ("solve_device_ms", ctypes.c_double),
+                ("qgt_ms", ctypes.c_double), ("solve_ms", ctypes.c_double), ("step_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def ng_step(plan, theta, lam=1e-3, eta=0.1, method=0, tol=1e-8, maxit=10000, mlp=None, field=0.0):
+    """One natural-gradient step on `plan` (Hamiltonian set): updates theta in place, returns NGInfo dict."""
+    L = lib()
+    if not getattr(L, "_ng_bound", False):
+        L.qgtl_ng_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(NGConfig), ctypes.c_void_p,
+                                   ctypes.c_double, ctypes.POINTER(NGInfo)]
+        L._ng_bound = True
+    if theta.dtype != np.float64 or not theta.flags["C_CONTIGUOUS"]:
+        raise ValueError("theta must be a C-contiguous float64 array (updated in place)")
+    cfg = NGConfig(lam, eta, method, maxit, tol)
+    info = NGInfo()
+    _check(L.qgtl_ng_step(plan._h, _p(theta), ctypes.byref(cfg), mlp._h if mlp is not None else None, float(field),
+                          ctypes.byref(info)))
+    return info.as_dict()

@@ -41,6 +41,45 @@ __device__ __forceinline__ void apply2(float2 &a, float2 &b, const float2 *m) {
     b = nb;
 }
 
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+// Real rotation [[c, -s], [s, c]] on a complex pair: 4 real FMA per amplitude.
+__device__ __forceinline__ void rot2(float2 &a, float2 &b, float c, float s) {
+    const float2 na = make_float2(fmaf(c, a.x, -s * b.x), fmaf(c, a.y, -s * b.y));
+    const float2 nb = make_float2(fmaf(s, a.x, c * b.x), fmaf(s, a.y, c * b.y));
+    a = na;
+    b = nb;
+}
+
+// Phase tables of a rotation-form pass: entry i < 128 is the product of the
+// post (pre) phases of the gates on tile bits 0..6 at bit pattern i, entry
+// 128 + i the same for tile bits 7..12, so amplitude j gets
+// tab[j & 127] * tab[128 + (j >> 7)].
+constexpr int kPhaseTab = 192;
+
+__device__ __forceinline__ void build_phase_tables(const qgtl_pass_desc_t &d, float2 *post_tab, float2 *pre_tab) {
+    for (int t = threadIdx.x; t < kPhaseTab; t += kThreads) {
+        const bool lo = t < 128;
+        const int i = lo ? t : t - 128;
+        float2 a = make_float2(1.f, 0.f), b = make_float2(1.f, 0.f);
+        for (int g = 0; g < d.ngates; g++) {
+            const int bit = d.gate_bit[g];
+            if (lo ? bit >= 7 : bit < 7) continue;
+            const int v = (i >> (lo ? bit : bit - 7)) & 1;
+            a = cmul(a, make_float2(d.post[g][v].re, d.post[g][v].im));
+            b = cmul(b, make_float2(d.pre[g][v].re, d.pre[g][v].im));
+        }
+        post_tab[t] = a;
+        pre_tab[t] = b;
+    }
+}
+
+__device__ __forceinline__ float2 phase_at(const float2 *tab, uint32_t j) {
+    return cmul(tab[j & 127u], tab[128u + (j >> 7)]);
+}
+
 __device__ __forceinline__ uint32_t insert_zero(uint32_t x, int b) {
     uint32_t lo = x & ((1u << b) - 1u);
     return ((x >> b) << (b + 1)) | lo;
@@ -79,6 +118,37 @@ __device__ __forceinline__ void apply_group(float2 *tile, int m, const int *bits
     }
 }
 
+template <int G>
+__device__ __forceinline__ void apply_group_rot(float2 *tile, int m, const int *bits, const float (*cs)[2]) {
+    const int count = 1 << (m - G);
+    for (int o = threadIdx.x; o < count; o += kThreads) {
+        uint32_t base = (uint32_t)o;
+#pragma unroll
+        for (int i = 0; i < G; i++) base = insert_zero(base, bits[i]);
+        float2 v[1 << G];
+        int idx[1 << G];
+#pragma unroll
+        for (int c = 0; c < (1 << G); c++) {
+            uint32_t id = base;
+#pragma unroll
+            for (int i = 0; i < G; i++)
+                if ((c >> i) & 1) id |= 1u << bits[i];
+            idx[c] = lds_pad((int)id);
+            v[c] = tile[idx[c]];
+        }
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+#pragma unroll
+            for (int c = 0; c < (1 << G); c++) {
+                if ((c >> i) & 1) continue;
+                rot2(v[c], v[c | (1 << i)], cs[i][0], cs[i][1]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < (1 << G); c++) tile[idx[c]] = v[c];
+    }
+}
+
 // Inverse of the local part of the ring map for the low tile (see header).
 __device__ __forceinline__ uint32_t ring_local_inverse(uint32_t y, int m, uint32_t par_r) {
     uint32_t x0 = (y & 1u) ^ ((y >> (m - 1)) & 1u) ^ par_r;
@@ -91,8 +161,11 @@ constexpr int kLdsTileElems = (1 << QGTL_TILE_BITS_MAX) + (1 << QGTL_TILE_BITS_M
 
 // One tile r of one state: load, gate groups through LDS, store.
 __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
-                                              float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off) {
+                                              float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
+                                              float2 *post_tab, float2 *pre_tab) {
     const int m = d.m, nlow = d.nlow, nhi = m - nlow;
+    const bool rotf = d.form == 1, pre = rotf && d.has_pre;
+    if (rotf) build_phase_tables(d, post_tab, pre_tab);
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -109,25 +182,44 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
         const int j = 2 * p;
         const uint32_t g = base + hi_off[j >> nlow] + ((uint32_t)j & lowmask);
         const float4 v = *reinterpret_cast<const float4 *>(src + g);
-        tile[lds_pad(j)] = make_float2(v.x, v.y);
-        tile[lds_pad(j + 1)] = make_float2(v.z, v.w);
+        float2 a = make_float2(v.x, v.y), b = make_float2(v.z, v.w);
+        if (pre) {
+            a = cmul(a, phase_at(pre_tab, (uint32_t)j));
+            b = cmul(b, phase_at(pre_tab, (uint32_t)j + 1u));
+        }
+        tile[lds_pad(j)] = a;
+        tile[lds_pad(j + 1)] = b;
     }
     __syncthreads();
 
     for (int g0 = 0; g0 < d.ngates; g0 += 3) {
         const int G = min(3, d.ngates - g0);
         int bits[3];
-        float2 mats[3][4];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int gi = min(g0 + i, d.ngates - 1);
-            bits[i] = d.gate_bit[gi];
+        for (int i = 0; i < 3; i++) bits[i] = d.gate_bit[min(g0 + i, d.ngates - 1)];
+        if (rotf) {
+            float cs[3][2];
 #pragma unroll
-            for (int e = 0; e < 4; e++) mats[i][e] = make_float2(d.mat[gi][e].re, d.mat[gi][e].im);
+            for (int i = 0; i < 3; i++) {
+                const int gi = min(g0 + i, d.ngates - 1);
+                cs[i][0] = d.rot[gi].re;
+                cs[i][1] = d.rot[gi].im;
+            }
+            if (G == 3) apply_group_rot<3>(tile, m, bits, cs);
+            else if (G == 2) apply_group_rot<2>(tile, m, bits, cs);
+            else apply_group_rot<1>(tile, m, bits, cs);
+        } else {
+            float2 mats[3][4];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const int gi = min(g0 + i, d.ngates - 1);
+#pragma unroll
+                for (int e = 0; e < 4; e++) mats[i][e] = make_float2(d.mat[gi][e].re, d.mat[gi][e].im);
+            }
+            if (G == 3) apply_group<3>(tile, m, bits, mats);
+            else if (G == 2) apply_group<2>(tile, m, bits, mats);
+            else apply_group<1>(tile, m, bits, mats);
         }
-        if (G == 3) apply_group<3>(tile, m, bits, mats);
-        else if (G == 2) apply_group<2>(tile, m, bits, mats);
-        else apply_group<1>(tile, m, bits, mats);
         __syncthreads();
     }
 
@@ -135,7 +227,11 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
         for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
             const int j = 2 * p;
             const uint32_t g = base + hi_off[j >> nlow] + ((uint32_t)j & lowmask);
-            const float2 a = tile[lds_pad(j)], b = tile[lds_pad(j + 1)];
+            float2 a = tile[lds_pad(j)], b = tile[lds_pad(j + 1)];
+            if (rotf) {
+                a = cmul(a, phase_at(post_tab, (uint32_t)j));
+                b = cmul(b, phase_at(post_tab, (uint32_t)j + 1u));
+            }
             *reinterpret_cast<float4 *>(dst + g) = make_float4(a.x, a.y, b.x, b.y);
         }
     } else if (d.store == QGTL_STORE_RING) {
@@ -148,8 +244,13 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
         const uint32_t par_r = (uint32_t)__popc(r) & 1u;
         for (int p = threadIdx.x; p < (T >> 1); p += kThreads) {
             const uint32_t y = 2u * (uint32_t)p;
-            const float2 a = tile[lds_pad((int)ring_local_inverse(y, m, par_r))];
-            const float2 b = tile[lds_pad((int)ring_local_inverse(y + 1u, m, par_r))];
+            const uint32_t ja = ring_local_inverse(y, m, par_r), jb = ring_local_inverse(y + 1u, m, par_r);
+            float2 a = tile[lds_pad((int)ja)];
+            float2 b = tile[lds_pad((int)jb)];
+            if (rotf) {
+                a = cmul(a, phase_at(post_tab, ja));
+                b = cmul(b, phase_at(post_tab, jb));
+            }
             const uint32_t yhi = ((y >> (m - 1)) & 1u) ? (pre ^ ones) : pre;
             const size_t g = ((size_t)yhi << m) | y;
             *reinterpret_cast<float4 *>(dst + g) = make_float4(a.x, a.y, b.x, b.y);
@@ -166,6 +267,7 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
                     else x ^= (ab ^ bb) * ((1u << d.perm_a[i]) | (1u << d.perm_b[i]));  // SWAP
                 }
                 o[e] = tile[lds_pad((int)x)];
+                if (rotf) o[e] = cmul(o[e], phase_at(post_tab, x));
             }
             *reinterpret_cast<float4 *>(dst + 2 * p) = make_float4(o[0].x, o[0].y, o[1].x, o[1].y);
         }
@@ -177,8 +279,9 @@ __global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc
                                                             float2 *dst_all, size_t stride) {
     __shared__ float2 tile[kLdsTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
+    __shared__ float2 tabs[2 * kPhaseTab];
     pass_lds_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
-                  tile, hi_off);
+                  tile, hi_off, tabs, tabs + kPhaseTab);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,6 +339,13 @@ __device__ __forceinline__ void apply_reg(float2 (&v)[kRegs], const float2 *m) {
         if (!((r >> P) & 1)) apply2(v[r], v[r | (1 << P)], m);
 }
 
+template <int P>
+__device__ __forceinline__ void apply_reg_rot(float2 (&v)[kRegs], float c, float s) {
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!((r >> P) & 1)) rot2(v[r], v[r | (1 << P)], c, s);
+}
+
 template <int R>
 struct RegLoop {
     __device__ __forceinline__ static void store_lds(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs]) {
@@ -254,6 +364,10 @@ struct RegLoop {
     __device__ __forceinline__ static void load_lds(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs]) {
         RegLoop<R - 1>::load_lds(tile, L, v);
         v[R - 1] = tile[rpad(reg_index<R - 1>(L))];
+    }
+    __device__ __forceinline__ static void phase(const float2 *tab, const RegLayout &L, float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::phase(tab, L, v);
+        v[R - 1] = cmul(v[R - 1], phase_at(tab, reg_index<R - 1>(L)));
     }
     // global access in pairs (register bit 0 = tile bit 0)
     __device__ __forceinline__ static void load_global(const float2 *src, uint32_t base, const uint32_t *hi_off,
@@ -287,6 +401,7 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void store_lds(float2 *, const RegLayout &, const float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_lds_ring(float2 *, const RegLayout &, const float2 (&)[kRegs], uint32_t) {}
     __device__ __forceinline__ static void load_lds(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void phase(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global(const float2 *, uint32_t, const uint32_t *, int, uint32_t,
                                                        const RegLayout &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global(float2 *, uint32_t, const uint32_t *, int, uint32_t,
@@ -298,9 +413,12 @@ struct RegLoop<0> {
 constexpr int kRegTileElems = (1 << kRT) + (1 << kRT) / 32;
 
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
-                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off) {
+                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
+                                               float2 *post_tab, float2 *pre_tab) {
     const int nlow = d.nlow, nhi = kRT - nlow;
     const uint32_t t = threadIdx.x;
+    const bool rotf = d.form == 1;
+    if (rotf) build_phase_tables(d, post_tab, pre_tab);
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -315,6 +433,7 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     float2 v[kRegs];
     const RegLayout L0 = make_layout(d.ph_rbits[0], t);
     RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
+    if (rotf && d.has_pre) RegLoop<kRegs>::phase(pre_tab, L0, v);
 
     RegLayout Lc = L0;
     for (int ph = 0; ph < d.nphase; ph++) {
@@ -327,18 +446,30 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
             Lc = Ln;
         }
         for (int g = d.ph_first[ph]; g < d.ph_first[ph + 1]; g++) {
-            float2 m[4];
+            if (rotf) {
+                const float c = d.rot[g].re, sn = d.rot[g].im;
+                switch (d.ph_greg[g]) {
+                case 0: apply_reg_rot<0>(v, c, sn); break;
+                case 1: apply_reg_rot<1>(v, c, sn); break;
+                case 2: apply_reg_rot<2>(v, c, sn); break;
+                case 3: apply_reg_rot<3>(v, c, sn); break;
+                default: apply_reg_rot<4>(v, c, sn); break;
+                }
+            } else {
+                float2 m[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) m[e] = make_float2(d.mat[g][e].re, d.mat[g][e].im);
-            switch (d.ph_greg[g]) {
-            case 0: apply_reg<0>(v, m); break;
-            case 1: apply_reg<1>(v, m); break;
-            case 2: apply_reg<2>(v, m); break;
-            case 3: apply_reg<3>(v, m); break;
-            default: apply_reg<4>(v, m); break;
+                for (int e = 0; e < 4; e++) m[e] = make_float2(d.mat[g][e].re, d.mat[g][e].im);
+                switch (d.ph_greg[g]) {
+                case 0: apply_reg<0>(v, m); break;
+                case 1: apply_reg<1>(v, m); break;
+                case 2: apply_reg<2>(v, m); break;
+                case 3: apply_reg<3>(v, m); break;
+                default: apply_reg<4>(v, m); break;
+                }
             }
         }
     }
+    if (rotf) RegLoop<kRegs>::phase(post_tab, Lc, v);
 
     if (d.store == QGTL_STORE_RING) {
         const int nh = d.n - kRT;
@@ -366,8 +497,9 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
                                                                  float2 *dst_all, size_t stride) {
     __shared__ float2 tile[kRegTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
+    __shared__ float2 tabs[2 * kPhaseTab];
     pass_regs_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
-                   tile, hi_off);
+                   tile, hi_off, tabs, tabs + kPhaseTab);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -585,6 +717,7 @@ __global__ __launch_bounds__(kThreads) void pass_grouped_kernel(const qgtl_pass_
                                                                 int nitems, uint32_t units) {
     __shared__ float2 tile[kLdsTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
+    __shared__ float2 tabs[2 * kPhaseTab];
     __shared__ qgtl_pass_desc_t sd;
     static_assert(kLdsTileElems >= kRegTileElems, "LDS tile too small for the register pass");
     static_assert(sizeof(qgtl_pass_desc_t) % 4 == 0, "descriptor copy works in words");
@@ -599,8 +732,8 @@ __global__ __launch_bounds__(kThreads) void pass_grouped_kernel(const qgtl_pass_
         __syncthreads();
         const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
         float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
-        if (it.variant) pass_regs_body(sd, src, dst, r, tile, hi_off);
-        else pass_lds_body(sd, src, dst, r, tile, hi_off);
+        if (it.variant) pass_regs_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
     }
 }
 

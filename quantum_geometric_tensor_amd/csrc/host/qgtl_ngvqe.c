@@ -10,7 +10,8 @@
  * step is: new angles into the plan, the mixed-precision QGT and, from the
  * same derivative states, E and grad E on the GPU; A = Re Q + lambda I; the
  * solve on the GPU (Cholesky, CG, Jacobi-PCG, or PCG with the learned
- * diagonal + low-rank preconditioner); the step.
+ * diagonal + low-rank preconditioner, falling back to Jacobi-PCG when it
+ * misses the tolerance); the step.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
@@ -80,6 +81,15 @@ int qgtl_ng_step(qgtl_plan_t *p, double *theta, const qgtl_ng_config_t *cfg, qgt
         if (rc) goto out;
     }
     rc = qgtl_solve_f64(P, 1, A, g, cfg->lambda, cfg->method, cfg->tol, cfg->maxit, d, U, rank, x, &si);
+    int fallback = 0;
+    if (!rc && cfg->method == QGTL_SOLVE_PCG && si.status != 0) {
+        /* the learned preconditioner did not reach the tolerance: Jacobi-PCG from scratch */
+        const int it = si.iterations;
+        memset(x, 0, sizeof(double) * (size_t)P);
+        rc = qgtl_solve_f64(P, 1, A, g, cfg->lambda, QGTL_SOLVE_JACOBI_PCG, cfg->tol, cfg->maxit, NULL, NULL, 0, x, &si);
+        si.iterations += it;
+        fallback = 1;
+    }
     if (rc) goto out;
     const double t2 = wall_ms();
     double gn = 0.0, xn = 0.0;
@@ -95,6 +105,7 @@ int qgtl_ng_step(qgtl_plan_t *p, double *theta, const qgtl_ng_config_t *cfg, qgt
         info->step_norm = cfg->eta * sqrt(xn);
         info->solve_iterations = si.iterations;
         info->solve_status = si.status;
+        info->fallback = fallback;
         info->solve_rel_residual = si.rel_residual;
         info->solve_device_ms = si.device_ms;
         info->qgt_ms = t1 - t0;

@@ -146,6 +146,84 @@ static void to_c64(const qgtl_mat2 *m, qgtl_c64_t out[4])
     }
 }
 
+static double cabs2(qgtl_cd z) { return sqrt(z.re * z.re + z.im * z.im); }
+static qgtl_cd cdiv_real(qgtl_cd z, double r) { return cd(z.re / r, z.im / r); }
+static qgtl_cd cconj(qgtl_cd z) { return cd(z.re, -z.im); }
+static qgtl_c64_t c64(qgtl_cd z) { qgtl_c64_t o = {(float)z.re, (float)z.im}; return o; }
+
+/* U = diag(post) [[c, -s], [s, c]] diag(pre), FP64, for a unitary 2 x 2 U.
+ * When U00 conj(U01) and U10 conj(U11) are real (U = RZ RY, the ansatz's
+ * product) pre = (1, 1) and c, s carry the signs; otherwise pre0 = 1. */
+static void factor_rotation(const qgtl_mat2 *U, double *c, double *s, qgtl_cd pre[2], qgtl_cd post[2])
+{
+    const qgtl_cd a = U->m[0], b = U->m[1], cc = U->m[2], d = U->m[3];
+    const double na = cabs2(a), nb = cabs2(b), nc = cabs2(cc), nd = cabs2(d);
+    const qgtl_cd r0 = cmul(a, cconj(b)), r1 = cmul(cc, cconj(d));
+    const double tol = 1e-13;
+    pre[0] = pre[1] = cd(1.0, 0.0);
+    if (fabs(r0.im) <= tol && fabs(r1.im) <= tol) {
+        /* rows are a phase times a real row */
+        post[0] = na >= nb ? cdiv_real(a, na) : cdiv_real(cd(-b.re, -b.im), nb);
+        post[1] = nd >= nc ? cdiv_real(d, nd) : cdiv_real(cc, nc);
+        const qgtl_cd p0c = cconj(post[0]), p1c = cconj(post[1]);
+        /* c = Re(U00 / post0), s = -Re(U01 / post0); check row 1 agrees */
+        *c = cmul(a, p0c).re;
+        *s = -cmul(b, p0c).re;
+        const double c1 = cmul(d, p1c).re, s1 = cmul(cc, p1c).re;
+        if (fabs(c1 - *c) <= 1e-12 && fabs(s1 - *s) <= 1e-12) return;
+        /* row 1 differs in sign: fold it into post1 */
+        post[1] = cd(-post[1].re, -post[1].im);
+        if (fabs(-c1 - *c) <= 1e-12 && fabs(-s1 - *s) <= 1e-12) return;
+        post[1] = cd(-post[1].re, -post[1].im);
+    }
+    /* general: pre0 = 1, c = |U00| >= 0, s = |U10| >= 0 */
+    *c = na;
+    *s = nc;
+    if (na > 0.0 && nc > 0.0) {
+        post[0] = cdiv_real(a, na);
+        post[1] = cdiv_real(cc, nc);
+        pre[1] = cdiv_real(cmul(cconj(post[1]), d), na);     /* U11 = post1 c pre1 */
+    } else if (nc == 0.0) {                                  /* diagonal */
+        *c = 1.0; *s = 0.0;
+        post[0] = a;
+        post[1] = cd(1.0, 0.0);
+        pre[1] = d;
+    } else {                                                 /* anti-diagonal: U01 = -post0 pre1, U10 = post1 */
+        *c = 0.0; *s = 1.0;
+        post[1] = cc;
+        post[0] = cd(1.0, 0.0);
+        pre[1] = cd(-b.re, -b.im);
+    }
+}
+
+/* matrix g of pass p: general form always, rotation form alongside */
+static void set_gate(qgtl_pass_desc_t *p, int g, const qgtl_mat2 *U)
+{
+    to_c64(U, p->mat[g]);
+    double c, s;
+    qgtl_cd pre[2], post[2];
+    factor_rotation(U, &c, &s, pre, post);
+    p->rot[g].re = (float)c;
+    p->rot[g].im = (float)s;
+    for (int b = 0; b < 2; b++) {
+        p->pre[g][b] = c64(pre[b]);
+        p->post[g][b] = c64(post[b]);
+    }
+}
+
+/* switch a finished pass to the rotation form */
+static void finish_form(qgtl_pass_desc_t *p, int general)
+{
+    p->form = 0;
+    p->has_pre = 0;
+    if (general) return;
+    for (int g = 0; g < p->ngates; g++) {
+        if (p->pre[g][0].re != 1.f || p->pre[g][0].im != 0.f || p->pre[g][1].re != 1.f || p->pre[g][1].im != 0.f)
+            p->has_pre = 1;
+    }
+    p->form = 1;
+}
+
 /* Register-phase plan for a full 2^13 tile (see qgtl_pass_desc_t): phase 0
  * holds tile bits {0, 7, 8, 9, 10}; the remaining gate bits go five per phase
  * (topped up with other tile bits).  Gates are reordered by phase (they
@@ -190,13 +268,20 @@ static void plan_phases(qgtl_pass_desc_t *p)
     p->ph_first[nph] = (int8_t)cnt;
     p->nphase = nph;
     /* reorder matrices and their tile bits into phase order */
-    qgtl_c64_t mats[QGTL_TILE_BITS_MAX][4];
+    qgtl_c64_t mats[QGTL_TILE_BITS_MAX][4], rot[QGTL_TILE_BITS_MAX], pre[QGTL_TILE_BITS_MAX][2],
+        post[QGTL_TILE_BITS_MAX][2];
     int8_t bits[QGTL_TILE_BITS_MAX];
     for (int i = 0; i < cnt; i++) {
         memcpy(mats[i], p->mat[order[i]], sizeof mats[i]);
+        rot[i] = p->rot[order[i]];
+        memcpy(pre[i], p->pre[order[i]], sizeof pre[i]);
+        memcpy(post[i], p->post[order[i]], sizeof post[i]);
         bits[i] = p->gate_bit[order[i]];
     }
     memcpy(p->mat, mats, sizeof(qgtl_c64_t) * 4 * (size_t)cnt);
+    memcpy(p->rot, rot, sizeof(qgtl_c64_t) * (size_t)cnt);
+    memcpy(p->pre, pre, sizeof(qgtl_c64_t) * 2 * (size_t)cnt);
+    memcpy(p->post, post, sizeof(qgtl_c64_t) * 2 * (size_t)cnt);
     memcpy(p->gate_bit, bits, (size_t)cnt);
 }
 
@@ -212,7 +297,7 @@ static void build_passes(qgtl_block_t *b, int n)
         for (int q = 0; q < n; q++)
             if (b->has_u[q]) {
                 p->gate_bit[p->ngates] = (int8_t)q;
-                to_c64(&b->u[q], p->mat[p->ngates]);
+                set_gate(p, p->ngates, &b->u[q]);
                 p->ngates++;
             }
         p->store = QGTL_STORE_INPLACE;
@@ -250,7 +335,7 @@ static void build_passes(qgtl_block_t *b, int n)
             p->hi_pos[i] = (int8_t)hq[g0 + i];
             in_tile[hq[g0 + i]] = 1;
             p->gate_bit[i] = (int8_t)(c + i);
-            to_c64(&b->u[hq[g0 + i]], p->mat[i]);
+            set_gate(p, i, &b->u[hq[g0 + i]]);
         }
         p->ngates = len;
         for (int q = 0; q < n; q++)
@@ -266,7 +351,7 @@ static void build_passes(qgtl_block_t *b, int n)
     for (int q = 0; q < TB; q++)
         if (b->has_u[q]) {
             p->gate_bit[p->ngates] = (int8_t)q;
-            to_c64(&b->u[q], p->mat[p->ngates]);
+            set_gate(p, p->ngates, &b->u[q]);
             p->ngates++;
         }
     p->store = b->is_ring ? QGTL_STORE_RING : QGTL_STORE_INPLACE;
@@ -435,6 +520,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
         qgtl_block_t *b = &plan->blocks[bi];
         detect_ring(b, n);
         build_passes(b, n);
+        for (int i = 0; i < b->npasses; i++) finish_form(&b->pass[i], plan->tune.sweep_form == 1);
         for (int j = b->param_begin; j < b->param_end; j++) {
             qgtl_param_t *pp = &plan->params[order[j]];
             const int q = pp->qubit;
@@ -458,5 +544,36 @@ int qgtl_plan_build(struct qgtl_plan *plan)
     free(order);
     free(u_at);
     free(g);
+    return 0;
+}
+
+/* Diagnostics: max |diag(post) R diag(pre) - U| over every gate of every pass
+ * (FP32 descriptor values, FP64 arithmetic) and the number of passes that
+ * need a pre-phase layer. */
+int qgtl_plan_debug_form(const qgtl_plan_t *plan, double *max_err, int *passes_with_pre)
+{
+    if (!plan) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null plan");
+    double err = 0.0;
+    int npre = 0;
+    for (int bi = 0; bi < plan->nblocks; bi++)
+        for (int i = 0; i < plan->blocks[bi].npasses; i++) {
+            const qgtl_pass_desc_t *p = &plan->blocks[bi].pass[i];
+            npre += p->has_pre;
+            for (int g = 0; g < p->ngates; g++) {
+                const double c = p->rot[g].re, sn = p->rot[g].im;
+                const double R[2][2] = {{c, -sn}, {sn, c}};
+                for (int r = 0; r < 2; r++)
+                    for (int k = 0; k < 2; k++) {
+                        const qgtl_cd po = cd(p->post[g][r].re, p->post[g][r].im);
+                        const qgtl_cd pr = cd(p->pre[g][k].re, p->pre[g][k].im);
+                        const qgtl_cd v = cmul(po, cmul(cd(R[r][k], 0.0), pr));
+                        const double dr = v.re - p->mat[g][2 * r + k].re, di = v.im - p->mat[g][2 * r + k].im;
+                        const double e = sqrt(dr * dr + di * di);
+                        if (e > err) err = e;
+                    }
+            }
+        }
+    if (max_err) *max_err = err;
+    if (passes_with_pre) *passes_with_pre = npre;
     return 0;
 }

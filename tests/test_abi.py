@@ -98,3 +98,46 @@ def test_batch_validation_errors():
         Q.Batch([(2, no_params)])
     with pytest.raises(Q.QGTLError, match="instance 1"):
         Q.Batch([(2, Q.hea_ops(2, 1, np.ones(4))), (2, Q.make_ops([("RY", 4, -1, 0, 0.1)]))])
+
+
+def _form(plan):
+    err = ctypes.c_double(0.0)
+    npre = ctypes.c_int(0)
+    lib = Q.lib()
+    lib.qgtl_plan_debug_form.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]
+    assert lib.qgtl_plan_debug_form(plan._h, ctypes.byref(err), ctypes.byref(npre)) == 0
+    return err.value, npre.value
+
+
+def test_rotation_form_reconstructs_every_gate():
+    """Sweep passes apply each qubit's 2x2 product as diag(post) [[c,-s],[s,c]] diag(pre);
+    the factors reproduce the general matrices to fp32 rounding (host only)."""
+    rng = np.random.default_rng(9)
+    # hardware-efficient ansatz: RZ.RY products need no pre-phase layer
+    for n in (3, 13, 16):
+        p = Q.Plan(n, Q.hea_ops(n, 3, rng.uniform(0, 2 * np.pi, 6 * n)))
+        err, npre = _form(p)
+        assert err <= 2e-7 and npre == 0, (n, err, npre)
+        p.close()
+    # general gate mix: H, S, T, SX, RX, Y, Z, PHASE products, anti-diagonal and diagonal cases
+    names = ["H", "S", "T", "SX", "RX", "RY", "RZ", "X", "Y", "Z", "SDG", "TDG"]
+    for n in (4, 14):
+        gates, k = [], 0
+        for layer in range(4):
+            for q in range(n):
+                for _ in range(3):
+                    name = names[rng.integers(len(names))]
+                    if name in ("RX", "RY", "RZ"):
+                        gates.append((name, q, -1, k, float(rng.uniform(0, 2 * np.pi))))
+                        k += 1
+                    else:
+                        gates.append((name, q, -1, -1, 0.0))
+            gates += [("CNOT", q, (q + 1) % n, -1, 0.0) for q in range(n)]
+        p = Q.Plan(n, Q.make_ops(gates))
+        err, npre = _form(p)
+        assert err <= 4e-7, (n, err)
+        p.close()
+    # lone X and Z: anti-diagonal and diagonal matrices
+    p = Q.Plan(2, Q.make_ops([("X", 0, -1, -1, 0.0), ("Z", 1, -1, -1, 0.0), ("RY", 0, -1, 0, 0.3)]))
+    assert _form(p)[0] <= 2e-7
+    p.close()

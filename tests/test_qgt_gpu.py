@@ -101,3 +101,35 @@ def test_pauli_strings_qgtl_convention():
         ep = sum(c * O.pauli_expectation(n, O.run(n, O.hea_ops(n, L, tp)), s) for s, c in zip(strings, coeffs))
         em = sum(c * O.pauli_expectation(n, O.run(n, O.hea_ops(n, L, tm)), s) for s, c in zip(strings, coeffs))
         assert abs(g[k] - (ep - em) / 2e-5) < 1e-6
+
+
+def _mixed_circuit(n, layers, seed):
+    """Random single-qubit gate mix (with RX/RY/RZ parameters) and a CNOT ring per layer."""
+    rng = np.random.default_rng(seed)
+    names = ["H", "S", "T", "SX", "RX", "RY", "RZ", "X", "Y", "Z", "SDG", "TDG"]
+    gates, k = [], 0
+    for _ in range(layers):
+        for q in range(n):
+            for _ in range(2):
+                name = names[rng.integers(len(names))]
+                if name in ("RX", "RY", "RZ"):
+                    gates.append((name, q, -1, k, float(rng.uniform(0, 2 * np.pi))))
+                    k += 1
+                else:
+                    gates.append((name, q, -1, -1, 0.0))
+        gates += [("CNOT", q, (q + 1) % n, -1, 0.0) for q in range(n)]
+    return gates
+
+
+@pytest.mark.parametrize("n", [5, 13, 14])
+@pytest.mark.parametrize("form", [0, 1])
+def test_gate_mix_both_sweep_forms(n, form):
+    """Rotation form (real rotations + phase tables, pre-phase layers for general gates)
+    and the general-matrix form against the oracle on a gate mix."""
+    gates = _mixed_circuit(n, 3, 40 + n)
+    ops, oops = Q.make_ops(gates), O.make_ops(gates)
+    ref = O.qgt(n, oops)
+    p = Q.Plan(n, ops, Q.MODE_MIXED, sweep_form=form)
+    got, _ = p.execute()
+    p.close()
+    assert Q.rel_err(got, ref) <= MIXED_TOL
