@@ -93,6 +93,33 @@ def batched_leg(Q, reps=10, warmup=2):
     return out
 
 
+def large_leg(Q, reps=2):
+    """Large single instance: n=26, L=8, P=416 (complex64 states of 512 MB; the 416
+    derivative states, 223 GB, stay resident in HBM), theta ~ U[0,2pi) seed 0."""
+    n, L = 26, 8
+    theta = splitmix_angles(0, 2 * n * L)
+    plan = Q.Plan(n, Q.hea_ops(n, L, theta), Q.MODE_MIXED)
+    t0 = time.perf_counter()
+    plan.execute()
+    first = time.perf_counter() - t0
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        plan.execute()
+        walls.append(time.perf_counter() - t0)
+    Q.set_profiling(True)
+    _, st = plan.execute()
+    Q.set_profiling(False)
+    plan.close()
+    wall = float(np.median(walls))
+    return {"workload": "HEA QGT n=26 L=8 P=416", "value": 1.0 / wall, "unit": "QGT/s", "ms_per_qgt": wall * 1e3,
+            "first_call_ms": first * 1e3,
+            "breakdown_ms": {"sweep": st["sweep_ms"], "inject": st["inject_ms"], "gram": st["gram_ms"],
+                             "launches": st["kernel_launches"]},
+            "sweep_GBps": st["sweep_bytes"] / (st["sweep_ms"] * 1e-3) / 1e9 if st["sweep_ms"] else None,
+            "gram_tflops": st["gram_flops"] / (st["gram_kernel_ms"] * 1e-3) / 1e12 if st["gram_kernel_ms"] else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +131,9 @@ def main():
                     help="time only the mixed-precision path (for profiler runs): no FP64 legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-instance batched leg")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (for profiler runs)")
+    ap.add_argument("--large", action="store_true",
+                    help="also time the large single instance n=26, L=8 (P=416, ~225 GB of device memory)")
+    ap.add_argument("--large-only", action="store_true", help="run only the large single instance")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain copy kernel of this size (traffic-counter reference)")
     args = ap.parse_args()
@@ -124,6 +154,9 @@ def main():
                           "GBps": 2 * nbytes / (ms.value * 1e-3) / 1e9}), flush=True)
     if args.batched_only:
         print(json.dumps(batched_leg(Q)), flush=True)
+        return
+    if args.large_only:
+        print(json.dumps(large_leg(Q)), flush=True)
         return
     n, L = N_QUBITS, LAYERS
     theta = splitmix_angles(0, 2 * n * L)
@@ -214,6 +247,8 @@ def main():
     }
     if not args.no_batched:
         line["batched"] = batched_leg(Q)
+    if args.large:
+        line["large"] = large_leg(Q)
     print(json.dumps(line), flush=True)
 
 
