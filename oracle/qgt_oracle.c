@@ -8,6 +8,7 @@
 #include "qgt_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -211,7 +212,13 @@ static int derivative_sweep(int n, int nops, const oracle_op_t *ops, const int *
     if (!active) return -1;
     memset(psi, 0, 2 * dim * sizeof(double));
     psi[0] = 1.0;
+    /* progress lines on stderr for long sweeps (2^24 amplitudes and up) */
+    const int progress = n >= 24 && getenv("QGT_ORACLE_PROGRESS") != NULL;
     for (int g = 0; g < nops; g++) {
+        if (progress && g % 32 == 0) {
+            fprintf(stderr, "oracle derivative sweep: gate %d of %d, %d states\n", g, nops, nact + 1);
+            fflush(stderr);
+        }
         for (int a = 0; a < nact; a++)
             if (oracle_apply_op(n, D + (size_t)2 * dim * active[a], &ops[g]) != 0) { free(active); return -1; }
         if (oracle_apply_op(n, psi, &ops[g]) != 0) { free(active); return -1; }

@@ -50,6 +50,7 @@ def _check(n, L, block, seed_rows=1):
         perm[i], perm[j] = perm[j], perm[i]
     rows = np.array(sorted(perm[:block]), np.int32)
     cols = np.array(sorted(perm[block:2 * block]), np.int32)
+    os.environ.setdefault("QGT_ORACLE_PROGRESS", "1")
     ref = O.qgt_block(n, O.hea_ops(n, L, theta), rows, cols)
     sub = Qm[np.ix_(rows, cols)]
     err = np.max(np.abs(sub - ref)) / np.max(np.abs(Qm))
