@@ -43,12 +43,13 @@ typedef struct {
     int32_t ngates;                            /* matrices in this pass (<= m) */
     int32_t nperm;                             /* QGTL_STORE_PERM: gates in perm_* */
     int32_t pad0;
-    int8_t hi_pos[QGTL_TILE_BITS_MAX];         /* physical bit of tile bit nlow+i */
-    int8_t rest_pos[QGTL_MAX_QUBITS];          /* physical bit of tile-index bit i */
-    int8_t gate_bit[QGTL_TILE_BITS_MAX];       /* tile bit of matrix g */
-    int8_t perm_type[QGTL_PERM_MAX];           /* 0 = CNOT(a -> b), 1 = SWAP(a, b) */
-    int8_t perm_a[QGTL_PERM_MAX];
-    int8_t perm_b[QGTL_PERM_MAX];
+    /* Index arrays are 32-bit so that kernels read them with scalar loads. */
+    int32_t hi_pos[QGTL_TILE_BITS_MAX];        /* physical bit of tile bit nlow+i */
+    int32_t rest_pos[QGTL_MAX_QUBITS];         /* physical bit of tile-index bit i */
+    int32_t gate_bit[QGTL_TILE_BITS_MAX];      /* tile bit of matrix g */
+    int32_t perm_type[QGTL_PERM_MAX];          /* 0 = CNOT(a -> b), 1 = SWAP(a, b) */
+    int32_t perm_a[QGTL_PERM_MAX];
+    int32_t perm_b[QGTL_PERM_MAX];
     qgtl_c64_t mat[QGTL_TILE_BITS_MAX][4];     /* row-major 2x2 */
     /* Register-phase plan (full 2^13 tiles): 32 amplitudes per thread; in
      * phase p the tile bits ph_rbits[p][0..4] index a thread's registers and
@@ -56,9 +57,19 @@ typedef struct {
      * bit ph_greg[g].  Phase 0 is the coalesced layout {0, 7, 8, 9, 10}.
      * nphase = 0: no plan (the LDS-only kernel runs the pass). */
     int32_t nphase;
-    int8_t ph_rbits[QGTL_MAX_PHASES][5];
-    int8_t ph_first[QGTL_MAX_PHASES + 1];
-    int8_t ph_greg[QGTL_TILE_BITS_MAX];
+    int32_t ph_rbits[QGTL_MAX_PHASES][5];
+    /* thread bit i of phase p's layout is tile bit ph_tbits[p][i]; the first four
+     * (one quarter-wave) are chosen so that the transposes through the
+     * XOR-swizzled LDS tile hit 16 distinct 8-byte banks */
+    int32_t ph_tbits[QGTL_MAX_PHASES][8];
+    int32_t ph_first[QGTL_MAX_PHASES + 1];
+    int32_t ph_greg[QGTL_TILE_BITS_MAX];
+    /* the same plan by register bit: in phase p register bit j carries a gate
+     * iff bit j of ph_mask[p] is set, with rotation ph_rot[p][j] (form 1) or
+     * matrix ph_mat[p][j] (form 0) */
+    int32_t ph_mask[QGTL_MAX_PHASES];
+    qgtl_c64_t ph_rot[QGTL_MAX_PHASES][5];
+    qgtl_c64_t ph_mat[QGTL_MAX_PHASES][5][4];
     /* Rotation form (form = 1): matrix g = diag(post[g]) R_g diag(pre[g]) with the
      * real rotation R_g = [[c, -s], [s, c]], c = rot[g].re, s = rot[g].im;
      * pre/post[g][b] is the unit phase on tile-bit value b.  A pass then costs

@@ -67,9 +67,14 @@ __device__ __forceinline__ void build_phase_tables(const qgtl_pass_desc_t &d, fl
         for (int g = 0; g < d.ngates; g++) {
             const int bit = d.gate_bit[g];
             if (lo ? bit >= 7 : bit < 7) continue;
-            const int v = (i >> (lo ? bit : bit - 7)) & 1;
-            a = cmul(a, make_float2(d.post[g][v].re, d.post[g][v].im));
-            b = cmul(b, make_float2(d.pre[g][v].re, d.pre[g][v].im));
+            const bool v = (i >> (lo ? bit : bit - 7)) & 1;
+            // uniform loads of both entries, then a per-lane select
+            const float2 p0 = make_float2(d.post[g][0].re, d.post[g][0].im);
+            const float2 p1 = make_float2(d.post[g][1].re, d.post[g][1].im);
+            const float2 q0 = make_float2(d.pre[g][0].re, d.pre[g][0].im);
+            const float2 q1 = make_float2(d.pre[g][1].re, d.pre[g][1].im);
+            a = cmul(a, v ? p1 : p0);
+            b = cmul(b, v ? q1 : q0);
         }
         post_tab[t] = a;
         pre_tab[t] = b;
@@ -295,7 +300,11 @@ __global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc
 constexpr int kRT = 13;
 constexpr int kRegs = 32;
 
-__device__ __forceinline__ int rpad(uint32_t j) { return (int)(j + (j >> 5)); }
+// XOR swizzle of the register kernel's LDS tile: bits 0..3 of j are XORed with
+// bits 4..7, 8..11 and 12, so a quarter-wave whose four lane bits have
+// independent bank vectors (the host picks such thread-bit orders) is
+// conflict-free for 8-byte accesses.
+__device__ __forceinline__ int rpad(uint32_t j) { return (int)(j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15u)); }
 
 // Tile index of register r for a thread whose other-bit deposit is tdep.
 struct RegLayout {
@@ -303,22 +312,13 @@ struct RegLayout {
     uint32_t rb[5];
 };
 
-__device__ __forceinline__ RegLayout make_layout(const int8_t *rbits, uint32_t t) {
+__device__ __forceinline__ RegLayout make_layout(const int32_t *rbits, const int32_t *tbits, uint32_t t) {
     RegLayout L;
-    uint32_t mask = 0;
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
-        L.rb[i] = 1u << rbits[i];
-        mask |= L.rb[i];
-    }
+    for (int i = 0; i < 5; i++) L.rb[i] = 1u << rbits[i];
     uint32_t d = 0;
-    int k = 0;
 #pragma unroll
-    for (int b = 0; b < kRT; b++)
-        if (!((mask >> b) & 1u)) {
-            d |= ((t >> k) & 1u) << b;
-            k++;
-        }
+    for (int i = 0; i < 8; i++) d |= ((t >> i) & 1u) << tbits[i];
     L.tdep = d;
     return L;
 }
@@ -344,6 +344,22 @@ __device__ __forceinline__ void apply_reg_rot(float2 (&v)[kRegs], float c, float
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
         if (!((r >> P) & 1)) rot2(v[r], v[r | (1 << P)], c, s);
+}
+
+// Gate of register bit J in phase ph, if any (scalar branch, compile-time pairing).
+template <int J>
+__device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const qgtl_pass_desc_t &d, int ph, int mask) {
+    if ((mask >> J) & 1) apply_reg_rot<J>(v, d.ph_rot[ph][J].re, d.ph_rot[ph][J].im);
+}
+
+template <int J>
+__device__ __forceinline__ void apply_phase_mat(float2 (&v)[kRegs], const qgtl_pass_desc_t &d, int ph, int mask) {
+    if ((mask >> J) & 1) {
+        float2 m[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) m[e] = make_float2(d.ph_mat[ph][J][e].re, d.ph_mat[ph][J][e].im);
+        apply_reg<J>(v, m);
+    }
 }
 
 template <int R>
@@ -410,7 +426,7 @@ struct RegLoop<0> {
                                                              const float2 (&)[kRegs]) {}
 };
 
-constexpr int kRegTileElems = (1 << kRT) + (1 << kRT) / 32;
+constexpr int kRegTileElems = 1 << kRT;
 
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
@@ -431,42 +447,33 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     const uint32_t lowmask = (1u << nlow) - 1u;
 
     float2 v[kRegs];
-    const RegLayout L0 = make_layout(d.ph_rbits[0], t);
+    const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], t);
     RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
     if (rotf && d.has_pre) RegLoop<kRegs>::phase(pre_tab, L0, v);
 
     RegLayout Lc = L0;
     for (int ph = 0; ph < d.nphase; ph++) {
         if (ph > 0) {
-            const RegLayout Ln = make_layout(d.ph_rbits[ph], t);
+            const RegLayout Ln = make_layout(d.ph_rbits[ph], d.ph_tbits[ph], t);
             RegLoop<kRegs>::store_lds(tile, Lc, v);
             __syncthreads();
             RegLoop<kRegs>::load_lds(tile, Ln, v);
             __syncthreads();
             Lc = Ln;
         }
-        for (int g = d.ph_first[ph]; g < d.ph_first[ph + 1]; g++) {
-            if (rotf) {
-                const float c = d.rot[g].re, sn = d.rot[g].im;
-                switch (d.ph_greg[g]) {
-                case 0: apply_reg_rot<0>(v, c, sn); break;
-                case 1: apply_reg_rot<1>(v, c, sn); break;
-                case 2: apply_reg_rot<2>(v, c, sn); break;
-                case 3: apply_reg_rot<3>(v, c, sn); break;
-                default: apply_reg_rot<4>(v, c, sn); break;
-                }
-            } else {
-                float2 m[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) m[e] = make_float2(d.mat[g][e].re, d.mat[g][e].im);
-                switch (d.ph_greg[g]) {
-                case 0: apply_reg<0>(v, m); break;
-                case 1: apply_reg<1>(v, m); break;
-                case 2: apply_reg<2>(v, m); break;
-                case 3: apply_reg<3>(v, m); break;
-                default: apply_reg<4>(v, m); break;
-                }
-            }
+        const int mask = d.ph_mask[ph];
+        if (rotf) {
+            apply_phase_rot<0>(v, d, ph, mask);
+            apply_phase_rot<1>(v, d, ph, mask);
+            apply_phase_rot<2>(v, d, ph, mask);
+            apply_phase_rot<3>(v, d, ph, mask);
+            apply_phase_rot<4>(v, d, ph, mask);
+        } else {
+            apply_phase_mat<0>(v, d, ph, mask);
+            apply_phase_mat<1>(v, d, ph, mask);
+            apply_phase_mat<2>(v, d, ph, mask);
+            apply_phase_mat<3>(v, d, ph, mask);
+            apply_phase_mat<4>(v, d, ph, mask);
         }
     }
     if (rotf) RegLoop<kRegs>::phase(post_tab, Lc, v);

@@ -224,6 +224,74 @@ static void finish_form(qgtl_pass_desc_t *p, int general)
     p->form = 1;
 }
 
+/* LDS bank of tile index j in the register kernel's swizzled tile:
+ * ((j ^ (j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15), a GF(2)-linear map, so tile
+ * bit b lands on bank bit (b & 3).  For the ring store the written index is
+ * the linear ring image of j. */
+static uint32_t bank_vec(uint32_t j)
+{
+    return (j ^ (j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15u;
+}
+
+static uint32_t ring_image(uint32_t p)
+{
+    p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
+    p &= (1u << QGTL_TILE_BITS_MAX) - 1u;
+    return p ^ ((p >> (QGTL_TILE_BITS_MAX - 1)) & 1u);
+}
+
+static int independent4(const uint32_t v[4])
+{
+    uint32_t r[4] = {v[0], v[1], v[2], v[3]};
+    for (int c = 0; c < 4; c++) {
+        int piv = -1;
+        for (int i = c; i < 4; i++)
+            if ((r[i] >> c) & 1u) { piv = i; break; }
+        if (piv < 0) {
+            /* no pivot in column c among the remaining rows: look for any set bit */
+            return 0;
+        }
+        const uint32_t t = r[c]; r[c] = r[piv]; r[piv] = t;
+        for (int i = 0; i < 4; i++)
+            if (i != c && ((r[i] >> c) & 1u)) r[i] ^= r[c];
+    }
+    return 1;
+}
+
+/* Thread-bit order of a layout with register bits rbits: the first four
+ * thread bits (lanes 0..15) map to tile bits whose bank vectors are
+ * independent, also through the ring map when the layout feeds a ring
+ * store; phase 0 keeps ascending order (coalesced global access). */
+static void choose_thread_bits(const int32_t rbits[5], int ascending, int ring, int32_t tbits[8])
+{
+    int others[QGTL_TILE_BITS_MAX], no = 0;
+    for (int b = 0; b < QGTL_TILE_BITS_MAX; b++) {
+        int isr = 0;
+        for (int i = 0; i < 5; i++) isr |= rbits[i] == b;
+        if (!isr) others[no++] = b;
+    }
+    for (int i = 0; i < 8; i++) tbits[i] = others[i];
+    if (ascending) return;
+    for (int a = 0; a < no; a++)
+        for (int b = a + 1; b < no; b++)
+            for (int c = b + 1; c < no; c++)
+                for (int d = c + 1; d < no; d++) {
+                    const int pick[4] = {others[a], others[b], others[c], others[d]};
+                    uint32_t v[4], w[4];
+                    for (int i = 0; i < 4; i++) {
+                        v[i] = bank_vec(1u << pick[i]);
+                        w[i] = bank_vec(ring_image(1u << pick[i]));
+                    }
+                    if (!independent4(v) || (ring && !independent4(w))) continue;
+                    int k = 0;
+                    for (int i = 0; i < 4; i++) tbits[k++] = pick[i];
+                    for (int i = 0; i < no; i++)
+                        if (others[i] != pick[0] && others[i] != pick[1] && others[i] != pick[2] && others[i] != pick[3])
+                            tbits[k++] = others[i];
+                    return;
+                }
+}
+
 /* Register-phase plan for a full 2^13 tile (see qgtl_pass_desc_t): phase 0
  * holds tile bits {0, 7, 8, 9, 10}; the remaining gate bits go five per phase
  * (topped up with other tile bits).  Gates are reordered by phase (they
@@ -237,7 +305,7 @@ static void plan_phases(qgtl_pass_desc_t *p)
     for (int i = 0; i < 5; i++) in0[ph0[i]] = 1;
     int order[QGTL_TILE_BITS_MAX], cnt = 0;
     int rest[QGTL_TILE_BITS_MAX], nrest = 0;
-    memcpy(p->ph_rbits[0], ph0, 5);
+    for (int i = 0; i < 5; i++) p->ph_rbits[0][i] = ph0[i];
     p->ph_first[0] = 0;
     for (int g = 0; g < p->ngates; g++)
         if (in0[p->gate_bit[g]]) {
@@ -270,7 +338,7 @@ static void plan_phases(qgtl_pass_desc_t *p)
     /* reorder matrices and their tile bits into phase order */
     qgtl_c64_t mats[QGTL_TILE_BITS_MAX][4], rot[QGTL_TILE_BITS_MAX], pre[QGTL_TILE_BITS_MAX][2],
         post[QGTL_TILE_BITS_MAX][2];
-    int8_t bits[QGTL_TILE_BITS_MAX];
+    int32_t bits[QGTL_TILE_BITS_MAX];
     for (int i = 0; i < cnt; i++) {
         memcpy(mats[i], p->mat[order[i]], sizeof mats[i]);
         rot[i] = p->rot[order[i]];
@@ -282,7 +350,20 @@ static void plan_phases(qgtl_pass_desc_t *p)
     memcpy(p->rot, rot, sizeof(qgtl_c64_t) * (size_t)cnt);
     memcpy(p->pre, pre, sizeof(qgtl_c64_t) * 2 * (size_t)cnt);
     memcpy(p->post, post, sizeof(qgtl_c64_t) * 2 * (size_t)cnt);
-    memcpy(p->gate_bit, bits, (size_t)cnt);
+    memcpy(p->gate_bit, bits, sizeof(int32_t) * (size_t)cnt);
+    /* thread-bit orders (the last layout also writes the ring store) */
+    for (int ph = 0; ph < nph; ph++)
+        choose_thread_bits(p->ph_rbits[ph], ph == 0, ph == nph - 1 && p->store == QGTL_STORE_RING, p->ph_tbits[ph]);
+    /* by register bit, for the kernels */
+    for (int ph = 0; ph < nph; ph++) {
+        p->ph_mask[ph] = 0;
+        for (int g = p->ph_first[ph]; g < p->ph_first[ph + 1]; g++) {
+            const int j = p->ph_greg[g];
+            p->ph_mask[ph] |= 1 << j;
+            p->ph_rot[ph][j] = p->rot[g];
+            memcpy(p->ph_mat[ph][j], p->mat[g], sizeof(qgtl_c64_t) * 4);
+        }
+    }
 }
 
 static void build_passes(qgtl_block_t *b, int n)
@@ -305,9 +386,11 @@ static void build_passes(qgtl_block_t *b, int n)
             if (b->nperm <= QGTL_PERM_MAX) {
                 p->store = QGTL_STORE_PERM;
                 p->nperm = b->nperm;
-                memcpy(p->perm_type, b->perm_type, (size_t)b->nperm);
-                memcpy(p->perm_a, b->perm_a, (size_t)b->nperm);
-                memcpy(p->perm_b, b->perm_b, (size_t)b->nperm);
+                for (int i = 0; i < b->nperm; i++) {
+                    p->perm_type[i] = b->perm_type[i];
+                    p->perm_a[i] = b->perm_a[i];
+                    p->perm_b[i] = b->perm_b[i];
+                }
             } else {
                 b->mstage_kernels = 1;
             }
