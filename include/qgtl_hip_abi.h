@@ -161,7 +161,8 @@ typedef struct {
 
 /* grid <= 0: one workgroup per unit (capped at 2^31 - 1) */
 int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t units, int grid);
-int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid);
+/* variant: every item of the launch has this qgtl_pass_item_t.variant */
+int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid, int variant);
 int qgtl_hip_inject_grouped(const qgtl_inject_item_t *items, int nitems, uint32_t units, int grid);
 int qgtl_hip_reduce_grouped(const qgtl_reduce_item_t *items, int nitems, uint32_t units, int grid);
 /* grid: multiple of 8 (XCD affinity of a chunk's tiles) */

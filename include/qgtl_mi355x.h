@@ -126,7 +126,8 @@ typedef struct {
     int32_t gram_chunk_len;   /* amplitudes per split-K chunk of a Gram (0: 16384) */
     int32_t pass_variant;     /* 0: register phases where they apply; 1: LDS gate groups only */
     int32_t no_graph;         /* 1: issue the launches directly instead of replaying the graph */
-    int32_t reserved[2];
+    int32_t sweep_form;       /* 0: rotation form of the pass matrices; 1: general complex matrices */
+    int32_t reserved[1];
 } qgtl_batch_tune_t;
 
 typedef struct {

@@ -265,7 +265,7 @@ def rel_err(Q, Qref):
 class BatchTune(ctypes.Structure):
     _fields_ = [("pass_grid", ctypes.c_int32), ("inject_grid", ctypes.c_int32), ("gram_grid", ctypes.c_int32),
                 ("gram_chunk_len", ctypes.c_int32), ("pass_variant", ctypes.c_int32), ("no_graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("sweep_form", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}

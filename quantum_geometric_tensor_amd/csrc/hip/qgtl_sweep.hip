@@ -24,7 +24,11 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kHiTableBits = 9;     // tile bits outside the contiguous low run
 
-__device__ __forceinline__ int lds_pad(int j) { return j + (j >> 3); }
+// LDS index of tile element j: bits 0..3 XORed with bits 4..7 (a bijection on
+// any 2^m range), so strided gate-group and store patterns spread over the
+// banks without padding and the tile stays 64 KB (two workgroups per CU even
+// in the grouped kernel, which also holds a descriptor).
+__device__ __forceinline__ int lds_pad(int j) { return j ^ ((j >> 4) & 15); }
 
 __device__ __forceinline__ float2 cmadd(float2 acc, float2 a, float2 b) {
     acc.x = fmaf(a.x, b.x, acc.x);
@@ -162,7 +166,7 @@ __device__ __forceinline__ uint32_t ring_local_inverse(uint32_t y, int m, uint32
     return rest | (x1 << 1) | x0;
 }
 
-constexpr int kLdsTileElems = (1 << QGTL_TILE_BITS_MAX) + (1 << QGTL_TILE_BITS_MAX) / 8;
+constexpr int kLdsTileElems = 1 << QGTL_TILE_BITS_MAX;
 
 // One tile r of one state: load, gate groups through LDS, store.
 __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
@@ -199,16 +203,26 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
 
     for (int g0 = 0; g0 < d.ngates; g0 += 3) {
         const int G = min(3, d.ngates - g0);
+        // apply_group inserts the gate bits in ascending order; a pass planned
+        // for the register kernel lists its gates in phase order, so sort the
+        // group's gate indices by tile bit first
+        int gidx[3];
+#pragma unroll
+        for (int i = 0; i < 3; i++) gidx[i] = min(g0 + i, d.ngates - 1);
+        if (G >= 2 && d.gate_bit[gidx[1]] < d.gate_bit[gidx[0]]) { const int t = gidx[0]; gidx[0] = gidx[1]; gidx[1] = t; }
+        if (G == 3) {
+            if (d.gate_bit[gidx[2]] < d.gate_bit[gidx[1]]) { const int t = gidx[1]; gidx[1] = gidx[2]; gidx[2] = t; }
+            if (d.gate_bit[gidx[1]] < d.gate_bit[gidx[0]]) { const int t = gidx[0]; gidx[0] = gidx[1]; gidx[1] = t; }
+        }
         int bits[3];
 #pragma unroll
-        for (int i = 0; i < 3; i++) bits[i] = d.gate_bit[min(g0 + i, d.ngates - 1)];
+        for (int i = 0; i < 3; i++) bits[i] = d.gate_bit[gidx[i]];
         if (rotf) {
             float cs[3][2];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const int gi = min(g0 + i, d.ngates - 1);
-                cs[i][0] = d.rot[gi].re;
-                cs[i][1] = d.rot[gi].im;
+                cs[i][0] = d.rot[gidx[i]].re;
+                cs[i][1] = d.rot[gidx[i]].im;
             }
             if (G == 3) apply_group_rot<3>(tile, m, bits, cs);
             else if (G == 2) apply_group_rot<2>(tile, m, bits, cs);
@@ -217,9 +231,8 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
             float2 mats[3][4];
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const int gi = min(g0 + i, d.ngates - 1);
 #pragma unroll
-                for (int e = 0; e < 4; e++) mats[i][e] = make_float2(d.mat[gi][e].re, d.mat[gi][e].im);
+                for (int e = 0; e < 4; e++) mats[i][e] = make_float2(d.mat[gidx[i]][e].re, d.mat[gidx[i]][e].im);
             }
             if (G == 3) apply_group<3>(tile, m, bits, mats);
             else if (G == 2) apply_group<2>(tile, m, bits, mats);
@@ -720,7 +733,10 @@ __global__ __launch_bounds__(kThreads) void fill_grouped_kernel(const qgtl_fill_
     }
 }
 
-__global__ __launch_bounds__(kThreads) void pass_grouped_kernel(const qgtl_pass_item_t *__restrict__ items,
+// VARIANT 1: every item runs the register-phase body; 0: the LDS body
+// (separate instantiations keep each body's register allocation).
+template <int VARIANT>
+__global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pass_item_t *__restrict__ items,
                                                                 int nitems, uint32_t units) {
     __shared__ float2 tile[kLdsTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
@@ -739,7 +755,7 @@ __global__ __launch_bounds__(kThreads) void pass_grouped_kernel(const qgtl_pass_
         __syncthreads();
         const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
         float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
-        if (it.variant) pass_regs_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        if (VARIANT) pass_regs_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
         else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
     }
 }
@@ -811,10 +827,14 @@ int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t un
     return 0;
 }
 
-int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid) {
+int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid, int variant) {
     QGTL_GROUPED_ARGS_OK(items, nitems, units);
-    hipLaunchKernelGGL(pass_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
-                       items, nitems, units);
+    if (variant)
+        hipLaunchKernelGGL(pass_grouped_kernel<1>, dim3(grouped_grid(units, grid)), dim3(kThreads), 0,
+                           qgtl::stream(), items, nitems, units);
+    else
+        hipLaunchKernelGGL(pass_grouped_kernel<0>, dim3(grouped_grid(units, grid)), dim3(kThreads), 0,
+                           qgtl::stream(), items, nitems, units);
     QGTL_LAUNCH_CHECK("pass_grouped");
     return 0;
 }

@@ -45,7 +45,7 @@ enum { BL_FILL, BL_PASS, BL_INJECT, BL_REDUCE, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL
 
 typedef struct {
     int kind;
-    int terms;                 /* BL_GRAM: split-BF16 product terms */
+    int terms;                 /* BL_GRAM: operand split; BL_PASS: kernel variant */
     int nitems;
     uint32_t units;
     size_t item_off;           /* bytes into the item arena */
@@ -264,12 +264,19 @@ static int build_schedule(struct qgtl_batch *B)
             if (b < B->inst[i].plan->nblocks && B->inst[i].plan->blocks[b].npasses > maxp)
                 maxp = B->inst[i].plan->blocks[b].npasses;
         for (int j = 0; j < maxp; j++) {
+          /* one launch per kernel variant (register-phase / LDS body) */
+          for (int var = 1; var >= 0; var--) {
             qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
             int k = 0;
             uint64_t u = 0;
             for (int i = 0; i < cnt; i++) {
                 const qgtl_plan_t *p = B->inst[i].plan;
                 if (b >= p->nblocks || j >= p->blocks[b].npasses) continue;
+                {
+                    size_t o2 = 0;
+                    for (int b2 = 0; b2 < b; b2++) o2 += (size_t)p->blocks[b2].npasses;
+                    if (regs_variant(&B->pass_h[pdesc[2 * i] + o2 + (size_t)j], B->tune.pass_variant) != var) continue;
+                }
                 const qgtl_block_t *blk = &p->blocks[b];
                 const size_t di = pdesc[2 * i];
                 size_t o = 0;
@@ -298,8 +305,9 @@ static int build_schedule(struct qgtl_batch *B)
                 }
                 k++;
             }
-            rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, 0, k, u);
+            rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, var, k, u);
             if (rc) goto out;
+          }
         }
         /* injections of block b */
         qgtl_inject_item_t *it = (qgtl_inject_item_t *)tmp;
@@ -440,7 +448,9 @@ static int issue(struct qgtl_batch *B)
         const int g = grid_for(B, L);
         switch (L->kind) {
         case BL_FILL: BTRY(qgtl_hip_fill_grouped((const qgtl_fill_item_t *)items, L->nitems, L->units, g)); break;
-        case BL_PASS: BTRY(qgtl_hip_pass_grouped((const qgtl_pass_item_t *)items, L->nitems, L->units, g)); break;
+        case BL_PASS:
+            BTRY(qgtl_hip_pass_grouped((const qgtl_pass_item_t *)items, L->nitems, L->units, g, L->terms));
+            break;
         case BL_INJECT: BTRY(qgtl_hip_inject_grouped((const qgtl_inject_item_t *)items, L->nitems, L->units, g)); break;
         case BL_REDUCE: BTRY(qgtl_hip_reduce_grouped((const qgtl_reduce_item_t *)items, L->nitems, L->units, g)); break;
         case BL_GRAM:
@@ -546,7 +556,10 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
     size_t op_off = 0, xo = 0, po = 0, so = 0, co = 0, qo = 0;
     for (int i = 0; i < count; i++) {
         inst_t *in = &B->inst[i];
-        in->plan = qgtl_plan_create(n[i], nops[i], ops + op_off, QGTL_MODE_MIXED, NULL);
+        qgtl_tune_t pt;
+        memset(&pt, 0, sizeof pt);
+        pt.sweep_form = B->tune.sweep_form == 1 ? 1 : 0;
+        in->plan = qgtl_plan_create(n[i], nops[i], ops + op_off, QGTL_MODE_MIXED, &pt);
         if (!in->plan) {
             char msg[512];
             snprintf(msg, sizeof msg, "%s", qgtl_last_error());
@@ -649,10 +662,29 @@ int qgtl_batch_set_angles(qgtl_batch_t *B, const double *angles)
     return 0;
 }
 
+/* plans follow the tune's sweep form (descriptors rebuilt on the host) */
+static int apply_sweep_form(struct qgtl_batch *B)
+{
+    const int form = B->tune.sweep_form == 1 ? 1 : 0;
+    int changed = 0;
+    for (int i = 0; i < B->count; i++) {
+        qgtl_plan_t *p = B->inst[i].plan;
+        if (p->tune.sweep_form == form) continue;
+        p->tune.sweep_form = form;
+        const int rc = qgtl_plan_build(p);
+        if (rc) return rc;
+        changed = 1;
+    }
+    if (changed) fill_descs(B);
+    return 0;
+}
+
 int qgtl_batch_set_tune(qgtl_batch_t *B, const qgtl_batch_tune_t *tune)
 {
     if (!B || !tune) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch or tune");
     B->tune = *tune;
+    const int rc = apply_sweep_form(B);
+    if (rc) return rc;
     return rebuild(B);
 }
 
@@ -736,7 +768,9 @@ int qgtl_batch_execute(qgtl_batch_t *B, double *Q, qgtl_batch_stats_t *st)
 static int time_tune(struct qgtl_batch *B, const qgtl_batch_tune_t *t, int reps, double *ms)
 {
     B->tune = *t;
-    int rc = rebuild(B);
+    int rc = apply_sweep_form(B);
+    if (rc) return rc;
+    rc = rebuild(B);
     if (rc) return rc;
     double v[16];
     if (reps > 16) reps = 16;
@@ -770,12 +804,14 @@ int qgtl_batch_autotune(qgtl_batch_t *B, int reps, const char *json_path, qgtl_b
     const int inj_grid[] = {4 * c, 8 * c, 16 * c, -1};
     const int chunk[] = {4096, 8192, 16384, 32768, 65536};
     const int gram_grid[] = {c, 2 * c, 4 * c, -1};
+    const int form[] = {0, 1};
     struct { int *field; const int *vals; int nv; } knobs[] = {
         {&best.pass_grid, pass_grid, 4},
         {&best.pass_variant, variant, 2},
         {&best.inject_grid, inj_grid, 4},
         {&best.gram_chunk_len, chunk, 5},
         {&best.gram_grid, gram_grid, 4},
+        {&best.sweep_form, form, 2},
     };
     for (size_t k = 0; k < sizeof knobs / sizeof knobs[0]; k++) {
         const int keep = *knobs[k].field;
@@ -792,7 +828,8 @@ int qgtl_batch_autotune(qgtl_batch_t *B, int reps, const char *json_path, qgtl_b
         *knobs[k].field = chosen;
     }
     B->tune = best;
-    rc = rebuild(B);
+    rc = apply_sweep_form(B);
+    if (!rc) rc = rebuild(B);
     if (rc) return rc;
     if (best_out) *best_out = best;
     if (json_path) {
@@ -800,9 +837,10 @@ int qgtl_batch_autotune(qgtl_batch_t *B, int reps, const char *json_path, qgtl_b
         if (!f) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "cannot write %s", json_path);
         fprintf(f,
                 "{\"pass_grid\": %d, \"pass_variant\": %d, \"inject_grid\": %d, \"gram_chunk_len\": %d, "
-                "\"gram_grid\": %d, \"cus\": %d, \"instances\": %d, \"device_ms\": %.4f, \"untuned_ms\": %.4f}\n",
-                best.pass_grid, best.pass_variant, best.inject_grid, best.gram_chunk_len, best.gram_grid, c,
-                B->count, best_ms, first_ms);
+                "\"gram_grid\": %d, \"sweep_form\": %d, \"cus\": %d, \"instances\": %d, \"device_ms\": %.4f, "
+                "\"untuned_ms\": %.4f}\n",
+                best.pass_grid, best.pass_variant, best.inject_grid, best.gram_chunk_len, best.gram_grid,
+                best.sweep_form, c, B->count, best_ms, first_ms);
         fclose(f);
     }
     return 0;
@@ -837,5 +875,6 @@ int qgtl_batch_tune_load(const char *json_path, qgtl_batch_tune_t *tune)
     json_int(text, "inject_grid", &tune->inject_grid);
     json_int(text, "gram_chunk_len", &tune->gram_chunk_len);
     json_int(text, "gram_grid", &tune->gram_grid);
+    json_int(text, "sweep_form", &tune->sweep_form);
     return 0;
 }

@@ -85,7 +85,7 @@ def test_batch_autotune_roundtrip(tmp_path):
     best = b.autotune(reps=1, path=path)
     assert os.path.exists(path)
     loaded = Q.load_batch_tune(path)
-    for k in ("pass_grid", "pass_variant", "inject_grid", "gram_chunk_len", "gram_grid"):
+    for k in ("pass_grid", "pass_variant", "inject_grid", "gram_chunk_len", "gram_grid", "sweep_form"):
         assert getattr(loaded, k) == getattr(best, k)
     Qs, _ = b.execute()
     for (n, L, th), got in zip(cir, Qs):

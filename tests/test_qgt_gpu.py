@@ -133,3 +133,15 @@ def test_gate_mix_both_sweep_forms(n, form):
     got, _ = p.execute()
     p.close()
     assert Q.rel_err(got, ref) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("n", [14, 16])
+def test_lds_pass_kernel_on_register_plans(n):
+    """Batched sweeps with the LDS gate-group kernel forced for full 13-bit tiles
+    (the register kernel's phase-ordered gate lists), against the oracle."""
+    L = 2
+    th = O.uniform_angles(700 + n, 2 * n * L)
+    b = Q.Batch([(n, Q.hea_ops(n, L, th))], Q.BatchTune(pass_variant=1))
+    got = b.execute()[0][0]
+    b.close()
+    assert Q.rel_err(got, O.qgt(n, O.hea_ops(n, L, th))) <= MIXED_TOL
