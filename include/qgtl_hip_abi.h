@@ -133,7 +133,7 @@ typedef struct {
     uint64_t D;             /* P states of N complex64 */
     uint64_t partials;      /* [nchunks][T][128 * 128] fp32, T = nt (nt + 1) / 2 */
     uint64_t N;
-    uint64_t chunk_len;     /* amplitudes per chunk, multiple of 32 */
+    uint64_t chunk_len;     /* amplitudes per chunk, multiple of QGTL_GRAM_KSTEP */
     int32_t P, nt, nchunks; /* nchunks: multiple of 8 */
     uint32_t unit_begin;    /* multiple of 8; units: T * nchunks */
 } qgtl_gram_item_t;
@@ -225,6 +225,7 @@ int qgtl_hip_gram_tiles(int P);                                          /* uppe
 /* terms = 3 / 4: two bf16 pieces, 3 or 4 products; 6: exact three-piece bf16
  * split, six products; QGTL_GRAM_F16X3 / X4: two fp16 pieces of 2^15 x
  * (C is then scaled by 2^30), 3 or 4 products */
+#define QGTL_GRAM_KSTEP 64   /* amplitudes per Gram LDS stage; chunk lengths are multiples of it */
 #define QGTL_GRAM_F16X3 33
 #define QGTL_GRAM_F16X4 34
 int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms);

@@ -37,8 +37,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kThreads = 256;
 constexpr int kTile = 128;          // output tile (rows of Z)
-constexpr int kKS = 32;             // amplitudes per LDS stage
-constexpr int kRow = kKS + 8;       // bf16 per LDS row (80 B: 16-B aligned, spreads banks)
+constexpr int kKS = QGTL_GRAM_KSTEP;  // amplitudes per LDS stage
+constexpr int kRow = kKS + 8;          // 16-bit pieces per LDS row (16-B aligned, spreads banks)
 constexpr int kXcd = 8;
 constexpr int kInner = 512;       // amplitudes per MFMA accumulation before the fp32 outer add
 
@@ -107,10 +107,11 @@ __device__ __forceinline__ void tile_coords(int t, int nt, int &ti, int &tj) {
 template <int S, bool F16>
 __device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, size_t N, int s0, size_t k0,
                                            uint32_t *const *rows) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
+    constexpr int kPairs = kKS / 2;                 // amplitude pairs per row
+#pragma unroll 4
+    for (int i = 0; i < 64 * kPairs / kThreads; i++) {
         const int L = i * kThreads + threadIdx.x;
-        const int sl = L >> 4, pair = L & 15;
+        const int sl = L / kPairs, pair = L % kPairs;
         const int s = s0 + sl;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         if (s < P && k0 + 2 * pair < N) v = *reinterpret_cast<const float4 *>(D + 2 * ((size_t)s * N + k0 + 2 * pair));
@@ -244,7 +245,7 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
 }
 
 template <int TERMS>
-__global__ __launch_bounds__(kThreads) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
+__global__ __launch_bounds__(kThreads, (TERMS == 6) ? 1 : 2) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
                                                               int nt, int nchunks, size_t chunk_len,
                                                               float *__restrict__ partials) {
     __shared__ GramLds<TERMS> sm;

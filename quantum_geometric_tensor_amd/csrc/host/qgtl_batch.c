@@ -38,7 +38,7 @@
 
 #define INJECT_TILE 2048          /* amplitudes per injection / fill unit */
 #define GRAM_TILE 128
-#define GRAM_KS 32
+#define GRAM_KS QGTL_GRAM_KSTEP
 #define XCDS 8
 
 enum { BL_FILL, BL_PASS, BL_INJECT, BL_REDUCE, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL };

@@ -7,8 +7,9 @@
  *   later blocks; then G = D^H D on MFMA and Q = normalised G - a a^H in FP64.
  *   When every pass is in place (states of <= 2^13 amplitudes) the derivative
  *   states advance breadth-first, one launch per pass for all of them; wider
- *   states advance depth-first in chunks, so a chunk's working set stays in
- *   the 256 MB Infinity Cache between passes.
+ *   states advance depth-first in balanced chunks of up to 1 GB (a chunk's states
+ *   move together through the later blocks, so launches stay large while
+ *   the ping-pong copy of a chunk is all the extra memory needed).
  * Plain FP64 mode: one complex128 kernel per gate, derivative states one at a
  *   time (gate insertion of -i/2 G), tiled FP64 Gram, FP64 correction.
  *
@@ -293,7 +294,10 @@ static int any_flip(const struct qgtl_plan *p)
 static int chunk_states(const struct qgtl_plan *p)
 {
     if (p->tune.chunk_states > 0) return p->tune.chunk_states < QGTL_INJECT_MAX ? p->tune.chunk_states : QGTL_INJECT_MAX;
-    size_t c = ((size_t)64 << 20) / (p->N * 8);
+    /* larger chunks mean fewer, larger launches: at n = 20 (8 MB states) one QGT
+     * took 10.3 ms with 8 states per chunk, 9.4 ms with 16, 8.9 ms with 24 and
+     * 8.5 ms with one chunk per block (40 states) */
+    size_t c = ((size_t)1 << 30) / (p->N * 8);
     if (c < 1) c = 1;
     if (c > QGTL_INJECT_MAX) c = QGTL_INJECT_MAX;
     return (int)c;
@@ -459,7 +463,9 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
         } else {
             int flips_after = 0;
             for (int b2 = bi + 1; b2 < p->nblocks; b2++) flips_after += block_flips(&p->blocks[b2]);
-            const int C = (int)p->tmp_states;
+            /* balanced chunks of at most tmp_states states */
+            const int nch = (K + (int)p->tmp_states - 1) / (int)p->tmp_states;
+            const int C = (K + nch - 1) / nch;
             for (int j0 = 0; j0 < K; j0 += C) {
                 const int cnt = (K - j0 < C) ? K - j0 : C;
                 void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
