@@ -270,6 +270,13 @@ int qgtl_hip_cholesky_solve_f64(double *A, const double *b, double *x, int P, in
 int qgtl_hip_pcg_f64(const double *A, const double *b, double *x, int P, int batch, double lambda, const double *d,
                      const double *U, int r, double tol, int maxit, int *iters, double *relres);
 int qgtl_hip_gram_small_c128(const void *M, void *B, int P);
+/* device-resident natural-gradient step: A = Re Q (slot -> parameter order) + lambda I, D = diag A,
+ * g = gradient (eg = [E, grad in slot order]); learned-preconditioner features and output transform */
+int qgtl_hip_ng_assemble(const double *Q, const int *slot, int P, double lambda, const double *eg, double *A,
+                         double *D, double *g);
+int qgtl_hip_pc_features(const double *theta, const double *D, double field, int P, float *x);
+int qgtl_hip_pc_transform(const float *y, const double *D, int P, int r, double *d, double *U);
+int qgtl_hip_reciprocal(const double *D, int P, double *d);
 int qgtl_hip_pinv_apply_c128(const void *V, const double *ev, const void *g, void *x, int P, double cutoff);
 
 #ifdef __cplusplus

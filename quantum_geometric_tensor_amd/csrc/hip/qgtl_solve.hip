@@ -431,9 +431,87 @@ __global__ __launch_bounds__(kT) void pinv_apply_c128_kernel(const double2 *V, c
     }
 }
 
+// ---- device-resident natural-gradient step ----
+// A[k][l] = Re Q[s(k)][s(l)] + lambda delta_kl (Q: P x P complex in D-slot order),
+// D[k] = A[k][k], g[k] = eg[1 + s(k)]: parameter order from slot order.
+__global__ void ng_assemble_kernel(const double *__restrict__ Q, const int *__restrict__ slot, int P, double lambda,
+                                   const double *__restrict__ eg, double *A, double *D, double *g) {
+    const size_t total = (size_t)P * P;
+    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total; idx += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(idx / P), l = (int)(idx % P);
+        double v = Q[2 * ((size_t)slot[k] * P + slot[l])];
+        if (k == l) {
+            v += lambda;
+            D[k] = v;
+            g[k] = eg[1 + slot[k]];
+        }
+        A[idx] = v;
+    }
+}
+
+// learned-preconditioner features: sin theta, cos theta, log D, field
+__global__ void pc_features_kernel(const double *__restrict__ theta, const double *__restrict__ D, double field,
+                                   int P, float *x) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= P; k += gridDim.x * blockDim.x) {
+        if (k == P) {
+            x[3 * P] = (float)field;
+        } else {
+            x[k] = (float)sin(theta[k]);
+            x[P + k] = (float)cos(theta[k]);
+            x[2 * P + k] = (float)log(D[k]);
+        }
+    }
+}
+
+// network output y (P (1 + r)) -> d = e^s / D, U = u / sqrt(P D)
+__global__ void pc_transform_kernel(const float *__restrict__ y, const double *__restrict__ D, int P, int r,
+                                    double *d, double *U) {
+    const int st = 1 + r;
+    const double us = 1.0 / sqrt((double)P);
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) {
+        d[k] = exp((double)y[(size_t)k * st]) / D[k];
+        const double sd = us / sqrt(D[k]);
+        for (int j = 0; j < r; j++) U[(size_t)k * r + j] = (double)y[(size_t)k * st + 1 + j] * sd;
+    }
+}
+
+__global__ void reciprocal_kernel(const double *__restrict__ D, int P, double *d) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P; k += gridDim.x * blockDim.x) d[k] = 1.0 / D[k];
+}
+
 }  // namespace
 
 extern "C" {
+
+int qgtl_hip_ng_assemble(const double *Q, const int *slot, int P, double lambda, const double *eg, double *A,
+                         double *D, double *g) {
+    const size_t total = (size_t)P * P;
+    unsigned grid = (unsigned)((total + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(ng_assemble_kernel, dim3(grid), dim3(256), 0, qgtl::stream(), Q, slot, P, lambda, eg, A, D, g);
+    QGTL_LAUNCH_CHECK("ng_assemble");
+    return 0;
+}
+
+int qgtl_hip_pc_features(const double *theta, const double *D, double field, int P, float *x) {
+    hipLaunchKernelGGL(pc_features_kernel, dim3((unsigned)((P + 256) / 256)), dim3(256), 0, qgtl::stream(), theta, D,
+                       field, P, x);
+    QGTL_LAUNCH_CHECK("pc_features");
+    return 0;
+}
+
+int qgtl_hip_pc_transform(const float *y, const double *D, int P, int r, double *d, double *U) {
+    hipLaunchKernelGGL(pc_transform_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, qgtl::stream(), y, D, P,
+                       r, d, U);
+    QGTL_LAUNCH_CHECK("pc_transform");
+    return 0;
+}
+
+int qgtl_hip_reciprocal(const double *D, int P, double *d) {
+    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, qgtl::stream(), D, P, d);
+    QGTL_LAUNCH_CHECK("reciprocal");
+    return 0;
+}
 
 int qgtl_hip_pinv_apply_c128(const void *V, const double *ev, const void *g, void *x, int P, double cutoff) {
     if (P > 1024) return qgtl::set_error_msg(-118, "pinv: P > 1024");

@@ -227,6 +227,19 @@ static void plan_free_device(struct qgtl_plan *p)
     qgtl_hip_free(p->eg_partials);
     qgtl_hip_free(p->eg_sums);
     qgtl_hip_free(p->eg_out);
+    qgtl_hip_free(p->ng_slot);
+    qgtl_hip_free(p->ng_A);
+    qgtl_hip_free(p->ng_D);
+    qgtl_hip_free(p->ng_g);
+    qgtl_hip_free(p->ng_x);
+    qgtl_hip_free(p->ng_d);
+    qgtl_hip_free(p->ng_U);
+    qgtl_hip_free(p->ng_theta);
+    qgtl_hip_free(p->ng_res);
+    qgtl_hip_free(p->ng_it);
+    p->ng_slot = p->ng_it = NULL;
+    p->ng_A = p->ng_D = p->ng_g = p->ng_x = p->ng_d = p->ng_U = p->ng_theta = p->ng_res = NULL;
+    p->ng_rank = 0;
     p->terms_dev = p->hpsi = NULL;
     p->eg_partials = p->eg_sums = p->eg_out = NULL;
     p->psi_final = NULL;

@@ -71,7 +71,7 @@ int qgtl_plan_set_hamiltonian(qgtl_plan_t *p, int nterms, const qgtl_pauli_term_
     return 0;
 }
 
-int qgtl_plan_energy_gradient(qgtl_plan_t *p, double *energy, double *grad)
+int qgtl_energy_gradient_device(qgtl_plan_t *p)
 {
     if (!p || p->mode != QGTL_MODE_MIXED) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "needs a mixed-mode plan");
     if (!p->psi_final) return qgtl_set_error(QGT_ERROR_INVALID_STATE, "execute the plan first");
@@ -93,6 +93,14 @@ int qgtl_plan_energy_gradient(qgtl_plan_t *p, double *energy, double *grad)
     TRYH(qgtl_hip_dot_states_c64(p->psi_final, N, 1, p->hpsi, p->n, p->eg_partials + P * tiles * 4, &t2));
     TRYH(qgtl_hip_reduce_partials_f64(p->eg_partials + P * tiles * 4, 1, t2, p->eg_sums + 4 * P));
     TRYH(qgtl_hip_energy_gradient_finalize(p->eg_sums, p->eg_sums + 4 * P, (int)P, p->eg_out));
+    return 0;
+}
+
+int qgtl_plan_energy_gradient(qgtl_plan_t *p, double *energy, double *grad)
+{
+    int rc0 = qgtl_energy_gradient_device(p);
+    if (rc0) return rc0;
+    const size_t P = (size_t)p->P;
     double *host = (double *)malloc((P + 1) * sizeof(double));
     if (!host) return QGT_ERROR_NO_MEMORY;
     int rc = qgtl_hip_d2h(host, p->eg_out, (P + 1) * sizeof(double));

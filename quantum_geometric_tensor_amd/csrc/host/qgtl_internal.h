@@ -91,6 +91,11 @@ struct qgtl_plan {
     double *eg_partials;
     double *eg_sums;                     /* (P + 1) x 4 */
     double *eg_out;                      /* 1 + P */
+    /* device-resident natural-gradient step (qgtl_ngvqe.c) */
+    int *ng_slot;                        /* parameter -> D slot */
+    double *ng_A, *ng_D, *ng_g, *ng_x, *ng_d, *ng_U, *ng_theta, *ng_res;
+    int *ng_it;
+    int ng_rank;
 };
 
 /* plan construction (qgtl_plan.c) */
@@ -109,4 +114,10 @@ int qgtl_init_if_needed(void);
 
 /* learned preconditioner (qgtl_mlp.c) */
 void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank);
+/* device input row of a batch of one, and the forward pass leaving y on the device */
+int qgtl_mlp_device_input(qgtl_mlp_t *m, float **x_dev);
+int qgtl_mlp_forward_device(qgtl_mlp_t *m, const float **y_dev);
+
+/* Hamiltonian (qgtl_hamiltonian.c): E and grad into p->eg_out on the device, no sync */
+int qgtl_energy_gradient_device(qgtl_plan_t *p);
 #endif

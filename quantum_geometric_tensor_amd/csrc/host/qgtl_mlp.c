@@ -281,6 +281,22 @@ int qgtl_mlp_train_step(qgtl_mlp_t *m, int B, const int *idx, float lr, double *
     return 0;
 }
 
+int qgtl_mlp_device_input(qgtl_mlp_t *m, float **x_dev)
+{
+    int rc = ensure_batch(m, 1);
+    if (rc) return rc;
+    *x_dev = m->x;
+    return 0;
+}
+
+int qgtl_mlp_forward_device(qgtl_mlp_t *m, const float **y_dev)
+{
+    int rc = forward_dev(m, 1);
+    if (rc) return rc;
+    *y_dev = m->pre[m->nh];
+    return 0;
+}
+
 int qgtl_mlp_forward(qgtl_mlp_t *m, int B, const float *X, float *Y)
 {
     if (!m || B < 1 || !X || !Y) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "bad forward arguments");

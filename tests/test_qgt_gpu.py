@@ -145,3 +145,25 @@ def test_lds_pass_kernel_on_register_plans(n):
     got = b.execute()[0][0]
     b.close()
     assert Q.rel_err(got, O.qgt(n, O.hea_ops(n, L, th))) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("method", [Q.SOLVE_CHOLESKY, Q.SOLVE_JACOBI_PCG, Q.SOLVE_CG])
+def test_ng_step_device_resident(method):
+    """One natural-gradient step on the device equals theta - eta (Re Q + lambda I)^-1 grad E
+    assembled on the host from the same plan (QGT, energy and gradient on the GPU), and E
+    matches the oracle's TFIM energy."""
+    n, L, J, h, lam, eta = 6, 3, 1.0, 0.9, 1e-3, 0.05
+    th, ops, oops = hea(n, L, seed=321)
+    p = Q.Plan(n, ops, Q.MODE_MIXED)
+    p.set_hamiltonian(Q.tfim_terms(n, J, h))
+    Qm, _ = p.execute()
+    e_ref, g = p.energy_gradient()
+    A = Qm.real + lam * np.eye(len(th))
+    expect = th - eta * np.linalg.solve(A, g)
+    theta = th.copy()
+    info = Q.ng_step(p, theta, lam, eta, method, 1e-11, 20000, None, h)
+    p.close()
+    assert abs(info["energy"] - O.tfim_gradient(n, oops, J, h)[0]) <= 1e-6
+    tol = 1e-9 if method == Q.SOLVE_CHOLESKY else 1e-6   # iterative: residual 1e-11 times the condition number
+    assert np.max(np.abs(theta - expect)) <= tol * np.max(np.abs(expect - th))
+    assert info["solve_status"] == 0
