@@ -327,6 +327,72 @@ __global__ __launch_bounds__(kT) void cholesky_f64_kernel(double *Lall, const do
 
 // ------------------------------------------------------------------- PCG
 // A symmetric (row-major, so A[j][i] = A[i][j] gives coalesced column reads).
+
+// PCG A-column cache: each thread keeps kPcgRegs of its matrix-vector columns in registers
+// and the next kPcgLds in LDS; the rest (P above about 170) come from global memory.
+constexpr int kPcgRegs = 18;
+constexpr int kPcgLds = 14;
+// U (P x r) is copied to LDS when P * r fits.
+constexpr int kPcgUCache = 2048;
+
+// One lane of a double moved by a DPP pattern (two 32-bit DPP moves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wave, identical in every lane and independent of timing:
+// pairs, quads (quad_perm), half rows and rows of 16 (row_half_mirror, row_mirror) by DPP,
+// each step adding two equal-shaped partial sums so every lane of a row holds the same bits;
+// then the four row sums (lanes 0, 16, 32, 48) in the order (r0 + r1) + (r2 + r3).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += dpp_f64<0xB1>(v);                            // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);                            // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);                           // row_half_mirror
+    v += dpp_f64<0x140>(v);                           // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// W block sums with one barrier: wave sums into wp[wave][c], barrier, every thread adds the
+// 16 wave sums in wave order.  wp must not be rewritten before the next barrier, so callers
+// alternate between two buffers.
+template <int W>
+__device__ __forceinline__ void block_sums(double (&v)[W], double (*wp)[8]) {
+#pragma unroll
+    for (int c = 0; c < W; c++) v[c] = wave_sum_f64(v[c]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < W; c++) wp[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+    double t[W];
+#pragma unroll
+    for (int c = 0; c < W; c++) t[c] = 0.0;
+    // wide sums read the wave partials a row at a time (all 16 rows at once would spill)
+#pragma unroll(W > 2 ? 1 : kT / 64)
+    for (int w = 0; w < kT / 64; w++)
+#pragma unroll
+        for (int c = 0; c < W; c++) t[c] += wp[w][c];
+#pragma unroll
+    for (int c = 0; c < W; c++) v[c] = t[c];
+}
+
+// PCG with M^-1 = diag(d) + U U^T (d = NULL: identity diagonal; U = NULL: no low-rank part),
+// one workgroup per system, three barriers per iteration (four with a low-rank part).  Matrix-vector product: thread t
+// takes row t % P and the column segment t / P (kT / P segments), its A entries cached in
+// registers and LDS at the start; the segments are added in a fixed order, and p^T A p comes
+// from the same partial sums (sum over threads of p_row * partial) with no extra pass.  One
+// more reduction gives |r|^2, r^T D r and w = U^T r together, so z = D r + U w and
+// r^T z = r^T D r + |w|^2.
 __global__ __launch_bounds__(kT) void pcg_f64_kernel(const double *Aall, const double *ball, double *xall, int P,
                                                      double lambda, const double *dall, const double *Uall, int r,
                                                      double tol, int maxit, int *iters, double *relres) {
@@ -335,54 +401,106 @@ __global__ __launch_bounds__(kT) void pcg_f64_kernel(const double *Aall, const d
     double *x = xall + (size_t)blockIdx.x * P;
     const double *d = dall ? dall + (size_t)blockIdx.x * P : nullptr;
     const double *U = Uall ? Uall + (size_t)blockIdx.x * P * r : nullptr;
-    __shared__ double red[kT];
-    __shared__ double sp[1024], sw[64];
+    __shared__ double sp[1024], seg_sum[kT], wbuf[2][kT / 64][8], tot[72], sd[1024];
+    __shared__ double sa[kPcgLds * kT], su[kPcgUCache];
     const int i = threadIdx.x;
     const bool own = i < P;
+    const int rk = (U && r > 0) ? r : 0;
+    const int seg = kT / P;
+    const int row = i % P, sg = i / P;
+    const int seg_len = (P + seg - 1) / seg;
+    const int j0 = min(P, sg * seg_len);
+    const int len = sg < seg ? min(P, j0 + seg_len) - j0 : 0;
+    double areg[kPcgRegs];
+#pragma unroll
+    for (int k = 0; k < kPcgRegs; k++) areg[k] = (k < len) ? A[(size_t)(j0 + k) * P + row] : 0.0;
+    for (int k = kPcgRegs; k < min(len, kPcgRegs + kPcgLds); k++)
+        sa[(k - kPcgRegs) * kT + i] = A[(size_t)(j0 + k) * P + row];
+    const bool u_lds = rk > 0 && P * rk <= kPcgUCache;
+    if (u_lds)
+        for (int e = i; e < P * rk; e += kT) su[(e % rk) * P + e / rk] = U[e];
+    const double di = own ? (d ? d[i] : 1.0) : 0.0;
+    if (own) sd[i] = di;
+    __syncthreads();
+    auto uval = [&](int c) -> double { return u_lds ? su[c * P + i] : U[(size_t)i * rk + c]; };
+    int ph = 0;
     double xi = 0.0, ri = own ? b[i] : 0.0;
-    const double bn = sqrt(block_sum(ri * ri, red));
+    // Reduction of r: returns |r|^2, sets z = D r + U (U^T r) and rz = r^T z.
+    auto reduce_r = [&](double &z, double &rz) -> double {
+        double rr;
+        z = di * ri;
+        if (rk == 0) {
+            double v[2] = {ri * ri, di * ri * ri};
+            block_sums<2>(v, wbuf[ph]);
+            ph ^= 1;
+            rr = v[0];
+            rz = v[1];
+        } else {
+            // the 2 + rk dot products r.r, r.(D r), (U^T r)_c share the waves: wave c % 16 takes
+            // value c as a lane-strided sum over r in LDS and a wave sum.  sp is free here: the
+            // matrix-vector reads of p ended at the previous barrier.
+            const int lane = i & 63, wv = i >> 6;
+            if (own) sp[i] = ri;
+            __syncthreads();
+            for (int c = wv; c < 2 + rk; c += kT / 64) {
+                double acc = 0.0;
+                for (int k = lane; k < P; k += 64) {
+                    const double rv = sp[k];
+                    const double f = (c == 0) ? rv
+                                   : (c == 1) ? sd[k] * rv
+                                   : (u_lds ? su[(c - 2) * P + k] : U[(size_t)k * rk + c - 2]);
+                    acc += f * rv;
+                }
+                acc = wave_sum_f64(acc);
+                if (lane == 0) tot[c] = acc;
+            }
+            __syncthreads();
+            rr = tot[0];
+            rz = tot[1];
+            for (int c = 0; c < rk; c++) {
+                const double wc = tot[2 + c];
+                if (own) z += uval(c) * wc;
+                rz += wc * wc;
+            }
+        }
+        if (!own) z = 0.0;
+        return rr;
+    };
+    double zi, rz;
+    const double bn = sqrt(reduce_r(zi, rz));
     if (bn == 0.0) {
         if (own) x[i] = 0.0;
         if (i == 0) { iters[blockIdx.x] = 0; relres[blockIdx.x] = 0.0; }
         return;
     }
-    auto precond = [&](double rv) -> double {
-        // z = diag(d) r + U (U^T r)
-        double z = own ? (d ? d[i] * rv : rv) : 0.0;
-        if (U && r > 0) {
-            for (int c = 0; c < r; c++) {
-                const double part = own ? U[(size_t)i * r + c] * rv : 0.0;
-                const double wc = block_sum(part, red);
-                if (i == 0) sw[c] = wc;
-            }
-            __syncthreads();
-            if (own)
-                for (int c = 0; c < r; c++) z += U[(size_t)i * r + c] * sw[c];
-            __syncthreads();
-        }
-        return z;
-    };
-    double zi = precond(ri);
     double pi = zi;
-    double rz = block_sum(ri * zi, red);
     int it = 0;
     double rel = 1.0;
     for (; it < maxit; it++) {
-        sp[i] = pi;
+        if (own) sp[i] = pi;
         __syncthreads();
+        double partial = 0.0;
+#pragma unroll
+        for (int k = 0; k < kPcgRegs; k++)
+            if (k < len) partial += areg[k] * sp[j0 + k];
+        for (int k = kPcgRegs; k < min(len, kPcgRegs + kPcgLds); k++) partial += sa[(k - kPcgRegs) * kT + i] * sp[j0 + k];
+        for (int k = kPcgRegs + kPcgLds; k < len; k++) partial += A[(size_t)(j0 + k) * P + row] * sp[j0 + k];
+        seg_sum[i] = partial;
+        // p^T (A + lambda I) p = sum over threads of p_row * partial, plus lambda |p|^2
+        double v[1] = {(len > 0 ? partial * sp[row] : 0.0) + (own ? lambda * pi * pi : 0.0)};
+        block_sums<1>(v, wbuf[ph]);                   // its barrier also publishes seg_sum
+        ph ^= 1;
         double api = 0.0;
         if (own) {
-            for (int j = 0; j < P; j++) api += A[(size_t)j * P + i] * sp[j];
+            for (int q = 0; q < seg; q++) api += seg_sum[q * P + i];
             api += lambda * pi;
         }
-        const double pap = block_sum(pi * api, red);
-        const double alpha = rz / pap;
+        const double alpha = rz / v[0];
         xi += alpha * pi;
         ri -= alpha * api;
-        rel = sqrt(block_sum(ri * ri, red)) / bn;
+        double rz_new;
+        rel = sqrt(reduce_r(zi, rz_new)) / bn;
         if (rel <= tol) { it++; break; }
-        zi = precond(ri);
-        const double rz_new = block_sum(ri * zi, red);
         const double beta = rz_new / rz;
         rz = rz_new;
         pi = zi + beta * pi;

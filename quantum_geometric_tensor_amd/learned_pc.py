@@ -238,8 +238,36 @@ def run(n=16, L=6, samples=20000, traj_fraction=0.5, lam=1e-3, eta=0.1, traj_ste
     return out
 
 
+def pcg_iteration_cost(sizes=(64, 192, 240, 512), rank=8, log=print):
+    """Device time per PCG iteration of the library's FP64 solver (one system, one workgroup):
+    the difference between 550 and 50 iterations (tolerance 1e-300, never met) over 500, best
+    of three, for Jacobi-PCG and diagonal + rank-`rank` PCG on an SPD matrix with spectrum
+    1e-7 .. 1."""
+    out = {}
+    for P in sizes:
+        rng = np.random.default_rng(P)
+        Qm, _ = np.linalg.qr(rng.normal(size=(P, P)))
+        A = (Qm * np.logspace(-7, 0, P)) @ Qm.T
+        A = 0.5 * (A + A.T)
+        b = rng.normal(size=P)
+        d = 1.0 / np.diag(A)
+        U = Qm[:, :rank] * 0.01
+        row = {}
+        for name, method, dd, UU in (("jacobi_pcg", SOLVE_JACOBI_PCG, None, None), ("lowrank_pcg", SOLVE_PCG, d, U)):
+            t = {}
+            for maxit in (50, 550):
+                t[maxit] = min(solve(A, b, 0.0, method, tol=1e-300, maxit=maxit, d=dd, U=UU, check=False)[1]["device_ms"]
+                               for _ in range(3))
+            row[name] = (t[550] - t[50]) / 500 * 1e3
+        out[P] = row
+        log(f"P={P}: Jacobi-PCG {row['jacobi_pcg']:.2f} us/iteration, diagonal + rank-{rank} "
+            f"{row['lowrank_pcg']:.2f} us/iteration")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--pcg-timing", action="store_true", help="only time PCG iterations and exit")
     ap.add_argument("--out", default=None)
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--n", type=int, default=16)
@@ -249,6 +277,9 @@ def main():
     ap.add_argument("--traj-steps", type=int, default=250)
     ap.add_argument("--lr", type=float, default=1e-3)
     args = ap.parse_args()
+    if args.pcg_timing:
+        pcg_iteration_cost(rank=args.rank, log=lambda s: print(s, flush=True))
+        return
     res = run(n=args.n, L=args.layers, samples=args.samples, epochs=args.epochs, rank=args.rank,
               traj_steps=args.traj_steps, lr=args.lr, log=lambda s: print(s, flush=True))
     text = json.dumps(res, indent=1)

@@ -92,6 +92,32 @@ def test_cholesky_and_cg_family():
     assert e.value.code == -55
 
 
+@pytest.mark.parametrize("P,rank", [(7, 0), (64, 8), (160, 8), (192, 8), (240, 16), (500, 4), (1024, 8)])
+def test_pcg_sizes_and_cache_tiers(P, rank):
+    """PCG at sizes where the kernel's A columns sit in registers only (P=7, 64), spill into its
+    LDS cache (160) and into global reads (192, 240, 500, 1024), with U in LDS (P*rank <= 2048)
+    or read from global memory (240 x 16, 1024 x 8).  Same iteration count as the oracle's PCG
+    (within 5%), 1e-8 relative residual, and the true residual checked on the host."""
+    rng = np.random.default_rng(P + rank)
+    Qm, _ = np.linalg.qr(rng.normal(size=(P, P)))
+    ev = np.logspace(-2, 1, P)
+    A = (Qm * ev) @ Qm.T
+    A = 0.5 * (A + A.T)
+    b = rng.normal(size=P)
+    d = 1.0 / np.diag(A)
+    U = None
+    if rank:
+        U = Qm[:, :rank] * np.sqrt(1.0 / ev[:rank] - d.mean())[None, :]
+    for method, dd, UU in [(Q.SOLVE_JACOBI_PCG, None, None), (Q.SOLVE_PCG, d, U)]:
+        if method == Q.SOLVE_PCG and U is None:
+            continue
+        x, info = Q.solve(A, b, 0.0, method, tol=1e-8, maxit=20000, d=dd, U=UU)
+        assert info["rel_residual"] <= 1e-8
+        assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 2e-8
+        _, ito, _ = O.pcg(A, b, 1e-8, 20000, d=d if dd is None else dd, U=UU)
+        assert abs(info["iterations"] - ito) <= max(3, ito // 20), (P, rank, method, info["iterations"], ito)
+
+
 def test_hermitian_eig_matches_oracle():
     n, L = 6, 3
     Qo = O.qgt(n, O.hea_ops(n, L, O.uniform_angles(4, 2 * n * L)))
