@@ -102,20 +102,46 @@ __device__ __forceinline__ void tile_coords(int t, int nt, int &ti, int &tj) {
     tj = i + rem;
 }
 
-// Load 64 states x kKS amplitudes of D starting at state s0, amplitude k0,
-// into LDS rows (2s, 2s+1) = (Re, Im) as S bf16 pieces (S = 2 or 3 arrays).
-template <int S, bool F16>
-__device__ __forceinline__ void stage_rows(const float *__restrict__ D, int P, size_t N, int s0, size_t k0,
-                                           uint32_t *const *rows) {
+constexpr int kStageLoads = 64 * (kKS / 2) / kThreads;   // float4 loads per thread for 64 states
+
+// Load 64 states x kKS amplitudes of D starting at state s0, amplitude k0 into
+// registers.  Every load is issued unconditionally (rows past P and amplitudes
+// past N read a clamped, valid address) and bit i of the returned mask says
+// whether load i is real; the zeroing happens at the split, so the loads of a
+// stage are all in flight together instead of each waiting in its own branch.
+// Load i of a thread reads state s0 + 8 i + t / 32, amplitudes k0 + 2 (t % 32)
+// + {0, 1}.
+__device__ __forceinline__ uint32_t load_rows(const float *__restrict__ D, int P, size_t N, int s0, size_t k0,
+                                              float4 (&v)[kStageLoads]) {
     constexpr int kPairs = kKS / 2;                 // amplitude pairs per row
-#pragma unroll 4
-    for (int i = 0; i < 64 * kPairs / kThreads; i++) {
+    constexpr int kRowsPerLoad = kThreads / kPairs;   // 8 states per load instruction
+    const int sl = threadIdx.x / kPairs, pair = threadIdx.x % kPairs;
+    const bool k_ok = k0 + 2 * pair < N;
+    const size_t kk = k_ok ? k0 + 2 * pair : k0;
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < kStageLoads; i++) {
+        const int s = s0 + i * kRowsPerLoad + sl;
+        const bool ok = s < P && k_ok;
+        mask |= (ok ? 1u : 0u) << i;
+        const int sc = s < P ? s : P - 1;
+        v[i] = *reinterpret_cast<const float4 *>(D + 2 * ((size_t)sc * N + kk));
+    }
+    return mask;
+}
+
+// Split the loaded values into LDS rows (2s, 2s+1) = (Re, Im) as S 16-bit
+// pieces (S = 2 or 3 arrays); loads outside the matrix count as zeros.
+template <int S, bool F16>
+__device__ __forceinline__ void store_rows(const float4 (&vin)[kStageLoads], uint32_t mask, uint32_t *const *rows) {
+    constexpr int kPairs = kKS / 2;
+#pragma unroll
+    for (int i = 0; i < kStageLoads; i++) {
         const int L = i * kThreads + threadIdx.x;
         const int sl = L / kPairs, pair = L % kPairs;
-        const int s = s0 + sl;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (s < P && k0 + 2 * pair < N) v = *reinterpret_cast<const float4 *>(D + 2 * ((size_t)s * N + k0 + 2 * pair));
         const int wr = (2 * sl) * (kRow / 2) + pair, wi = (2 * sl + 1) * (kRow / 2) + pair;
+        const bool ok = (mask >> i) & 1u;
+        const float4 v = ok ? vin[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (S == 2) {
             uint32_t hr, lr, hm, lm;
             if (F16) {
@@ -144,7 +170,9 @@ struct GramLds {
 };
 
 // Unit `bid` of one Gram: chunk c's tiles are the units with bid % 8 == c % 8.
-template <int TERMS>
+// ONE_ROUND: a stage issues the A and B loads together (one memory latency per
+// stage); otherwise A, then B (two latencies, 32 fewer registers).
+template <int TERMS, bool ONE_ROUND>
 __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, size_t N, int nt, int nchunks,
                                           size_t chunk_len, float *__restrict__ partials, int bid,
                                           GramLds<TERMS> &sm) {
@@ -185,8 +213,24 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
     for (size_t kb = kbeg; kb < kend; kb += kInner) {
         const size_t ke = (kb + kInner < kend) ? kb + kInner : kend;
         for (size_t k0 = kb; k0 < ke; k0 += kKS) {
-            stage_rows<S, F16>(D, P, N, ti * (kTile / 2), k0, arows);
-            if (!diag) stage_rows<S, F16>(D, P, N, tj * (kTile / 2), k0, brows);
+            if (ONE_ROUND) {
+                float4 va[kStageLoads], vb[kStageLoads];
+                const uint32_t ma = load_rows(D, P, N, ti * (kTile / 2), k0, va);
+                const uint32_t mb = diag ? 0u : load_rows(D, P, N, tj * (kTile / 2), k0, vb);
+                store_rows<S, F16>(va, ma, arows);
+                if (!diag) store_rows<S, F16>(vb, mb, brows);
+            } else {
+                {
+                    float4 va[kStageLoads];
+                    const uint32_t ma = load_rows(D, P, N, ti * (kTile / 2), k0, va);
+                    store_rows<S, F16>(va, ma, arows);
+                }
+                if (!diag) {
+                    float4 vb[kStageLoads];
+                    const uint32_t mb = load_rows(D, P, N, tj * (kTile / 2), k0, vb);
+                    store_rows<S, F16>(vb, mb, brows);
+                }
+            }
             __syncthreads();
 #pragma unroll 1
             for (int kk = 0; kk < kKS / 16; kk++) {
@@ -249,7 +293,7 @@ __global__ __launch_bounds__(kThreads, (TERMS == 6) ? 1 : 2) void gram_split_ker
                                                               int nt, int nchunks, size_t chunk_len,
                                                               float *__restrict__ partials) {
     __shared__ GramLds<TERMS> sm;
-    gram_unit<TERMS>(D, P, N, nt, nchunks, chunk_len, partials, blockIdx.x, sm);
+    gram_unit<TERMS, TERMS != 6>(D, P, N, nt, nchunks, chunk_len, partials, blockIdx.x, sm);
 }
 
 // Grouped: grid and every unit_begin are multiples of 8, so u % 8 == local % 8
@@ -261,8 +305,10 @@ __global__ __launch_bounds__(kThreads, (TERMS == 6) ? 1 : 2) void gram_grouped_k
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const qgtl_gram_item_t it = items[qgtl::find_item(items, nitems, u)];
         __syncthreads();
-        gram_unit<TERMS>(reinterpret_cast<const float *>(it.D), it.P, (size_t)it.N, it.nt, it.nchunks,
-                         (size_t)it.chunk_len, reinterpret_cast<float *>(it.partials), (int)(u - it.unit_begin), sm);
+        gram_unit<TERMS, false>(qgtl::global_ptr<const float>(qgtl::uniform(it.D)), qgtl::uniform(it.P),
+                         (size_t)qgtl::uniform(it.N), qgtl::uniform(it.nt), qgtl::uniform(it.nchunks),
+                         (size_t)qgtl::uniform(it.chunk_len), qgtl::global_ptr<float>(qgtl::uniform(it.partials)),
+                         (int)(u - qgtl::uniform(it.unit_begin)), sm);
     }
 }
 

@@ -23,6 +23,23 @@ __device__ __forceinline__ int find_item(const Item *__restrict__ items, int nit
     return lo;
 }
 
+// A value every lane of the wave holds (e.g. loaded from a grouped launch's item
+// table) moved into scalar registers, so the code using it stays scalar.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ int32_t uniform(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform(uint64_t v) {
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+
+// A device address held as an integer (grouped launch item tables) as a pointer
+// the compiler knows to be global memory, so accesses through it become
+// global_load/store (scalar base + 32-bit offsets) rather than flat ones.
+template <class T>
+__device__ __forceinline__ T *global_ptr(uint64_t a) {
+    typedef __attribute__((address_space(1))) T gT;
+    return (T *)(gT *)a;
+}
+
 }  // namespace qgtl
 
 // Map a HIP status to the library's error codes and remember the message.
