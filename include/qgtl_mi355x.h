@@ -237,6 +237,27 @@ int qgtl_pc_features(int P, const double *theta, const double *D, double field, 
 int qgtl_mlp_preconditioner(qgtl_mlp_t *m, const float *x, const double *D, double *d, double *U);
 void qgtl_mlp_destroy(qgtl_mlp_t *m);
 
+/* ---- files: computed QGTs, training sets, network checkpoints ----
+ * QGTL declares geometric_tensor_save / geometric_tensor_load
+ * (include/quantum_geometric/core/quantum_geometric_tensor.h:144-147) without
+ * an implementation or a format; these are this library's formats for what the
+ * QGT path produces.  One container for all three: magic, version, sizes and an
+ * FNV-1a 64 checksum over header and payload, all checked on load (a mismatch
+ * is QGT_ERROR_VALIDATION_FAILED, a different P/S/F QGT_ERROR_DIMENSION_MISMATCH,
+ * an unreadable file QGT_ERROR_IO_ERROR). */
+int qgtl_qgt_save(const char *path, int n, int P, const double *theta /* P or NULL */,
+                  const double *Q /* P x P complex128 */);
+int qgtl_qgt_file_info(const char *path, int *n, int *P, int *has_theta);
+int qgtl_qgt_load(const char *path, int P, double *theta /* P or NULL */, double *Q);
+int qgtl_dataset_save(const char *path, int S, int F, int P, const float *X /* S x F */,
+                      const float *As /* S x P x P */);
+int qgtl_dataset_file_info(const char *path, int *S, int *F, int *P);
+int qgtl_dataset_load(const char *path, int S, int F, int P, float *X, float *As);
+/* fp32 master weights [+ Adam moments and step]; load builds a network of the saved shape */
+int qgtl_mlp_save(const qgtl_mlp_t *m, const char *path, int with_optimizer);
+qgtl_mlp_t *qgtl_mlp_load(const char *path);
+int qgtl_mlp_get_shape(const qgtl_mlp_t *m, int *in_dim, int *hidden, int *hidden_layers, int *P, int *rank);
+
 /* ---- natural-gradient VQE step (QGTL: run_vqe / update_natural_gradient,
  * src/quantum_geometric/hybrid/quantum_classical_algorithms.c:854,
  * src/quantum_geometric/hybrid/classical_optimization_engine.c:390-416) ----

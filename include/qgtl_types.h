@@ -62,8 +62,11 @@ typedef enum {
     QGT_ERROR_ALLOCATION_FAILED = -2,
     QGT_ERROR_DIMENSION_MISMATCH = -3,
     QGT_ERROR_INVALID_STATE = -4,
+    QGT_ERROR_VALIDATION_FAILED = -5,
     QGT_ERROR_HARDWARE_FAILURE = -6,
     QGT_ERROR_NOT_IMPLEMENTED = -7,
+    QGT_ERROR_IO_ERROR = -16,
+    QGT_ERROR_IO = -16,
     QGT_ERROR_INCOMPATIBLE = -19,
     QGT_ERROR_INVALID_DIMENSION = -22,
     QGT_ERROR_NO_MEMORY = -23,

@@ -392,6 +392,10 @@ def _mlp_lib():
         L.qgtl_pc_features.argtypes = [ip, vp, vp, ctypes.c_double, vp]
         L.qgtl_mlp_preconditioner.argtypes = [vp, vp, vp, vp, vp]
         L.qgtl_mlp_destroy.argtypes = [vp]
+        L.qgtl_mlp_save.argtypes = [vp, ctypes.c_char_p, ip]
+        L.qgtl_mlp_load.argtypes = [ctypes.c_char_p]
+        L.qgtl_mlp_load.restype = vp
+        L.qgtl_mlp_get_shape.argtypes = [vp, vp, vp, vp, vp, vp]
         L._mlp_bound = True
     return L
 
@@ -408,15 +412,30 @@ def pc_features(theta, D, field):
 class MLP:
     """The learned preconditioner's network, trained and evaluated on the GPU."""
 
-    def __init__(self, in_dim, hidden, hidden_layers, P, rank, seed=0):
+    def __init__(self, in_dim, hidden, hidden_layers, P, rank, seed=0, _handle=None):
         L = _mlp_lib()
-        h = L.qgtl_mlp_create(in_dim, hidden, hidden_layers, P, rank, seed)
+        h = _handle if _handle else L.qgtl_mlp_create(in_dim, hidden, hidden_layers, P, rank, seed)
         if not h:
             raise QGTLError(-1, L.qgtl_last_error().decode(errors="replace"))
         self._h = h
         self.in_dim, self.hidden, self.hidden_layers, self.P, self.rank = in_dim, hidden, hidden_layers, P, rank
         self.out_dim = P * (1 + rank)
         self.num_params = L.qgtl_mlp_num_params(h)
+
+    def save(self, path, with_optimizer=True):
+        """Checkpoint: fp32 master weights, plus the Adam moments and step when with_optimizer."""
+        _check(_mlp_lib().qgtl_mlp_save(self._h, os.fsencode(path), int(bool(with_optimizer))))
+
+    @classmethod
+    def load(cls, path):
+        """A network of the saved shape with the saved weights (and optimizer state, if saved)."""
+        L = _mlp_lib()
+        h = L.qgtl_mlp_load(os.fsencode(path))
+        if not h:
+            raise QGTLError(-1, L.qgtl_last_error().decode(errors="replace"))
+        shape = [ctypes.c_int(0) for _ in range(5)]
+        L.qgtl_mlp_get_shape(h, *[ctypes.byref(v) for v in shape])
+        return cls(*[v.value for v in shape], _handle=h)
 
     def get_params(self):
         p = np.zeros(self.num_params, np.float32)
@@ -474,6 +493,59 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- files
+def _io_lib():
+    L = lib()
+    if not getattr(L, "_io_bound", False):
+        vp, ip, cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p
+        ipp = ctypes.POINTER(ctypes.c_int)
+        L.qgtl_qgt_save.argtypes = [cp, ip, ip, vp, vp]
+        L.qgtl_qgt_file_info.argtypes = [cp, ipp, ipp, ipp]
+        L.qgtl_qgt_load.argtypes = [cp, ip, vp, vp]
+        L.qgtl_dataset_save.argtypes = [cp, ip, ip, ip, vp, vp]
+        L.qgtl_dataset_file_info.argtypes = [cp, ipp, ipp, ipp]
+        L.qgtl_dataset_load.argtypes = [cp, ip, ip, ip, vp, vp]
+        L._io_bound = True
+    return L
+
+
+def qgt_save(path, n, Q, theta=None):
+    """Write a P x P complex QGT (and optionally its angles) in the library's QGT file format."""
+    Q = np.ascontiguousarray(Q, np.complex128)
+    th = None if theta is None else np.ascontiguousarray(theta, np.float64)
+    _check(_io_lib().qgtl_qgt_save(os.fsencode(path), int(n), Q.shape[0], None if th is None else _p(th), _p(Q)))
+
+
+def qgt_load(path):
+    """(n, Q, theta or None) from a QGT file; size, format and checksum are checked."""
+    L = _io_lib()
+    n, P, ht = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _check(L.qgtl_qgt_file_info(os.fsencode(path), ctypes.byref(n), ctypes.byref(P), ctypes.byref(ht)))
+    Q = np.zeros((P.value, P.value), np.complex128)
+    th = np.zeros(P.value) if ht.value else None
+    _check(L.qgtl_qgt_load(os.fsencode(path), P.value, None if th is None else _p(th), _p(Q)))
+    return n.value, Q, th
+
+
+def dataset_save(path, X, As):
+    """Training pairs of the learned preconditioner: features X (S x F) and scaled matrices As (S x P x P)."""
+    X = np.ascontiguousarray(X, np.float32)
+    As = np.ascontiguousarray(As, np.float32)
+    if X.ndim != 2 or As.ndim != 3 or As.shape[0] != X.shape[0] or As.shape[1] != As.shape[2]:
+        raise ValueError("X must be S x F and As S x P x P")
+    _check(_io_lib().qgtl_dataset_save(os.fsencode(path), X.shape[0], X.shape[1], As.shape[1], _p(X), _p(As)))
+
+
+def dataset_load(path):
+    L = _io_lib()
+    S, F, P = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _check(L.qgtl_dataset_file_info(os.fsencode(path), ctypes.byref(S), ctypes.byref(F), ctypes.byref(P)))
+    X = np.zeros((S.value, F.value), np.float32)
+    As = np.zeros((S.value, P.value, P.value), np.float32)
+    _check(L.qgtl_dataset_load(os.fsencode(path), S.value, F.value, P.value, _p(X), _p(As)))
+    return X, As
 
 
 # ---------------------------------------------------------------- natural-gradient VQE step

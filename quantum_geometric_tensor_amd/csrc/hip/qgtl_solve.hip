@@ -368,6 +368,7 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 // alternate between two buffers.
 template <int W>
 __device__ __forceinline__ void block_sums(double (&v)[W], double (*wp)[8]) {
+    static_assert(W >= 1 && W <= 2, "one or two sums per call");
 #pragma unroll
     for (int c = 0; c < W; c++) v[c] = wave_sum_f64(v[c]);
     if ((threadIdx.x & 63) == 0)
@@ -377,8 +378,7 @@ __device__ __forceinline__ void block_sums(double (&v)[W], double (*wp)[8]) {
     double t[W];
 #pragma unroll
     for (int c = 0; c < W; c++) t[c] = 0.0;
-    // wide sums read the wave partials a row at a time (all 16 rows at once would spill)
-#pragma unroll(W > 2 ? 1 : kT / 64)
+#pragma unroll
     for (int w = 0; w < kT / 64; w++)
 #pragma unroll
         for (int c = 0; c < W; c++) t[c] += wp[w][c];

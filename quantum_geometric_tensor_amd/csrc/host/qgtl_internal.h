@@ -12,6 +12,7 @@
 #define QGTL_INTERNAL_H
 
 #include <stdint.h>
+#include <stdio.h>
 #include <stddef.h>
 
 #include "qgtl_hip_abi.h"
@@ -114,6 +115,15 @@ int qgtl_init_if_needed(void);
 
 /* learned preconditioner (qgtl_mlp.c) */
 void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank);
+
+/* file container of qgtl_io.c (prefix, kind header, payload, FNV-1a 64 checksum) */
+uint64_t qgtl_fnv1a(uint64_t h, const void *data, size_t n);
+int qgtl_io_write(const char *path, const char magic[8], const void *hdr, uint32_t hdr_bytes, int count,
+                  const void *const *pieces, const size_t *sizes);
+int qgtl_io_open(const char *path, const char magic[8], void *hdr, uint32_t hdr_bytes, uint64_t *payload_bytes,
+                 uint64_t *checksum, FILE **out);
+int qgtl_io_read_payload(FILE *f, const char *path, const void *hdr, uint32_t hdr_bytes, uint64_t payload_bytes,
+                         uint64_t checksum, int count, void *const *pieces, const size_t *sizes);
 /* device input row of a batch of one, and the forward pass leaving y on the device */
 int qgtl_mlp_device_input(qgtl_mlp_t *m, float **x_dev);
 int qgtl_mlp_forward_device(qgtl_mlp_t *m, const float **y_dev);

@@ -20,6 +20,7 @@
  * (src/quantum_geometric/hybrid/classical_optimization_engine.c:543-610).
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -339,5 +340,92 @@ int qgtl_mlp_preconditioner(qgtl_mlp_t *m, const float *x, const double *D, doub
             for (int j = 0; j < m->r; j++) U[(size_t)k * m->r + j] = (double)y[(size_t)k * st + 1 + j] * us / sqrt(D[k]);
     }
     free(y);
+    return 0;
+}
+
+/* ---- checkpoints (container of qgtl_io.c, magic "QGTLMLP1") ----
+ * kind header: shape, Adam step, whether the Adam moments follow;
+ * payload: fp32 master weights [+ first and second moments]. */
+typedef struct {
+    int32_t F, H, nh, P, r, step, has_opt, reserved;
+} mlp_hdr_t;
+
+int qgtl_mlp_save(const qgtl_mlp_t *m, const char *path, int with_optimizer)
+{
+    if (!m || !path) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_mlp_save: null argument");
+    const size_t nb = m->nparams * sizeof(float);
+    const int k = with_optimizer ? 3 : 1;
+    float *h = (float *)malloc(nb * (size_t)k);
+    if (!h) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+    int rc = 0;
+    if (qgtl_hip_d2h(h, m->w, nb) || (k == 3 && (qgtl_hip_d2h(h + m->nparams, m->m, nb) ||
+                                                  qgtl_hip_d2h(h + 2 * m->nparams, m->v, nb))) ||
+        qgtl_hip_sync()) {
+        rc = qgtl_set_error(QGT_ERROR_GPU_TRANSFER_FAILED, "mlp checkpoint: %s", qgtl_hip_last_error());
+    } else {
+        const mlp_hdr_t hd = {m->F, m->H, m->nh, m->P, m->r, m->step, k == 3, 0};
+        const void *pieces[1] = {h};
+        const size_t sizes[1] = {nb * (size_t)k};
+        rc = qgtl_io_write(path, "QGTLMLP1", &hd, sizeof hd, 1, pieces, sizes);
+    }
+    free(h);
+    return rc;
+}
+
+qgtl_mlp_t *qgtl_mlp_load(const char *path)
+{
+    mlp_hdr_t hd;
+    uint64_t pb = 0, ck = 0;
+    FILE *f = NULL;
+    if (qgtl_io_open(path, "QGTLMLP1", &hd, sizeof hd, &pb, &ck, &f)) return NULL;
+    /* the payload size the header implies, before anything is allocated for it */
+    uint64_t np = 0;
+    const int ok_shape = hd.F >= 1 && hd.F <= (1 << 20) && hd.H >= 1 && hd.H <= 65536 && hd.nh >= 1 &&
+                         hd.nh < MLP_MAX_LAYERS && hd.P >= 1 && hd.P <= 1024 && hd.r >= 0 && hd.r <= 16;
+    if (ok_shape)
+        for (int i = 0; i <= hd.nh; i++) {
+            const uint64_t in = (i == 0) ? (uint64_t)hd.F : (uint64_t)hd.H;
+            const uint64_t out = (i == hd.nh) ? (uint64_t)hd.P * (1u + (uint64_t)hd.r) : (uint64_t)hd.H;
+            np += in * out + out;
+        }
+    if (!ok_shape || pb != np * sizeof(float) * (hd.has_opt ? 3u : 1u)) {
+        fclose(f);
+        qgtl_set_error(QGT_ERROR_VALIDATION_FAILED, "%s: inconsistent checkpoint header", path);
+        return NULL;
+    }
+    qgtl_mlp_t *m = qgtl_mlp_create(hd.F, hd.H, hd.nh, hd.P, hd.r, 0);
+    if (!m) { fclose(f); return NULL; }
+    const size_t nb = m->nparams * sizeof(float);
+    const int k = hd.has_opt ? 3 : 1;
+    float *h = (float *)malloc(nb * (size_t)k);
+    if (!h) {
+        fclose(f);
+        qgtl_mlp_destroy(m);
+        qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+        return NULL;
+    }
+    void *pieces[1] = {h};
+    const size_t sizes[1] = {nb * (size_t)k};
+    int rc = qgtl_io_read_payload(f, path, &hd, sizeof hd, pb, ck, 1, pieces, sizes);
+    if (!rc) rc = qgtl_mlp_set_params(m, h);       /* also refreshes the bf16 mirror, zeroes Adam */
+    if (!rc && k == 3) {
+        if (qgtl_hip_h2d(m->m, h + m->nparams, nb) || qgtl_hip_h2d(m->v, h + 2 * m->nparams, nb) || qgtl_hip_sync())
+            rc = qgtl_set_error(QGT_ERROR_GPU_TRANSFER_FAILED, "mlp checkpoint: %s", qgtl_hip_last_error());
+        else
+            m->step = hd.step;
+    }
+    free(h);
+    if (rc) { qgtl_mlp_destroy(m); return NULL; }
+    return m;
+}
+
+int qgtl_mlp_get_shape(const qgtl_mlp_t *m, int *in_dim, int *hidden, int *hidden_layers, int *P, int *rank)
+{
+    if (!m) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null mlp");
+    if (in_dim) *in_dim = m->F;
+    if (hidden) *hidden = m->H;
+    if (hidden_layers) *hidden_layers = m->nh;
+    if (P) *P = m->P;
+    if (rank) *rank = m->r;
     return 0;
 }

@@ -18,8 +18,8 @@ import time
 
 import numpy as np
 
-from . import (MLP, MODE_MIXED, SOLVE_CG, SOLVE_CHOLESKY, SOLVE_JACOBI_PCG, SOLVE_PCG, Batch, Plan, hea_ops,
-               ng_step, solve, tfim_terms)
+from . import (MLP, MODE_MIXED, SOLVE_CG, SOLVE_CHOLESKY, SOLVE_JACOBI_PCG, SOLVE_PCG, Batch, Plan, dataset_save,
+               hea_ops, ng_step, solve, tfim_terms)
 from .workloads import SplitMix64
 
 
@@ -146,7 +146,8 @@ def solve_stats(A, G, method, tol, d=None, U=None, maxit=20000):
 
 
 def run(n=16, L=6, samples=20000, traj_fraction=0.5, lam=1e-3, eta=0.1, traj_steps=250, rank=8, hidden=256,
-        hidden_layers=3, epochs=20, batch=256, lr=1e-3, tol=1e-8, seed=0, ng_runs=3, ng_steps=30, log=print):
+        hidden_layers=3, epochs=20, batch=256, lr=1e-3, tol=1e-8, seed=0, ng_runs=3, ng_steps=30, log=print,
+        save_prefix=None):
     g = SplitMix64(seed)
     P = 2 * n * L
     out = {"config": dict(n=n, L=L, P=P, samples=samples, traj_fraction=traj_fraction, lam=lam, eta=eta,
@@ -177,6 +178,11 @@ def run(n=16, L=6, samples=20000, traj_fraction=0.5, lam=1e-3, eta=0.1, traj_ste
     log(f"training: {len(train_idx)} pairs, {mlp.num_params} parameters")
     hist, train_s = train(mlp, X, As, train_idx, test_idx, epochs, batch, lr, g, log=log)
     out["train"] = {"history": hist, "seconds": train_s, "parameters": int(mlp.num_params)}
+    if save_prefix:
+        # the training set and the trained network in the library's file formats
+        dataset_save(save_prefix + ".set", X, As)
+        mlp.save(save_prefix + ".mlp", with_optimizer=True)
+        out["saved"] = [save_prefix + ".set", save_prefix + ".mlp"]
 
     # held-out solves: right-hand sides grad E (trajectory samples have it; uniform ones get it now)
     G = np.zeros((S, P))
@@ -276,12 +282,13 @@ def main():
     ap.add_argument("--rank", type=int, default=8)
     ap.add_argument("--traj-steps", type=int, default=250)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--save", default=None, help="path prefix: write <prefix>.set (training set) and <prefix>.mlp")
     args = ap.parse_args()
     if args.pcg_timing:
         pcg_iteration_cost(rank=args.rank, log=lambda s: print(s, flush=True))
         return
     res = run(n=args.n, L=args.layers, samples=args.samples, epochs=args.epochs, rank=args.rank,
-              traj_steps=args.traj_steps, lr=args.lr, log=lambda s: print(s, flush=True))
+              traj_steps=args.traj_steps, lr=args.lr, log=lambda s: print(s, flush=True), save_prefix=args.save)
     text = json.dumps(res, indent=1)
     if args.out:
         with open(args.out, "w") as f:

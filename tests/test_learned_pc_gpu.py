@@ -11,14 +11,22 @@ systems; at this size and seed it is 1.14x (47.0 vs 53.6 iterations).
 import numpy as np
 import pytest
 
+import quantum_geometric_tensor_amd as Q
 from quantum_geometric_tensor_amd import learned_pc
 
 pytestmark = pytest.mark.gpu
 
 
-def test_learned_preconditioner_small_end_to_end():
+def test_learned_preconditioner_small_end_to_end(tmp_path):
+    prefix = str(tmp_path / "pc")
     r = learned_pc.run(n=8, L=4, samples=1000, epochs=15, traj_steps=100, ng_runs=1, ng_steps=5,
-                       log=lambda s: print(s, flush=True))
+                       log=lambda s: print(s, flush=True), save_prefix=prefix)
+    # the saved training set and network reload (shapes of the n=8, L=4 workload: P = 64)
+    X, As = Q.dataset_load(prefix + ".set")
+    assert X.shape == (1000, 3 * 64 + 1) and As.shape == (1000, 64, 64)
+    m = Q.MLP.load(prefix + ".mlp")
+    assert (m.in_dim, m.P, m.rank) == (3 * 64 + 1, 64, 8)
+    m.close()
     hist = r["train"]["history"]
     assert hist[-1]["test_loss"] < hist[0]["test_loss"] - 1.0          # Kaporin loss fell on held-out data
     ho = r["held_out"]

@@ -118,3 +118,44 @@ def test_mlp_training_reduces_loss():
     print(f"Kaporin loss {first:.3f} -> {last:.3f}")
     assert last < first - 0.5
     m.close()
+
+
+def test_checkpoint_round_trip(tmp_path):
+    """A checkpoint restores the network: same weights and bit-identical forward output; with the
+    optimizer state, the next Adam steps of the reloaded network equal those of the original."""
+    m = Q.MLP(13, 64, 3, 8, 2, seed=21)
+    S = 32
+    rng = np.random.default_rng(9)
+    X = rng.normal(size=(S, m.in_dim)).astype(np.float32)
+    As = np.stack([spd_unit_diag(m.P, 700 + s) for s in range(S)]).astype(np.float32)
+    m.set_data(X, As)
+    for _ in range(5):
+        m.train_step(rng.choice(S, 8, replace=False).astype(np.int32), 2e-3)
+    f = tmp_path / "pc.mlp"
+    m.save(f, with_optimizer=True)
+    m2 = Q.MLP.load(f)
+    assert (m2.in_dim, m2.hidden, m2.hidden_layers, m2.P, m2.rank) == (13, 64, 3, 8, 2)
+    assert np.array_equal(m2.get_params(), m.get_params())
+    assert np.array_equal(m2.forward(X), m.forward(X))
+    m2.set_data(X, As)
+    batches = [rng.choice(S, 8, replace=False).astype(np.int32) for _ in range(3)]
+    for b in batches:
+        l1 = m.train_step(b, 2e-3)
+        l2 = m2.train_step(b, 2e-3)
+        assert l1 == l2
+    assert np.array_equal(m2.get_params(), m.get_params())
+    # weights only: the optimizer restarts
+    m.save(f, with_optimizer=False)
+    m3 = Q.MLP.load(f)
+    assert np.array_equal(m3.forward(X), m.forward(X))
+    # a training set and a GPU-computed QGT round-trip through their files too
+    Q.dataset_save(tmp_path / "t.set", X, As)
+    X4, As4 = Q.dataset_load(tmp_path / "t.set")
+    assert np.array_equal(X4, X) and np.array_equal(As4, As)
+    th = np.random.default_rng(5).uniform(0, 2 * np.pi, 2 * 6 * 2)
+    Qg = Q.qgt(6, Q.hea_ops(6, 2, th))
+    Q.qgt_save(tmp_path / "g.qgt", 6, Qg, th)
+    n, Qr, thr = Q.qgt_load(tmp_path / "g.qgt")
+    assert n == 6 and np.array_equal(Qr, Qg) and np.array_equal(thr, th)
+    for x in (m, m2, m3):
+        x.close()
