@@ -200,11 +200,21 @@ int qgtl_hip_event_destroy(qgtl_hip_event_t ev);
 
 /* ---- complex64 sweeps (mixed mode) ---- */
 int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0> */
-/* nstates states at src + s*stride -> dst + s*stride (elements of complex64) */
+/* Split-plane form of a final derivative state (the Gram's operand form): the
+ * 8 N bytes of state s hold, for each block of 64 amplitudes (512 bytes, the
+ * footprint of the same block in complex64), four fp16 rows of 64 values: Re hi,
+ * Re lo, Im hi, Im lo, where for each component x: hi = 2^15 x rounded to 11
+ * significant bits (nearest, ties away) and lo = fp16(2^15 x - hi).  Amplitude
+ * y, row r: fp16 index 256 (y / 64) + 64 r + y % 64. */
+/* nstates states at src + s*stride -> dst + s*stride (elements of complex64);
+ * split_out = 1: dst receives the split-plane form (register-phase passes only,
+ * see qgtl_hip_pass_split_ok) */
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
-                      int nstates, int tile_variant);
+                      int nstates, int tile_variant, int split_out);
+int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc);
+/* split_out = 1: the derivative states are written in split-plane form */
 int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
-                        double *partials, int *tiles_out);
+                        double *partials, int *tiles_out, int split_out);
 /* out[k*4 + c] = sum over t of partials[(k*tiles + t)*4 + c], fixed order, FP64 */
 int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, double *out);
 int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t);
@@ -214,9 +224,10 @@ int qgtl_hip_fill_f32(void *dst, size_t count, float value);
 int qgtl_hip_copy_f32x4(void *dst, const void *src, size_t count16);    /* count16 units of 16 bytes */
 /* out = H psi for a Pauli sum (terms in device memory) */
 int qgtl_hip_pauli_apply_c64(const void *psi, void *out, int n, int nterms, const void *terms_dev);
-/* per (state, tile) FP64 sums: <D_k|v>, |D_k|^2, |v|^2 (same layout as inject partials) */
+/* per (state, tile) FP64 sums: <D_k|v>, |D_k|^2, |v|^2 (same layout as inject partials);
+ * planes = 1: D holds split-plane states, read as (hi + lo) 2^-15 */
 int qgtl_hip_dot_states_c64(const void *D, size_t stride, int K, const void *v, int n, double *partials,
-                            int *tiles_out);
+                            int *tiles_out, int planes);
 /* out[0] = energy, out[1 + k] = gradient (from reduced dot sums) */
 int qgtl_hip_energy_gradient_finalize(const double *dk_h, const double *psi_h, int P, double *out);
 
@@ -228,7 +239,8 @@ int qgtl_hip_gram_tiles(int P);                                          /* uppe
 #define QGTL_GRAM_KSTEP 64   /* amplitudes per Gram LDS stage; chunk lengths are multiples of it */
 #define QGTL_GRAM_F16X3 33
 #define QGTL_GRAM_F16X4 34
-int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms);
+/* presplit = 1: D holds split-plane states (QGTL_GRAM_F16X3 / X4 only) */
+int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms, int presplit);
 int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C);   /* C: R x R tiles, FP64 */
 /* Q = normalised G - a a^H from C and injection sums (P x 4 doubles). */
 int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q);

@@ -58,6 +58,8 @@ typedef struct {
     int64_t kernel_launches;        /* all launches of the call */
     int32_t P;
     int32_t n;
+    int32_t presplit;               /* 1: final derivative states were written as the Gram's fp16 pieces */
+    int32_t reserved;
 } qgtl_stats_t;
 
 /* Tunables chosen by the autotuner (or defaults when NULL / zero fields). */
@@ -68,7 +70,10 @@ typedef struct {
                                  three-piece bf16 split, 33 / 34 = two fp16 pieces of 2^15 x with 3 / 4
                                  products; 0 = default (33) */
     int32_t sweep_form;       /* 0 = rotation form (real rotations + phase tables), 1 = general complex matrices */
-    int32_t reserved[4];
+    int32_t no_presplit;      /* 0: where the plan allows it, the final pass of every derivative state (and the
+                                 last block's injection) writes the Gram's fp16 operand pieces directly, so the
+                                 Gram does no conversion; 1: states stay complex64 and the Gram splits them */
+    int32_t reserved[3];
 } qgtl_tune_t;
 
 int qgtl_init(int device);

@@ -31,6 +31,26 @@ __device__ __forceinline__ uint64_t uniform(uint64_t v) {
     return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
 }
 
+// Two fp16 pieces of sa = 2^15 x and sb: hi = the value rounded to 11
+// significant bits (nearest, ties away from zero: add half an ulp to the bit
+// pattern, mask the 13 low mantissa bits), exact in fp16 for |value| >= 2^-14;
+// lo = fp16(value - hi), the difference being exact in fp32.  So
+// |value - hi - lo| <= 2^-11 |value - hi| <= 2^-22 |value|, the bound of two
+// round-to-nearest fp16 conversions, at about 4 VALU operations per value (each
+// pair's two conversions are one v_cvt_pk_f16_f32).  Returns the packed pairs
+// (sa in the low half).
+typedef float qgtl_f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 qgtl_f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_f16_pair(float sa, float sb, uint32_t &hi, uint32_t &lo) {
+    const float ra = __uint_as_float((__float_as_uint(sa) + 0x1000u) & 0xffffe000u);
+    const float rb = __uint_as_float((__float_as_uint(sb) + 0x1000u) & 0xffffe000u);
+    const qgtl_f32x2 h32 = {ra, rb}, l32 = {sa - ra, sb - rb};
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(h32, qgtl_f16x2));
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(l32, qgtl_f16x2));
+}
+
+constexpr float kSplitScale = 32768.0f;          // 2^15: |x| <= 1 keeps 2^15 x below the fp16 maximum
+
 // A device address held as an integer (grouped launch item tables) as a pointer
 // the compiler knows to be global memory, so accesses through it become
 // global_load/store (scalar base + 32-bit offsets) rather than flat ones.

@@ -375,6 +375,21 @@ __device__ __forceinline__ void apply_phase_mat(float2 (&v)[kRegs], const qgtl_p
     }
 }
 
+// Amplitudes g (even) and g + 1 of a state in split-plane form: per block of 64
+// amplitudes four fp16 rows (Re hi, Re lo, Im hi, Im lo), here as uint32 pairs
+// (128 per block, 32 per row), the same pieces the Gram would cut from the
+// complex64 values.
+__device__ __forceinline__ void write_split_pair(uint32_t *planes, size_t g, float2 a, float2 b) {
+    uint32_t rh, rl, ih, il;
+    qgtl::split_f16_pair(a.x * qgtl::kSplitScale, b.x * qgtl::kSplitScale, rh, rl);
+    qgtl::split_f16_pair(a.y * qgtl::kSplitScale, b.y * qgtl::kSplitScale, ih, il);
+    uint32_t *blk = planes + (g >> 6) * 128 + ((g & 63) >> 1);
+    blk[0] = rh;
+    blk[32] = rl;
+    blk[64] = ih;
+    blk[96] = il;
+}
+
 template <int R>
 struct RegLoop {
     __device__ __forceinline__ static void store_lds(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs]) {
@@ -423,6 +438,22 @@ struct RegLoop {
         const size_t g = ((size_t)yhi << kRT) | y;
         *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
     }
+    // the same two stores writing the split-plane form
+    __device__ __forceinline__ static void store_global_split(uint32_t *planes, uint32_t base,
+                                                              const uint32_t *hi_off, int nlow, uint32_t lowmask,
+                                                              const RegLayout &L, const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global_split(planes, base, hi_off, nlow, lowmask, L, v);
+        const uint32_t j = reg_index<R - 2>(L);
+        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
+        write_split_pair(planes, g, v[R - 2], v[R - 1]);
+    }
+    __device__ __forceinline__ static void store_global_ring_split(uint32_t *planes, uint32_t pre, uint32_t ones,
+                                                                   const RegLayout &L, const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global_ring_split(planes, pre, ones, L, v);
+        const uint32_t y = reg_index<R - 2>(L);
+        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
+        write_split_pair(planes, ((size_t)yhi << kRT) | y, v[R - 2], v[R - 1]);
+    }
 };
 
 template <>
@@ -437,10 +468,16 @@ struct RegLoop<0> {
                                                         const RegLayout &, const float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global_ring(float2 *, uint32_t, uint32_t, const RegLayout &,
                                                              const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global_split(uint32_t *, uint32_t, const uint32_t *, int, uint32_t,
+                                                              const RegLayout &, const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global_ring_split(uint32_t *, uint32_t, uint32_t, const RegLayout &,
+                                                                   const float2 (&)[kRegs]) {}
 };
 
 constexpr int kRegTileElems = 1 << kRT;
 
+// SPLIT: dst receives the split-plane form of the result (a final pass).
+template <bool SPLIT = false>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
                                                float2 *post_tab, float2 *pre_tab) {
@@ -501,25 +538,32 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
         RegLoop<kRegs>::store_lds_ring(tile, Lc, v, par_r);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
-        RegLoop<kRegs>::store_global_ring(dst, pre, ones, L0, v);
+        if constexpr (SPLIT)
+            RegLoop<kRegs>::store_global_ring_split(reinterpret_cast<uint32_t *>(dst), pre, ones, L0, v);
+        else
+            RegLoop<kRegs>::store_global_ring(dst, pre, ones, L0, v);
     } else {
         if (d.nphase > 1) {
             RegLoop<kRegs>::store_lds(tile, Lc, v);
             __syncthreads();
             RegLoop<kRegs>::load_lds(tile, L0, v);
         }
-        RegLoop<kRegs>::store_global(dst, base, hi_off, nlow, lowmask, L0, v);
+        if constexpr (SPLIT)
+            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), base, hi_off, nlow, lowmask, L0, v);
+        else
+            RegLoop<kRegs>::store_global(dst, base, hi_off, nlow, lowmask, L0, v);
     }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
                                                                  const float2 *__restrict__ src_all,
                                                                  float2 *dst_all, size_t stride) {
     __shared__ float2 tile[kRegTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
-    pass_regs_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
-                   tile, hi_off, tabs, tabs + kPhaseTab);
+    pass_regs_body<SPLIT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
+                          blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -533,7 +577,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Tile `tile` of derivative state k; its four sums go to out[0..3].
+// Tile `tile` of derivative state k; its four sums go to out[0..3].  SPLIT:
+// dst receives the split-plane form (four fp16 planes of N values).
+template <bool SPLIT = false>
 __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const float2 *__restrict__ psi,
                                             float2 *dst, int k, uint32_t tile, double *out,
                                             double (*red)[4]) {
@@ -550,7 +596,17 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
             const bool s = (__popc(y & w) & 1) != 0;
             const float2 mss = s ? a11 : a00, mso = s ? a10 : a01;
             const float2 o = cmadd(cmadd(make_float2(0.f, 0.f), mss, a), mso, b);
-            dst[y] = o;
+            if constexpr (SPLIT) {
+                uint32_t h, l;                        // (Re, Im) pieces of this amplitude
+                qgtl::split_f16_pair(o.x * qgtl::kSplitScale, o.y * qgtl::kSplitScale, h, l);
+                uint16_t *pl = reinterpret_cast<uint16_t *>(dst) + (size_t)(y >> 6) * 256 + (y & 63);
+                pl[0] = (uint16_t)h;
+                pl[64] = (uint16_t)l;
+                pl[128] = (uint16_t)(h >> 16);
+                pl[192] = (uint16_t)(l >> 16);
+            } else {
+                dst[y] = o;
+            }
             const double dr = o.x, di = o.y, pr = a.x, pi = a.y;
             sr += dr * pr + di * pi;
             si += dr * pi - di * pr;
@@ -571,14 +627,15 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
     }
 }
 
+template <bool SPLIT>
 __global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_desc_t d,
                                                               const float2 *__restrict__ psi,
                                                               float2 *dst_all, size_t stride,
                                                               double *partials) {
     __shared__ double red[kThreads / 64][4];
     const int k = blockIdx.y;
-    inject_body(d, psi, dst_all + (size_t)k * stride, k, blockIdx.x,
-                partials + ((size_t)k * gridDim.x + blockIdx.x) * 4, red);
+    inject_body<SPLIT>(d, psi, dst_all + (size_t)k * stride, k, blockIdx.x,
+                       partials + ((size_t)k * gridDim.x + blockIdx.x) * 4, red);
 }
 
 // out[c] = sum over t of pk[t * 4 + c] (fixed tree order, FP64).
@@ -640,6 +697,8 @@ __global__ __launch_bounds__(kThreads) void pauli_apply_c64_kernel(const float2 
 }
 
 // FP64 sums per (state k, tile): <D_k|v> (re, im), |D_k|^2, |v|^2.
+// PLANES: D holds split-plane states, read back as (hi + lo) 2^-15 (exact in fp32).
+template <bool PLANES>
 __global__ __launch_bounds__(kThreads) void dot_states_c64_kernel(const float2 *__restrict__ D, size_t stride,
                                                                   const float2 *__restrict__ v, uint32_t N,
                                                                   double *partials) {
@@ -650,7 +709,15 @@ __global__ __launch_bounds__(kThreads) void dot_states_c64_kernel(const float2 *
     for (int i = 0; i < kInjectPerThread; i++) {
         const uint32_t y = blockIdx.x * kInjectTile + i * kThreads + threadIdx.x;
         if (y < N) {
-            const float2 a = d[y], b = v[y];
+            float2 a;
+            if constexpr (PLANES) {
+                const _Float16 *pl = reinterpret_cast<const _Float16 *>(d) + (size_t)(y >> 6) * 256 + (y & 63);
+                const float inv = 1.0f / qgtl::kSplitScale;
+                a = make_float2(((float)pl[0] + (float)pl[64]) * inv, ((float)pl[128] + (float)pl[192]) * inv);
+            } else {
+                a = d[y];
+            }
+            const float2 b = v[y];
             const double ar = a.x, aim = a.y, br = b.x, bi = b.y;
             sr += ar * br + aim * bi;
             si += ar * bi - aim * br;
@@ -863,9 +930,15 @@ int qgtl_hip_fill_zero_state_c64(void *psi, int n) {
     return 0;
 }
 
+static bool regs_pass(const qgtl_pass_desc_t *desc, int tile_variant) {
+    return desc->m == kRT && desc->nphase >= 1 && desc->nphase <= QGTL_MAX_PHASES && desc->store != QGTL_STORE_PERM &&
+           tile_variant != 1;
+}
+
+int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc) { return regs_pass(desc, 0) ? 1 : 0; }
+
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
-                      int nstates, int tile_variant) {
-    (void)tile_variant;
+                      int nstates, int tile_variant, int split_out) {
     if (desc->m < 1 || desc->m > QGTL_TILE_BITS_MAX || desc->nlow < 1 || desc->nlow > desc->m ||
         desc->m - desc->nlow > kHiTableBits || desc->ngates > desc->m || desc->nperm > QGTL_PERM_MAX ||
         desc->m + desc->nrest != desc->n || nstates < 1)
@@ -877,10 +950,14 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
     if ((desc->store == QGTL_STORE_RING || desc->store == QGTL_STORE_PERM) && src == dst && desc->store == QGTL_STORE_RING)
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: ring store cannot run in place");
     const dim3 grid(1u << desc->nrest, (unsigned)nstates);
-    const bool regs = desc->m == kRT && desc->nphase >= 1 && desc->nphase <= QGTL_MAX_PHASES &&
-                      desc->store != QGTL_STORE_PERM && tile_variant != 1;
-    if (regs)
-        hipLaunchKernelGGL(pass_c64_regs_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+    const bool regs = regs_pass(desc, tile_variant);
+    if (split_out && (!regs || src == dst || stride < ((size_t)1 << desc->n)))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: split-plane output needs a register pass, not in place");
+    if (regs && split_out)
+        hipLaunchKernelGGL(pass_c64_regs_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)src, (float2 *)dst, stride);
+    else if (regs)
+        hipLaunchKernelGGL(pass_c64_regs_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride);
     else
         hipLaunchKernelGGL(pass_c64_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
@@ -890,14 +967,19 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
 }
 
 int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
-                        double *partials, int *tiles_out) {
+                        double *partials, int *tiles_out, int split_out) {
     if (desc->K < 1 || desc->K > QGTL_INJECT_MAX || desc->n < 1 || desc->n > QGTL_MAX_QUBITS)
         return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: bad descriptor");
     const uint32_t N = 1u << desc->n;
+    if (split_out && stride < N) return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: states overlap");
     const unsigned tiles = (N + kInjectTile - 1) / kInjectTile;
     *tiles_out = (int)tiles;
-    hipLaunchKernelGGL(inject_c64_kernel, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0, qgtl::stream(),
-                       *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
+    if (split_out)
+        hipLaunchKernelGGL(inject_c64_kernel<true>, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0,
+                           qgtl::stream(), *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
+    else
+        hipLaunchKernelGGL(inject_c64_kernel<false>, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0,
+                           qgtl::stream(), *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
     QGTL_LAUNCH_CHECK("inject_c64");
     return 0;
 }
@@ -936,12 +1018,16 @@ int qgtl_hip_pauli_apply_c64(const void *psi, void *out, int n, int nterms, cons
 }
 
 int qgtl_hip_dot_states_c64(const void *D, size_t stride, int K, const void *v, int n, double *partials,
-                            int *tiles_out) {
+                            int *tiles_out, int planes) {
     const uint32_t N = 1u << n;
     const unsigned tiles = (N + kInjectTile - 1) / kInjectTile;
     *tiles_out = (int)tiles;
-    hipLaunchKernelGGL(dot_states_c64_kernel, dim3(tiles, (unsigned)K), dim3(kThreads), 0, qgtl::stream(),
-                       (const float2 *)D, stride, (const float2 *)v, N, partials);
+    if (planes)
+        hipLaunchKernelGGL(dot_states_c64_kernel<true>, dim3(tiles, (unsigned)K), dim3(kThreads), 0, qgtl::stream(),
+                           (const float2 *)D, stride, (const float2 *)v, N, partials);
+    else
+        hipLaunchKernelGGL(dot_states_c64_kernel<false>, dim3(tiles, (unsigned)K), dim3(kThreads), 0,
+                           qgtl::stream(), (const float2 *)D, stride, (const float2 *)v, N, partials);
     QGTL_LAUNCH_CHECK("dot_states_c64");
     return 0;
 }

@@ -390,19 +390,22 @@ static int alloc_plain(struct qgtl_plan *p)
 }
 
 /* Apply block b to nstates complex64 states at *cur (stride N); a pass that
- * flips writes to *alt and swaps the two. */
+ * flips writes to *alt and swaps the two.  split_final: the block's last pass
+ * writes the split-plane form, always to *alt (it cannot run in place). */
 static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
-                       qgtl_stats_t *st)
+                       qgtl_stats_t *st, int split_final)
 {
     for (int i = 0; i < b->npasses; i++) {
-        void *dst = b->pass_flips[i] ? *alt : *cur;
-        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0));
+        const int split = split_final && i == b->npasses - 1;
+        const int flip = b->pass_flips[i] || split;
+        void *dst = flip ? *alt : *cur;
+        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0, split));
         if (st) {
             st->sweep_launches++;
             st->sweep_state_passes += nstates;
             st->sweep_bytes += (double)nstates * (double)p->N * 16.0;
         }
-        if (b->pass_flips[i]) { void *t = *cur; *cur = *alt; *alt = t; }
+        if (flip) { void *t = *cur; *cur = *alt; *alt = t; }
     }
     if (b->mstage_kernels)
         for (int i = 0; i < b->nperm; i++) {
@@ -419,15 +422,29 @@ static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, v
     return 0;
 }
 
-static int block_flips(const qgtl_block_t *b)
+/* flips of block b for a derivative-state chunk (the forced one of a split final pass included) */
+static int block_flips(const struct qgtl_plan *p, int bi)
 {
+    const qgtl_block_t *b = &p->blocks[bi];
     int f = 0;
-    for (int i = 0; i < b->npasses; i++) f += b->pass_flips[i];
+    for (int i = 0; i < b->npasses; i++)
+        f += b->pass_flips[i] || (p->presplit && bi == p->nblocks - 1 && i == b->npasses - 1);
     return f;
 }
 
-/* Inject parameters slots [s0, s0+cnt) (all from block b) into dst. */
-static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st)
+/* Split-plane final states: depth-first plans whose last block ends with a
+ * register-phase pass and no separate M-stage kernels, fp16 Gram pieces. */
+static int presplit_ok(const struct qgtl_plan *p, int depth_first, int terms)
+{
+    if (p->tune.no_presplit || !depth_first || p->P == 0 || p->nblocks < 1 || p->N % 64 != 0) return 0;
+    if (terms != QGTL_GRAM_F16X3 && terms != QGTL_GRAM_F16X4) return 0;
+    const qgtl_block_t *b = &p->blocks[p->nblocks - 1];
+    if (b->npasses < 1 || (b->mstage_kernels && b->nperm > 0)) return 0;
+    return qgtl_hip_pass_split_ok(&b->pass[b->npasses - 1]);
+}
+
+/* Inject parameters slots [s0, s0+cnt) (all from block b) into dst (split-plane form when split). */
+static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st, int split)
 {
     qgtl_inject_desc_t d;
     memset(&d, 0, sizeof d);
@@ -443,7 +460,7 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
         }
     }
     int tiles = 0;
-    LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles));
+    LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles, split));
     LAUNCH(K_INJECT, qgtl_hip_reduce_partials_f64(p->inj_partials, cnt, tiles, p->inj_sums + 4 * (size_t)s0));
     return 0;
 }
@@ -455,27 +472,29 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     void *cur = p->psi[0], *alt = p->psi[1];
     LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));
     const int depth_first = any_flip(p);
+    p->presplit = presplit_ok(p, depth_first, gram_terms(p));
     int slot = 0;
     for (int bi = 0; bi < p->nblocks; bi++) {
         const qgtl_block_t *b = &p->blocks[bi];
         if (!depth_first && slot > 0) {
             void *dc = p->D, *da = NULL;
-            rc = apply_block(p, b, &dc, &da, slot, st);
+            rc = apply_block(p, b, &dc, &da, slot, st, 0);
             if (rc) return rc;
         }
-        rc = apply_block(p, b, &cur, &alt, 1, st);
+        rc = apply_block(p, b, &cur, &alt, 1, st, 0);
         if (rc) return rc;
         const int K = b->param_end - b->param_begin;
         if (K == 0) continue;
         if (!depth_first) {
             for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                 const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
-                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st);
+                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st, 0);
                 if (rc) return rc;
             }
         } else {
             int flips_after = 0;
-            for (int b2 = bi + 1; b2 < p->nblocks; b2++) flips_after += block_flips(&p->blocks[b2]);
+            for (int b2 = bi + 1; b2 < p->nblocks; b2++) flips_after += block_flips(p, b2);
+            const int last = bi == p->nblocks - 1;
             /* balanced chunks of at most tmp_states states */
             const int nch = (K + (int)p->tmp_states - 1) / (int)p->tmp_states;
             const int C = (K + nch - 1) / nch;
@@ -484,10 +503,11 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
                 void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
                 void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
                 void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
-                rc = inject(p, slot + j0, cnt, cur, c_cur, st);
+                rc = inject(p, slot + j0, cnt, cur, c_cur, st, p->presplit && last);
                 if (rc) return rc;
                 for (int b2 = bi + 1; b2 < p->nblocks; b2++) {
-                    rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st);
+                    rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st,
+                                     p->presplit && b2 == p->nblocks - 1);
                     if (rc) return rc;
                 }
                 if (c_cur != dslot) return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: chunk ended off its slot");
@@ -497,7 +517,8 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     }
     p->psi_final = cur;
     if (p->P == 0) return 0;
-    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p)));
+    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p),
+                                             p->presplit));
     LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, p->P, p->gram_chunks, p->C));
     LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
     if (st) {
@@ -588,6 +609,7 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
     prof_reset();
     rc = (p->mode == QGTL_MODE_MIXED) ? run_mixed(p, st) : run_plain(p, st);
     if (rc) { qgtl_hip_sync(); prof_reset(); return rc; }
+    st->presplit = p->mode == QGTL_MODE_MIXED ? p->presplit : 0;
     const size_t P = (size_t)p->P;
     double *Qslot = NULL;
     if (Q && P > 0) {
@@ -692,7 +714,7 @@ int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *ps
         if (rc) { qgtl_hip_free(out); qgtl_plan_destroy(p); return hip_fail(rc, "statevector buffer"); }
         void *cur = p->psi[0], *alt = p->psi[1];
         rc = qgtl_hip_fill_zero_state_c64(cur, n);
-        for (int b = 0; !rc && b < p->nblocks; b++) rc = apply_block(p, &p->blocks[b], &cur, &alt, 1, st);
+        for (int b = 0; !rc && b < p->nblocks; b++) rc = apply_block(p, &p->blocks[b], &cur, &alt, 1, st, 0);
         if (!rc) rc = qgtl_hip_c64_to_c128(cur, out, N);
     } else {
         rc = qgtl_hip_fill_zero_state_c128(out, n);

@@ -85,6 +85,7 @@ struct qgtl_plan {
     size_t plain_partials_len;
     uint64_t structure_hash;
     void *psi_final;                     /* primal after the last execute (mixed) */
+    int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
     /* Hamiltonian for energy / gradient */
     int nterms;
     void *terms_dev;

@@ -135,6 +135,37 @@ def test_gate_mix_both_sweep_forms(n, form):
     assert Q.rel_err(got, ref) <= MIXED_TOL
 
 
+@pytest.mark.parametrize("n,tail", [(14, False), (15, True), (16, False)])
+def test_split_plane_final_states(n, tail):
+    """Depth-first plans write their final derivative states as the Gram's fp16 pieces (the
+    last pass and the last block's injection): Q equals the plan that keeps complex64 states
+    and splits in the Gram bit for bit, matches the oracle, and the TFIM energy and gradient
+    read back from the pieces match the oracle to 1e-6.  tail: the circuit ends with a
+    rotation layer without a CNOT ring, so the final pass would run in place and is made to
+    write out of place."""
+    gates = _mixed_circuit(n, 2, 900 + n)
+    if tail:
+        k = sum(1 for g in gates if g[3] >= 0)
+        rng = np.random.default_rng(n)
+        gates += [("RY", q, -1, k + q, float(rng.uniform(0, 2 * np.pi))) for q in range(n)]
+    ops, oops = Q.make_ops(gates), O.make_ops(gates)
+    J, h = 1.0, 0.7
+    out = {}
+    for pre in (True, False):
+        p = Q.Plan(n, ops, Q.MODE_MIXED, presplit=pre)
+        p.set_hamiltonian(Q.tfim_terms(n, J, h))
+        q, st = p.execute()
+        assert st["presplit"] == int(pre)
+        out[pre] = (q, *p.energy_gradient())
+        p.close()
+    assert np.array_equal(out[True][0], out[False][0])
+    assert Q.rel_err(out[True][0], O.qgt(n, oops)) <= MIXED_TOL
+    e_ref, g_ref = O.tfim_gradient(n, oops, J, h)
+    for pre in (True, False):
+        assert abs(out[pre][1] - e_ref) <= 1e-6 * max(1.0, abs(e_ref))
+        assert np.max(np.abs(out[pre][2] - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
+
+
 @pytest.mark.parametrize("n", [14, 16])
 def test_lds_pass_kernel_on_register_plans(n):
     """Batched sweeps with the LDS gate-group kernel forced for full 13-bit tiles
