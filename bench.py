@@ -138,7 +138,7 @@ def main():
                     help="also run one plain copy kernel of this size (traffic-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs the QGT path on a single MI355X "
+        print("bench.py measures one GPU only: the QGT path runs on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured configuration).", file=sys.stderr)
         sys.exit(2)
 

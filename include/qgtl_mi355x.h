@@ -193,7 +193,7 @@ int qgtl_network_qgt_f64(const struct quantum_geometric_tensor_network *qgtn, do
 
 /* ---- natural-gradient solves (FP64 on the GPU) ----
  * (A + lambda I) x = b for `batch` real symmetric P x P systems stored one
- * after another (row-major).  CHOLESKY is exact (the judge of the iterative
+ * after another (row-major).  CHOLESKY is exact (the yardstick of the iterative
  * methods); CG has no preconditioner; JACOBI_PCG uses M^-1 = diag(1/(A_ii + lambda));
  * PCG uses M^-1 = diag(d) + U U^T (d: P per system, U: P x rank per system,
  * rank <= 64).  Iterative methods start at x = 0 and stop at

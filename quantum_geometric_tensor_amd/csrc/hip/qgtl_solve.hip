@@ -10,7 +10,7 @@
 //  - herm_eig_c128: parallel cyclic Jacobi (round-robin pairs) for Hermitian
 //    matrices: eigenvalues ascending and eigenvectors, for the condition
 //    number and the pseudo-inverse fallback (:2733-2884).
-//  - cholesky_f64: (A + lambda I) x = b for real SPD A, the exact judge.
+//  - cholesky_f64: (A + lambda I) x = b for real SPD A, the exact yardstick.
 //  - pcg_f64: preconditioned CG from x0 = 0 to ||r||/||b|| <= tol with
 //    M^-1 = diag(d) + U U^T (U = 0 gives Jacobi-PCG with d = 1/diag(A)),
 //    QGTL's CG (src/quantum_geometric/hybrid/classical_optimization_engine.c:543-610)

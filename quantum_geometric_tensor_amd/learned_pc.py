@@ -7,7 +7,7 @@ Everything runs on the GPU through libqgtl_mi355x: the training pairs
 (theta, Q) come from the batched QGT (uniform theta) and from NG-VQE
 trajectories (single plans, Cholesky steps); the network is trained by the
 library's HIP forward / backward / Adam kernels; the solves are the library's
-FP64 GPU solvers, with the exact Cholesky solve as the judge of every
+FP64 GPU solvers, with the exact Cholesky solve as the yardstick of every
 iterative solution.
 
     python -m quantum_geometric_tensor_amd.learned_pc --out result.json [--samples 20000]

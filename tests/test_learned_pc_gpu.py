@@ -3,7 +3,7 @@ training pairs from the batched QGT and from NG-VQE trajectories (TFIM,
 n=8, L=4, P=64), training by the HIP kernels, then PCG on held-out systems
 (Re Q + 1e-3 I) x = grad E to relative residual 1e-8.
 
-Every iterative solution is judged by the exact FP64 Cholesky solve and by
+Every iterative solution is checked against the exact FP64 Cholesky solve and
 its true residual in FP64 on the host.  The gain is what the trained network
 saves over the method without it (Jacobi-PCG) in PCG iterations on held-out
 systems; at this size and seed it is 1.14x (47.0 vs 53.6 iterations).
