@@ -488,7 +488,7 @@ int qgtl_hip_gram_tiles(int P) {
 int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms, int presplit) {
     if (P < 1 || nchunks < 1 || nchunks % kXcd != 0 || (N & 1))
         return qgtl::set_error_msg(-118, "qgtl_hip_gram_bf16x3: nchunks must be a multiple of 8 and N even");
-    if (presplit && (terms != QGTL_GRAM_F16X3 && terms != QGTL_GRAM_F16X4 || N % kKS != 0))
+    if (presplit && ((terms != QGTL_GRAM_F16X3 && terms != QGTL_GRAM_F16X4) || N % kKS != 0))
         return qgtl::set_error_msg(-118, "qgtl_hip_gram_bf16x3: split planes need an fp16 variant and N % 64 == 0");
     const int nt = (2 * P + kTile - 1) / kTile;
     const int T = nt * (nt + 1) / 2;

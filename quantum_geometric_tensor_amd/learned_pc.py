@@ -244,6 +244,37 @@ def run(n=16, L=6, samples=20000, traj_fraction=0.5, lam=1e-3, eta=0.1, traj_ste
     return out
 
 
+def ideal_lowrank_bound(n=16, L=6, samples=400, lam=1e-3, eta=0.1, traj_steps=250, ranks=(8, 16, 32), tol=1e-8,
+                        seed=7, log=print):
+    """What the preconditioner family can give at best: PCG iterations with M^-1 = D^-1 + U U^T where
+    U holds the EXACT eigenvectors of the r smallest eigenvalues of the Jacobi-scaled matrix
+    D^-1/2 A D^-1/2 (deflated to 1), against Jacobi-PCG, on the learned workload's systems (half
+    uniform theta, half NG trajectories, right-hand side grad E).  A network can only approximate
+    these U, so the iteration ratios bound what any diag + rank-r prediction can achieve."""
+    g = SplitMix64(seed)
+    P = 2 * n * L
+    th_u, h_u, A_u = uniform_samples(samples // 2, n, L, g, lam, log=log)
+    th_t, h_t, A_t, G_t, _ = trajectory_samples(samples - samples // 2, n, L, g, lam, eta, traj_steps, log=log)
+    A = np.concatenate([A_u, A_t])
+    G = np.concatenate([gradients_for(th_u, h_u, n, L), G_t])
+    D = np.einsum("sii->si", A)
+    s = 1.0 / np.sqrt(D)
+    As = A * s[:, :, None] * s[:, None, :]
+    w, V = np.linalg.eigh(As)
+    _, it_j, st_j, _ = solve_stats(A, G, SOLVE_JACOBI_PCG, tol)
+    out = {"systems": int(len(A)), "P": P, "jacobi_iterations_mean": float(it_j.mean()),
+           "kappa_scaled_median": float(np.median(w[:, -1] / w[:, 0])), "ideal": {}}
+    for r in ranks:
+        scale = np.sqrt(np.clip(1.0 / w[:, :r] - 1.0, 0.0, None))
+        U = s[:, :, None] * V[:, :, :r] * scale[:, None, :]
+        _, it, st, _ = solve_stats(A, G, SOLVE_PCG, tol, 1.0 / D, U)
+        out["ideal"][r] = {"iterations_mean": float(it.mean()), "converged": int((st == 0).sum()),
+                           "iteration_ratio_jacobi_over_ideal": float(it_j.mean() / it.mean())}
+        log(f"rank {r}: ideal deflation {it.mean():.1f} iterations vs Jacobi {it_j.mean():.1f} "
+            f"({it_j.mean() / it.mean():.2f}x), converged {(st == 0).sum()}/{len(st)}")
+    return out
+
+
 def pcg_iteration_cost(sizes=(64, 192, 240, 512), rank=8, log=print):
     """Device time per PCG iteration of the library's FP64 solver (one system, one workgroup):
     the difference between 550 and 50 iterations (tolerance 1e-300, never met) over 500, best
@@ -274,6 +305,8 @@ def pcg_iteration_cost(sizes=(64, 192, 240, 512), rank=8, log=print):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--pcg-timing", action="store_true", help="only time PCG iterations and exit")
+    ap.add_argument("--ideal-bound", action="store_true",
+                    help="only measure PCG with exact smallest-eigenvector deflation (the family's best case) and exit")
     ap.add_argument("--out", default=None)
     ap.add_argument("--samples", type=int, default=20000)
     ap.add_argument("--n", type=int, default=16)
@@ -286,6 +319,11 @@ def main():
     args = ap.parse_args()
     if args.pcg_timing:
         pcg_iteration_cost(rank=args.rank, log=lambda s: print(s, flush=True))
+        return
+    if args.ideal_bound:
+        res = ideal_lowrank_bound(n=args.n, L=args.layers, traj_steps=args.traj_steps,
+                                  log=lambda s: print(s, flush=True))
+        print(json.dumps(res, indent=1), flush=True)
         return
     res = run(n=args.n, L=args.layers, samples=args.samples, epochs=args.epochs, rank=args.rank,
               traj_steps=args.traj_steps, lr=args.lr, log=lambda s: print(s, flush=True), save_prefix=args.save)
