@@ -208,9 +208,14 @@ int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0
  * y, row r: fp16 index 256 (y / 64) + 64 r + y % 64. */
 /* nstates states at src + s*stride -> dst + s*stride (elements of complex64);
  * split_out = 1: dst receives the split-plane form (register-phase passes only,
- * see qgtl_hip_pass_split_ok) */
+ * see qgtl_hip_pass_split_ok); tabs: NULL, or the pass's rotation-form phase
+ * tables in device memory (QGTL_PHASE_TAB post entries, then as many pre
+ * entries, complex64; entry i < 128: product over the gates on tile bits 0..6
+ * of their phase at bit pattern i; entry 128 + i: the same for tile bits 7..12),
+ * which the workgroups then copy instead of building */
+#define QGTL_PHASE_TAB 192
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
-                      int nstates, int tile_variant, int split_out);
+                      int nstates, int tile_variant, int split_out, const void *tabs);
 int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc);
 /* split_out = 1: the derivative states are written in split-plane form */
 int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,

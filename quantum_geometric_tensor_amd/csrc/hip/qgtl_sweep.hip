@@ -61,7 +61,7 @@ __device__ __forceinline__ void rot2(float2 &a, float2 &b, float c, float s) {
 // post (pre) phases of the gates on tile bits 0..6 at bit pattern i, entry
 // 128 + i the same for tile bits 7..12, so amplitude j gets
 // tab[j & 127] * tab[128 + (j >> 7)].
-constexpr int kPhaseTab = 192;
+constexpr int kPhaseTab = QGTL_PHASE_TAB;
 
 __device__ __forceinline__ void build_phase_tables(const qgtl_pass_desc_t &d, float2 *post_tab, float2 *pre_tab) {
     for (int t = threadIdx.x; t < kPhaseTab; t += kThreads) {
@@ -82,6 +82,15 @@ __device__ __forceinline__ void build_phase_tables(const qgtl_pass_desc_t &d, fl
         }
         post_tab[t] = a;
         pre_tab[t] = b;
+    }
+}
+
+// The same tables built by the host (post then pre, kPhaseTab entries each),
+// copied in instead of rebuilt by every workgroup of the launch.
+__device__ __forceinline__ void load_phase_tables(const float2 *__restrict__ g, float2 *post_tab, float2 *pre_tab) {
+    for (int t = threadIdx.x; t < 2 * kPhaseTab; t += kThreads) {
+        if (t < kPhaseTab) post_tab[t] = g[t];
+        else pre_tab[t - kPhaseTab] = g[t];
     }
 }
 
@@ -171,10 +180,13 @@ constexpr int kLdsTileElems = 1 << QGTL_TILE_BITS_MAX;
 // One tile r of one state: load, gate groups through LDS, store.
 __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
-                                              float2 *post_tab, float2 *pre_tab) {
+                                              float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr) {
     const int m = d.m, nlow = d.nlow, nhi = m - nlow;
     const bool rotf = d.form == 1, pre = rotf && d.has_pre;
-    if (rotf) build_phase_tables(d, post_tab, pre_tab);
+    if (rotf) {
+        if (gtabs) load_phase_tables(gtabs, post_tab, pre_tab);
+        else build_phase_tables(d, post_tab, pre_tab);
+    }
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -294,12 +306,13 @@ __device__ __forceinline__ void pass_lds_body(const qgtl_pass_desc_t &d, const f
 
 __global__ __launch_bounds__(kThreads) void pass_c64_kernel(const qgtl_pass_desc_t d,
                                                             const float2 *__restrict__ src_all,
-                                                            float2 *dst_all, size_t stride) {
+                                                            float2 *dst_all, size_t stride,
+                                                            const float2 *__restrict__ gtabs) {
     __shared__ float2 tile[kLdsTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
     pass_lds_body(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride, blockIdx.x,
-                  tile, hi_off, tabs, tabs + kPhaseTab);
+                  tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,11 +493,14 @@ constexpr int kRegTileElems = 1 << kRT;
 template <bool SPLIT = false>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
-                                               float2 *post_tab, float2 *pre_tab) {
+                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr) {
     const int nlow = d.nlow, nhi = kRT - nlow;
     const uint32_t t = threadIdx.x;
     const bool rotf = d.form == 1;
-    if (rotf) build_phase_tables(d, post_tab, pre_tab);
+    if (rotf) {
+        if (gtabs) load_phase_tables(gtabs, post_tab, pre_tab);
+        else build_phase_tables(d, post_tab, pre_tab);
+    }
 
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
@@ -558,12 +574,13 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
 template <bool SPLIT>
 __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
                                                                  const float2 *__restrict__ src_all,
-                                                                 float2 *dst_all, size_t stride) {
+                                                                 float2 *dst_all, size_t stride,
+                                                                 const float2 *__restrict__ gtabs) {
     __shared__ float2 tile[kRegTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
     pass_regs_body<SPLIT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
-                          blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab);
+                          blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -938,7 +955,7 @@ static bool regs_pass(const qgtl_pass_desc_t *desc, int tile_variant) {
 int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc) { return regs_pass(desc, 0) ? 1 : 0; }
 
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
-                      int nstates, int tile_variant, int split_out) {
+                      int nstates, int tile_variant, int split_out, const void *tabs) {
     if (desc->m < 1 || desc->m > QGTL_TILE_BITS_MAX || desc->nlow < 1 || desc->nlow > desc->m ||
         desc->m - desc->nlow > kHiTableBits || desc->ngates > desc->m || desc->nperm > QGTL_PERM_MAX ||
         desc->m + desc->nrest != desc->n || nstates < 1)
@@ -953,15 +970,16 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
     const bool regs = regs_pass(desc, tile_variant);
     if (split_out && (!regs || src == dst || stride < ((size_t)1 << desc->n)))
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: split-plane output needs a register pass, not in place");
+    const float2 *gt = (const float2 *)tabs;
     if (regs && split_out)
         hipLaunchKernelGGL(pass_c64_regs_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)src, (float2 *)dst, stride);
+                           (const float2 *)src, (float2 *)dst, stride, gt);
     else if (regs)
         hipLaunchKernelGGL(pass_c64_regs_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)src, (float2 *)dst, stride);
+                           (const float2 *)src, (float2 *)dst, stride, gt);
     else
         hipLaunchKernelGGL(pass_c64_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)src, (float2 *)dst, stride);
+                           (const float2 *)src, (float2 *)dst, stride, gt);
     QGTL_LAUNCH_CHECK("pass_c64");
     return 0;
 }

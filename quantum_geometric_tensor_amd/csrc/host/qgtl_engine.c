@@ -216,6 +216,9 @@ static void plan_free_device(struct qgtl_plan *p)
     qgtl_hip_free(p->psi[1]);
     qgtl_hip_free(p->D);
     qgtl_hip_free(p->tmp);
+    qgtl_hip_free(p->tabs_dev);
+    p->tabs_dev = NULL;
+    p->tabs_passes = 0;
     qgtl_hip_free(p->inj_partials);
     qgtl_hip_free(p->inj_sums);
     qgtl_hip_free(p->gram_partials);
@@ -289,6 +292,7 @@ void qgtl_plan_destroy(qgtl_plan_t *p)
 {
     if (!p) return;
     plan_free_device(p);
+    free(p->tabs_host);
     free(p->ops);
     free(p->blocks);
     free(p->params);
@@ -392,6 +396,41 @@ static int alloc_plain(struct qgtl_plan *p)
 /* Apply block b to nstates complex64 states at *cur (stride N); a pass that
  * flips writes to *alt and swaps the two.  split_final: the block's last pass
  * writes the split-plane form, always to *alt (it cannot run in place). */
+/* device phase tables of pass i of block b (NULL before they exist) */
+static const void *pass_tabs(const struct qgtl_plan *p, const qgtl_block_t *b, int i)
+{
+    if (!p->tabs_dev) return NULL;
+    int k = i;
+    for (const qgtl_block_t *q = p->blocks; q < b; q++) k += q->npasses;
+    return (const char *)p->tabs_dev + (size_t)k * 2 * QGTL_PHASE_TAB * 8;
+}
+
+/* Build and upload the phase tables of every pass after the angles changed. */
+static int upload_tabs(struct qgtl_plan *p)
+{
+    if (!p->tabs_dirty && p->tabs_dev) return 0;
+    int total = 0;
+    for (int bi = 0; bi < p->nblocks; bi++) total += p->blocks[bi].npasses;
+    if (total == 0) { p->tabs_dirty = 0; return 0; }
+    const size_t per = 2 * QGTL_PHASE_TAB * 2;                /* floats per pass */
+    if (total != p->tabs_passes) {
+        qgtl_hip_free(p->tabs_dev);
+        p->tabs_dev = NULL;
+        free(p->tabs_host);
+        p->tabs_host = (float *)malloc(sizeof(float) * per * (size_t)total);
+        if (!p->tabs_host) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+        HIPCALL(qgtl_hip_malloc(&p->tabs_dev, sizeof(float) * per * (size_t)total));
+        p->tabs_passes = total;
+    }
+    int k = 0;
+    for (int bi = 0; bi < p->nblocks; bi++)
+        for (int i = 0; i < p->blocks[bi].npasses; i++, k++)
+            qgtl_pass_phase_tables(&p->blocks[bi].pass[i], p->tabs_host + per * (size_t)k);
+    HIPCALL(qgtl_hip_h2d(p->tabs_dev, p->tabs_host, sizeof(float) * per * (size_t)total));
+    p->tabs_dirty = 0;
+    return 0;
+}
+
 static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
                        qgtl_stats_t *st, int split_final)
 {
@@ -399,7 +438,7 @@ static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, v
         const int split = split_final && i == b->npasses - 1;
         const int flip = b->pass_flips[i] || split;
         void *dst = flip ? *alt : *cur;
-        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0, split));
+        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0, split, pass_tabs(p, b, i)));
         if (st) {
             st->sweep_launches++;
             st->sweep_state_passes += nstates;
@@ -468,6 +507,8 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
 static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
 {
     int rc = alloc_mixed(p);
+    if (rc) return rc;
+    rc = upload_tabs(p);
     if (rc) return rc;
     void *cur = p->psi[0], *alt = p->psi[1];
     LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));

@@ -86,6 +86,11 @@ struct qgtl_plan {
     uint64_t structure_hash;
     void *psi_final;                     /* primal after the last execute (mixed) */
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
+    /* rotation-form phase tables of every pass (block order), built on the host */
+    int tabs_dirty;                      /* angles changed since the last upload */
+    int tabs_passes;
+    float *tabs_host;                    /* tabs_passes x 2 x QGTL_PHASE_TAB complex64 */
+    void *tabs_dev;
     /* Hamiltonian for energy / gradient */
     int nterms;
     void *terms_dev;
@@ -116,6 +121,7 @@ int qgtl_init_if_needed(void);
 
 /* learned preconditioner (qgtl_mlp.c) */
 void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank);
+void qgtl_pass_phase_tables(const qgtl_pass_desc_t *p, float *out);
 
 /* file container of qgtl_io.c (prefix, kind header, payload, FNV-1a 64 checksum) */
 uint64_t qgtl_fnv1a(uint64_t h, const void *data, size_t n);

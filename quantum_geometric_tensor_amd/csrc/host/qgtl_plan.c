@@ -627,7 +627,36 @@ int qgtl_plan_build(struct qgtl_plan *plan)
     free(order);
     free(u_at);
     free(g);
+    plan->tabs_dirty = 1;                                     /* new angles: phase tables to rebuild */
     return 0;
+}
+
+/* The rotation-form phase tables of pass p (the kernel's build_phase_tables,
+ * here in FP64 and rounded once): out[t], t < QGTL_PHASE_TAB, the post phases,
+ * out[QGTL_PHASE_TAB + t] the pre phases, as interleaved float pairs. */
+void qgtl_pass_phase_tables(const qgtl_pass_desc_t *p, float *out)
+{
+    for (int t = 0; t < QGTL_PHASE_TAB; t++) {
+        const int lo = t < 128;
+        const int i = lo ? t : t - 128;
+        double ar = 1.0, ai = 0.0, br = 1.0, bi = 0.0;
+        for (int g = 0; g < p->ngates; g++) {
+            const int bit = p->gate_bit[g];
+            if (lo ? bit >= 7 : bit < 7) continue;
+            const int v = (i >> (lo ? bit : bit - 7)) & 1;
+            const double pr = p->post[g][v].re, pi = p->post[g][v].im;
+            const double qr = p->pre[g][v].re, qi = p->pre[g][v].im;
+            const double a2 = ar * pr - ai * pi, b2 = br * qr - bi * qi;
+            ai = ar * pi + ai * pr;
+            ar = a2;
+            bi = br * qi + bi * qr;
+            br = b2;
+        }
+        out[2 * t] = (float)ar;
+        out[2 * t + 1] = (float)ai;
+        out[2 * (QGTL_PHASE_TAB + t)] = (float)br;
+        out[2 * (QGTL_PHASE_TAB + t) + 1] = (float)bi;
+    }
 }
 
 /* Diagnostics: max |diag(post) R diag(pre) - U| over every gate of every pass
