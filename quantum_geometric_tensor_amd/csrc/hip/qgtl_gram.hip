@@ -378,8 +378,17 @@ __device__ __forceinline__ void gram_reduce_elem(const float *__restrict__ parti
     const int R = nt * kTile;
     const int tile = (int)(idx / (kTile * kTile));
     const int e = (int)(idx % (kTile * kTile));
+    // nchunks is a multiple of 8: eight loads in flight, then the adds in chunk order
+    const float *q = partials + (size_t)tile * (kTile * kTile) + e;
+    const size_t cs = (size_t)T * (kTile * kTile);
     double s = 0.0;
-    for (int c = 0; c < nchunks; c++) s += (double)partials[((size_t)c * T + tile) * (kTile * kTile) + e];
+    for (int c = 0; c < nchunks; c += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = q[(size_t)(c + k) * cs];
+#pragma unroll
+        for (int k = 0; k < 8; k++) s += (double)v[k];
+    }
     int ti, tj;
     tile_coords(tile, nt, ti, tj);
     C[(size_t)(ti * kTile + e / kTile) * R + tj * kTile + (e % kTile)] = s;
@@ -513,6 +522,8 @@ int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nc
 }
 
 int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C) {
+    if (P < 1 || nchunks < kXcd || nchunks % kXcd != 0)
+        return qgtl::set_error_msg(-118, "qgtl_hip_gram_reduce: nchunks must be a positive multiple of 8");
     const int nt = (2 * P + kTile - 1) / kTile;
     const size_t total = (size_t)(nt * (nt + 1) / 2) * kTile * kTile;
     unsigned grid = (unsigned)((total + kThreads - 1) / kThreads);
