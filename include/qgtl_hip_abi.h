@@ -32,6 +32,13 @@ enum {
     QGTL_STORE_PERM = 2       /* whole state in one tile; apply perm_* (CNOT/SWAP list) while storing */
 };
 
+enum {
+    QGTL_LOAD_NATURAL = 0,    /* read the tile from where its amplitudes are */
+    QGTL_LOAD_RING = 1        /* tile = bits 0..m-1 of the output; read it from the two half tiles the ring
+                                 store would have written it to (the inverse ring), register pass only,
+                                 not in place, natural store */
+};
+
 /* One LDS-tiled sweep pass: a set of commuting 2x2 matrices on tile bits,
  * followed by an optional index permutation on store. */
 typedef struct {
@@ -42,7 +49,7 @@ typedef struct {
     int32_t store;                             /* QGTL_STORE_* */
     int32_t ngates;                            /* matrices in this pass (<= m) */
     int32_t nperm;                             /* QGTL_STORE_PERM: gates in perm_* */
-    int32_t pad0;
+    int32_t load;                              /* QGTL_LOAD_* */
     /* Index arrays are 32-bit so that kernels read them with scalar loads. */
     int32_t hi_pos[QGTL_TILE_BITS_MAX];        /* physical bit of tile bit nlow+i */
     int32_t rest_pos[QGTL_MAX_QUBITS];         /* physical bit of tile-index bit i */
@@ -81,6 +88,10 @@ typedef struct {
     qgtl_c64_t rot[QGTL_TILE_BITS_MAX];
     qgtl_c64_t pre[QGTL_TILE_BITS_MAX][2];
     qgtl_c64_t post[QGTL_TILE_BITS_MAX][2];
+    /* QGTL_LOAD_RING: thread bits of the layout the gathered tile is read into
+     * from LDS (register bits ph_rbits[0]; lanes 0..3 on independent banks
+     * through the ring map) */
+    int32_t gather_tbits[8];
 } qgtl_pass_desc_t;
 
 /* Derivative-state injection in the frame after a block:

@@ -59,7 +59,7 @@ typedef struct {
     int32_t P;
     int32_t n;
     int32_t presplit;               /* 1: final derivative states were written as the Gram's fp16 pieces */
-    int32_t reserved;
+    int32_t meet_block;             /* derivative states met after this block (the last one: all carried forward) */
 } qgtl_stats_t;
 
 /* Tunables chosen by the autotuner (or defaults when NULL / zero fields). */
@@ -73,7 +73,11 @@ typedef struct {
     int32_t no_presplit;      /* 0: where the plan allows it, the final pass of every derivative state (and the
                                  last block's injection) writes the Gram's fp16 operand pieces directly, so the
                                  Gram does no conversion; 1: states stay complex64 and the Gram splits them */
-    int32_t reserved[3];
+    int32_t sweep_schedule;   /* 0: two-sided where the plan allows it (n > 13, CNOT-ring or empty M-stages):
+                                 the derivative states of late blocks are carried backwards through the
+                                 inverse blocks and all states meet after the block that minimises the
+                                 sweep work; 1: every derivative state is carried forward to the end */
+    int32_t reserved[2];
 } qgtl_tune_t;
 
 int qgtl_init(int device);

@@ -40,7 +40,7 @@ class Stats(ctypes.Structure):
                 ("sweep_bytes", ctypes.c_double), ("gram_launches", ctypes.c_int64),
                 ("gram_flops", ctypes.c_double), ("gram_kernel_ms", ctypes.c_double),
                 ("kernel_launches", ctypes.c_int64), ("P", ctypes.c_int32), ("n", ctypes.c_int32),
-                ("presplit", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("presplit", ctypes.c_int32), ("meet_block", ctypes.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -49,7 +49,7 @@ class Stats(ctypes.Structure):
 class Tune(ctypes.Structure):
     _fields_ = [("chunk_states", ctypes.c_int32), ("gram_chunks", ctypes.c_int32),
                 ("gram_variant", ctypes.c_int32), ("sweep_form", ctypes.c_int32), ("no_presplit", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("sweep_schedule", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 _lib = None
@@ -142,10 +142,10 @@ class Plan:
     """A compiled circuit with device buffers, re-run with new angles."""
 
     def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0, sweep_form=0,
-                 presplit=True):
+                 presplit=True, two_sided=True):
         self.ops = np.ascontiguousarray(ops, OP_DTYPE).copy()
         self.n = n
-        tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form, 0 if presplit else 1)
+        tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form, 0 if presplit else 1, 0 if two_sided else 1)
         h = lib().qgtl_plan_create(n, len(self.ops), _p(self.ops), mode, ctypes.byref(tune))
         if not h:
             raise QGTLError(-1, lib().qgtl_last_error().decode(errors="replace"))

@@ -422,6 +422,16 @@ struct RegLoop {
         RegLoop<R - 1>::load_lds(tile, L, v);
         v[R - 1] = tile[rpad(reg_index<R - 1>(L))];
     }
+    // element x of the tile from where store_lds_ring puts it
+    __device__ __forceinline__ static void load_lds_ring(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs],
+                                                        uint32_t par_r) {
+        RegLoop<R - 1>::load_lds_ring(tile, L, v, par_r);
+        uint32_t p = reg_index<R - 1>(L);
+        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
+        p &= (1u << kRT) - 1u;
+        const uint32_t y = p ^ (((p >> (kRT - 1)) & 1u) ^ par_r);
+        v[R - 1] = tile[rpad(y)];
+    }
     __device__ __forceinline__ static void phase(const float2 *tab, const RegLayout &L, float2 (&v)[kRegs]) {
         RegLoop<R - 1>::phase(tab, L, v);
         v[R - 1] = cmul(v[R - 1], phase_at(tab, reg_index<R - 1>(L)));
@@ -442,6 +452,16 @@ struct RegLoop {
         const uint32_t j = reg_index<R - 2>(L);
         const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
         *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
+    }
+    // the two half tiles of store_global_ring, read
+    __device__ __forceinline__ static void load_global_ring(const float2 *src, uint32_t pre, uint32_t ones,
+                                                            const RegLayout &L, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global_ring(src, pre, ones, L, v);
+        const uint32_t y = reg_index<R - 2>(L);
+        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
+        const float4 q = *reinterpret_cast<const float4 *>(src + (((size_t)yhi << kRT) | y));
+        v[R - 2] = make_float2(q.x, q.y);
+        v[R - 1] = make_float2(q.z, q.w);
     }
     __device__ __forceinline__ static void store_global_ring(float2 *dst, uint32_t pre, uint32_t ones,
                                                              const RegLayout &L, const float2 (&v)[kRegs]) {
@@ -474,6 +494,9 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void store_lds(float2 *, const RegLayout &, const float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_lds_ring(float2 *, const RegLayout &, const float2 (&)[kRegs], uint32_t) {}
     __device__ __forceinline__ static void load_lds(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_lds_ring(const float2 *, const RegLayout &, float2 (&)[kRegs], uint32_t) {}
+    __device__ __forceinline__ static void load_global_ring(const float2 *, uint32_t, uint32_t, const RegLayout &,
+                                                            float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void phase(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global(const float2 *, uint32_t, const uint32_t *, int, uint32_t,
                                                        const RegLayout &, float2 (&)[kRegs]) {}
@@ -488,6 +511,21 @@ struct RegLoop<0> {
 };
 
 constexpr int kRegTileElems = 1 << kRT;
+
+// Output high bits of a ring store from tile r (bits 0..12 of an n-qubit
+// state): prefix_xor(r), complemented (xor ones) when output bit 12 is set;
+// par_r = parity(r) enters output bit 0.
+struct RingFrame {
+    uint32_t pre, ones, par_r;
+};
+
+__device__ __forceinline__ RingFrame ring_frame(int n, uint32_t r) {
+    const int nh = n - kRT;
+    uint32_t pre = r;
+    pre ^= pre << 1; pre ^= pre << 2; pre ^= pre << 4; pre ^= pre << 8; pre ^= pre << 16;
+    const uint32_t ones = (nh >= 32) ? 0xffffffffu : ((1u << nh) - 1u);
+    return RingFrame{pre & ones, ones, (uint32_t)__popc(r) & 1u};
+}
 
 // SPLIT: dst receives the split-plane form of the result (a final pass).
 template <bool SPLIT = false>
@@ -514,10 +552,24 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
 
     float2 v[kRegs];
     const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], t);
-    RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
-    if (rotf && d.has_pre) RegLoop<kRegs>::phase(pre_tab, L0, v);
-
     RegLayout Lc = L0;
+    const bool ring_load = d.load == QGTL_LOAD_RING;
+    if (ring_load) {
+        // inverse ring: tile r of the output gathers from the two half tiles the
+        // ring store writes tile r to (coalesced), then undoes the in-tile part
+        // of the map through LDS into a layout with phase 0's register bits
+        const RingFrame f = ring_frame(d.n, r);
+        RegLoop<kRegs>::load_global_ring(src, f.pre, f.ones, L0, v);
+        RegLoop<kRegs>::store_lds(tile, L0, v);
+        __syncthreads();
+        Lc = make_layout(d.ph_rbits[0], d.gather_tbits, t);
+        RegLoop<kRegs>::load_lds_ring(tile, Lc, v, f.par_r);
+        __syncthreads();
+    } else {
+        RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
+    }
+    if (rotf && d.has_pre) RegLoop<kRegs>::phase(pre_tab, Lc, v);
+
     for (int ph = 0; ph < d.nphase; ph++) {
         if (ph > 0) {
             const RegLayout Ln = make_layout(d.ph_rbits[ph], d.ph_tbits[ph], t);
@@ -545,21 +597,16 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     if (rotf) RegLoop<kRegs>::phase(post_tab, Lc, v);
 
     if (d.store == QGTL_STORE_RING) {
-        const int nh = d.n - kRT;
-        uint32_t pre = r;
-        pre ^= pre << 1; pre ^= pre << 2; pre ^= pre << 4; pre ^= pre << 8; pre ^= pre << 16;
-        const uint32_t ones = (nh >= 32) ? 0xffffffffu : ((1u << nh) - 1u);
-        pre &= ones;
-        const uint32_t par_r = (uint32_t)__popc(r) & 1u;
-        RegLoop<kRegs>::store_lds_ring(tile, Lc, v, par_r);
+        const RingFrame f = ring_frame(d.n, r);
+        RegLoop<kRegs>::store_lds_ring(tile, Lc, v, f.par_r);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
         if constexpr (SPLIT)
-            RegLoop<kRegs>::store_global_ring_split(reinterpret_cast<uint32_t *>(dst), pre, ones, L0, v);
+            RegLoop<kRegs>::store_global_ring_split(reinterpret_cast<uint32_t *>(dst), f.pre, f.ones, L0, v);
         else
-            RegLoop<kRegs>::store_global_ring(dst, pre, ones, L0, v);
+            RegLoop<kRegs>::store_global_ring(dst, f.pre, f.ones, L0, v);
     } else {
-        if (d.nphase > 1) {
+        if (d.nphase > 1 || ring_load) {
             RegLoop<kRegs>::store_lds(tile, Lc, v);
             __syncthreads();
             RegLoop<kRegs>::load_lds(tile, L0, v);
@@ -952,6 +999,16 @@ static bool regs_pass(const qgtl_pass_desc_t *desc, int tile_variant) {
            tile_variant != 1;
 }
 
+static bool tbits_ok(const qgtl_pass_desc_t *desc, const int32_t *tb) {
+    uint32_t used = 0;
+    for (int i = 0; i < 5; i++) used |= 1u << desc->ph_rbits[0][i];
+    for (int i = 0; i < 8; i++) {
+        if (tb[i] < 0 || tb[i] >= kRT || ((used >> tb[i]) & 1u)) return false;
+        used |= 1u << tb[i];
+    }
+    return used == (1u << kRT) - 1u;
+}
+
 int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc) { return regs_pass(desc, 0) ? 1 : 0; }
 
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
@@ -968,6 +1025,11 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: ring store cannot run in place");
     const dim3 grid(1u << desc->nrest, (unsigned)nstates);
     const bool regs = regs_pass(desc, tile_variant);
+    if (desc->load != QGTL_LOAD_NATURAL &&
+        (desc->load != QGTL_LOAD_RING || !regs || desc->nlow != kRT || desc->n <= kRT ||
+         desc->store != QGTL_STORE_INPLACE || src == dst || !tbits_ok(desc, desc->gather_tbits)))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: the ring gather needs a register pass over bits 0..12 "
+                                         "of a wider state, a natural store and separate buffers");
     if (split_out && (!regs || src == dst || stride < ((size_t)1 << desc->n)))
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: split-plane output needs a register pass, not in place");
     const float2 *gt = (const float2 *)tabs;

@@ -393,15 +393,11 @@ static int alloc_plain(struct qgtl_plan *p)
     return 0;
 }
 
-/* Apply block b to nstates complex64 states at *cur (stride N); a pass that
- * flips writes to *alt and swaps the two.  split_final: the block's last pass
- * writes the split-plane form, always to *alt (it cannot run in place). */
-/* device phase tables of pass i of block b (NULL before they exist) */
-static const void *pass_tabs(const struct qgtl_plan *p, const qgtl_block_t *b, int i)
+/* device phase tables of pass i of block b (inv: of its inverse), NULL before they exist */
+static const void *pass_tabs(const struct qgtl_plan *p, const qgtl_block_t *b, int i, int inv)
 {
     if (!p->tabs_dev) return NULL;
-    int k = i;
-    for (const qgtl_block_t *q = p->blocks; q < b; q++) k += q->npasses;
+    const int k = b->tab0 + (inv ? b->npasses : 0) + i;
     return (const char *)p->tabs_dev + (size_t)k * 2 * QGTL_PHASE_TAB * 8;
 }
 
@@ -410,7 +406,10 @@ static int upload_tabs(struct qgtl_plan *p)
 {
     if (!p->tabs_dirty && p->tabs_dev) return 0;
     int total = 0;
-    for (int bi = 0; bi < p->nblocks; bi++) total += p->blocks[bi].npasses;
+    for (int bi = 0; bi < p->nblocks; bi++) {
+        p->blocks[bi].tab0 = total;
+        total += p->blocks[bi].npasses + p->blocks[bi].ninv;
+    }
     if (total == 0) { p->tabs_dirty = 0; return 0; }
     const size_t per = 2 * QGTL_PHASE_TAB * 2;                /* floats per pass */
     if (total != p->tabs_passes) {
@@ -423,22 +422,32 @@ static int upload_tabs(struct qgtl_plan *p)
         p->tabs_passes = total;
     }
     int k = 0;
-    for (int bi = 0; bi < p->nblocks; bi++)
+    for (int bi = 0; bi < p->nblocks; bi++) {
         for (int i = 0; i < p->blocks[bi].npasses; i++, k++)
             qgtl_pass_phase_tables(&p->blocks[bi].pass[i], p->tabs_host + per * (size_t)k);
+        for (int i = 0; i < p->blocks[bi].ninv; i++, k++)
+            qgtl_pass_phase_tables(&p->blocks[bi].ipass[i], p->tabs_host + per * (size_t)k);
+    }
     HIPCALL(qgtl_hip_h2d(p->tabs_dev, p->tabs_host, sizeof(float) * per * (size_t)total));
     p->tabs_dirty = 0;
     return 0;
 }
 
-static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
-                       qgtl_stats_t *st, int split_final)
+/* Run passes[0..np) (of block b; inv: its inverse) over nstates complex64
+ * states at *cur (stride N); a pass that flips writes to *alt and swaps the
+ * two.  split_final: the last pass writes the split-plane form, always to
+ * *alt (it cannot run in place). */
+static int run_passes(struct qgtl_plan *p, const qgtl_block_t *b, int inv, void **cur, void **alt, int nstates,
+                      qgtl_stats_t *st, int split_final)
 {
-    for (int i = 0; i < b->npasses; i++) {
-        const int split = split_final && i == b->npasses - 1;
-        const int flip = b->pass_flips[i] || split;
+    const qgtl_pass_desc_t *passes = inv ? b->ipass : b->pass;
+    const int *flips = inv ? b->ipass_flips : b->pass_flips;
+    const int np = inv ? b->ninv : b->npasses;
+    for (int i = 0; i < np; i++) {
+        const int split = split_final && i == np - 1;
+        const int flip = flips[i] || split;
         void *dst = flip ? *alt : *cur;
-        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], *cur, dst, p->N, nstates, 0, split, pass_tabs(p, b, i)));
+        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&passes[i], *cur, dst, p->N, nstates, 0, split, pass_tabs(p, b, i, inv)));
         if (st) {
             st->sweep_launches++;
             st->sweep_state_passes += nstates;
@@ -446,6 +455,15 @@ static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, v
         }
         if (flip) { void *t = *cur; *cur = *alt; *alt = t; }
     }
+    return 0;
+}
+
+/* Apply block b (its passes, then any per-gate M-stage kernels). */
+static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
+                       qgtl_stats_t *st, int split_final)
+{
+    int rc = run_passes(p, b, 0, cur, alt, nstates, st, split_final);
+    if (rc) return rc;
     if (b->mstage_kernels)
         for (int i = 0; i < b->nperm; i++) {
             if (b->perm_type[i] == 0)
@@ -461,29 +479,89 @@ static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, v
     return 0;
 }
 
-/* flips of block b for a derivative-state chunk (the forced one of a split final pass included) */
-static int block_flips(const struct qgtl_plan *p, int bi)
+/* flips of block bi for a derivative-state chunk carried forward (inv = 0) or
+ * backward (inv = 1), the forced one of a split final pass included: the last
+ * forward block is meet, the last backward block meet + 1 */
+static int block_flips(const struct qgtl_plan *p, int bi, int inv)
 {
     const qgtl_block_t *b = &p->blocks[bi];
+    const int np = inv ? b->ninv : b->npasses;
+    const int *flips = inv ? b->ipass_flips : b->pass_flips;
+    const int final_block = inv ? p->meet + 1 : p->meet;
     int f = 0;
-    for (int i = 0; i < b->npasses; i++)
-        f += b->pass_flips[i] || (p->presplit && bi == p->nblocks - 1 && i == b->npasses - 1);
+    for (int i = 0; i < np; i++) f += flips[i] || (p->presplit && bi == final_block && i == np - 1);
     return f;
 }
 
-/* Split-plane final states: depth-first plans whose last block ends with a
- * register-phase pass and no separate M-stage kernels, fp16 Gram pieces. */
+/* Sweep passes of block bi applied to one state forward (M-stage kernels included) */
+static long fwd_cost(const qgtl_block_t *b) { return b->npasses + (b->mstage_kernels ? b->nperm : 0); }
+
+/* State-passes of a two-sided schedule meeting after block c (-1: not possible):
+ * parameters of blocks b <= c are injected after block b and carried forward
+ * to c; those of blocks b > c are injected at block b's input (the block's
+ * inverse applied to the derivative is a 2 x 2 on the parameter's qubit) and
+ * carried backward through blocks b-1 .. c+1.  The inner products the Gram
+ * forms are the same after any common unitary, so all states may meet at c. */
+static long meet_cost(const struct qgtl_plan *p, int c)
+{
+    for (int b = c + 1; b < p->nblocks; b++)
+        if (!p->blocks[b].inv_ok) return -1;       /* H psi is carried back through all of them */
+    long cost = 0;
+    for (int b = 0; b < p->nblocks; b++) {
+        const long K = p->blocks[b].param_end - p->blocks[b].param_begin;
+        if (b <= c)
+            for (int b2 = b + 1; b2 <= c; b2++) cost += K * fwd_cost(&p->blocks[b2]);
+        else
+            for (int b2 = c + 1; b2 < b; b2++) cost += K * p->blocks[b2].ninv;
+    }
+    return cost;
+}
+
+static int choose_meet(const struct qgtl_plan *p, int depth_first)
+{
+    const int last = p->nblocks - 1;
+    if (!depth_first || p->tune.sweep_schedule == 1 || last < 1) return last;
+    int best = last;
+    long best_cost = meet_cost(p, last);
+    for (int c = last - 1; c >= 0; c--) {
+        const long k = meet_cost(p, c);
+        if (k >= 0 && k < best_cost) { best_cost = k; best = c; }
+    }
+    return best;
+}
+
+/* Split-plane final states: depth-first plans in which every chunk that is
+ * carried anywhere ends with a register-phase pass (the last pass of block
+ * meet forward, the last inverse pass of block meet + 1 backward) and no
+ * separate M-stage kernels, fp16 Gram pieces; chunks injected at the meeting
+ * point are written split by the injection. */
 static int presplit_ok(const struct qgtl_plan *p, int depth_first, int terms)
 {
     if (p->tune.no_presplit || !depth_first || p->P == 0 || p->nblocks < 1 || p->N % 64 != 0) return 0;
     if (terms != QGTL_GRAM_F16X3 && terms != QGTL_GRAM_F16X4) return 0;
-    const qgtl_block_t *b = &p->blocks[p->nblocks - 1];
-    if (b->npasses < 1 || (b->mstage_kernels && b->nperm > 0)) return 0;
-    return qgtl_hip_pass_split_ok(&b->pass[b->npasses - 1]);
+    int fwd = 0, bwd = 0;
+    for (int b = 0; b < p->nblocks; b++) {
+        if (p->blocks[b].param_end == p->blocks[b].param_begin) continue;
+        if (b < p->meet) fwd = 1;
+        if (b > p->meet + 1) bwd = 1;
+    }
+    if (fwd) {
+        const qgtl_block_t *b = &p->blocks[p->meet];
+        if (b->npasses < 1 || (b->mstage_kernels && b->nperm > 0)) return 0;
+        if (!qgtl_hip_pass_split_ok(&b->pass[b->npasses - 1])) return 0;
+    }
+    if (bwd) {
+        const qgtl_block_t *b = &p->blocks[p->meet + 1];
+        if (b->ninv < 1 || !qgtl_hip_pass_split_ok(&b->ipass[b->ninv - 1])) return 0;
+    }
+    return 1;
 }
 
-/* Inject parameters slots [s0, s0+cnt) (all from block b) into dst (split-plane form when split). */
-static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st, int split)
+/* Inject parameters slots [s0, s0+cnt) (all from block b) into dst (split-plane
+ * form when split): after the block from its output, or (backward) at the
+ * block's input from the state before it. */
+static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st, int split,
+                  int backward)
 {
     qgtl_inject_desc_t d;
     memset(&d, 0, sizeof d);
@@ -491,16 +569,55 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
     d.K = cnt;
     for (int j = 0; j < cnt; j++) {
         const qgtl_param_t *pp = &p->params[p->slot_param[s0 + j]];
-        d.v[j] = pp->v;
-        d.w[j] = pp->w;
+        const qgtl_mat2 *M = backward ? &pp->B : &pp->A;
+        d.v[j] = backward ? 1u << pp->qubit : pp->v;
+        d.w[j] = backward ? 1u << pp->qubit : pp->w;
         for (int e = 0; e < 4; e++) {
-            d.A[j][e].re = (float)pp->A.m[e].re;
-            d.A[j][e].im = (float)pp->A.m[e].im;
+            d.A[j][e].re = (float)M->m[e].re;
+            d.A[j][e].im = (float)M->m[e].im;
         }
     }
     int tiles = 0;
     LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles, split));
     LAUNCH(K_INJECT, qgtl_hip_reduce_partials_f64(p->inj_partials, cnt, tiles, p->inj_sums + 4 * (size_t)s0));
+    return 0;
+}
+
+/* Depth-first: inject the K parameters of block bi (D slots slot..slot+K-1)
+ * from psi in balanced chunks of at most tmp_states states and carry each
+ * chunk to the meeting point, forward through blocks bi+1 .. meet or
+ * (backward) through the inverses of blocks bi-1 .. meet+1, so that it ends
+ * in its D slot, in split-plane form when the plan says so. */
+static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *psi, qgtl_stats_t *st, int backward)
+{
+    int flips_after = 0;
+    if (!backward)
+        for (int b2 = bi + 1; b2 <= p->meet; b2++) flips_after += block_flips(p, b2, 0);
+    else
+        for (int b2 = bi - 1; b2 > p->meet; b2--) flips_after += block_flips(p, b2, 1);
+    const int direct = backward ? bi == p->meet + 1 : bi == p->meet;   /* nothing to carry */
+    const int nch = (K + (int)p->tmp_states - 1) / (int)p->tmp_states;
+    const int C = (K + nch - 1) / nch;
+    for (int j0 = 0; j0 < K; j0 += C) {
+        const int cnt = (K - j0 < C) ? K - j0 : C;
+        void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
+        void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
+        void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
+        int rc = inject(p, slot + j0, cnt, psi, c_cur, st, p->presplit && direct, backward);
+        if (rc) return rc;
+        if (!backward) {
+            for (int b2 = bi + 1; b2 <= p->meet; b2++) {
+                rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet);
+                if (rc) return rc;
+            }
+        } else {
+            for (int b2 = bi - 1; b2 > p->meet; b2--) {
+                rc = run_passes(p, &p->blocks[b2], 1, &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet + 1);
+                if (rc) return rc;
+            }
+        }
+        if (c_cur != dslot) return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: chunk ended off its slot");
+    }
     return 0;
 }
 
@@ -513,10 +630,16 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     void *cur = p->psi[0], *alt = p->psi[1];
     LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));
     const int depth_first = any_flip(p);
+    p->meet = choose_meet(p, depth_first);
     p->presplit = presplit_ok(p, depth_first, gram_terms(p));
     int slot = 0;
     for (int bi = 0; bi < p->nblocks; bi++) {
         const qgtl_block_t *b = &p->blocks[bi];
+        const int K = b->param_end - b->param_begin;
+        if (depth_first && bi > p->meet && K > 0) {
+            rc = run_chunks(p, bi, slot, K, cur, st, 1);    /* cur: the state before block bi */
+            if (rc) return rc;
+        }
         if (!depth_first && slot > 0) {
             void *dc = p->D, *da = NULL;
             rc = apply_block(p, b, &dc, &da, slot, st, 0);
@@ -524,35 +647,15 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
         }
         rc = apply_block(p, b, &cur, &alt, 1, st, 0);
         if (rc) return rc;
-        const int K = b->param_end - b->param_begin;
-        if (K == 0) continue;
-        if (!depth_first) {
+        if (K > 0 && !depth_first) {
             for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                 const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
-                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st, 0);
+                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st, 0, 0);
                 if (rc) return rc;
             }
-        } else {
-            int flips_after = 0;
-            for (int b2 = bi + 1; b2 < p->nblocks; b2++) flips_after += block_flips(p, b2);
-            const int last = bi == p->nblocks - 1;
-            /* balanced chunks of at most tmp_states states */
-            const int nch = (K + (int)p->tmp_states - 1) / (int)p->tmp_states;
-            const int C = (K + nch - 1) / nch;
-            for (int j0 = 0; j0 < K; j0 += C) {
-                const int cnt = (K - j0 < C) ? K - j0 : C;
-                void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
-                void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
-                void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
-                rc = inject(p, slot + j0, cnt, cur, c_cur, st, p->presplit && last);
-                if (rc) return rc;
-                for (int b2 = bi + 1; b2 < p->nblocks; b2++) {
-                    rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st,
-                                     p->presplit && b2 == p->nblocks - 1);
-                    if (rc) return rc;
-                }
-                if (c_cur != dslot) return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: chunk ended off its slot");
-            }
+        } else if (K > 0 && bi <= p->meet) {
+            rc = run_chunks(p, bi, slot, K, cur, st, 0);
+            if (rc) return rc;
         }
         slot += K;
     }
@@ -565,6 +668,20 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     if (st) {
         st->gram_launches += 2;
         st->gram_flops += 4.0 * (double)p->N * p->P * (p->P + 1);
+    }
+    return 0;
+}
+
+/* Carry one complex64 state from the end of the circuit back to the meeting
+ * point of the last execution (the inverses of blocks nblocks-1 .. meet+1);
+ * *state and *scratch may swap. */
+int qgtl_carry_to_meet(struct qgtl_plan *p, void **state, void **scratch)
+{
+    qgtl_stats_t *st = NULL;
+    for (int b = p->nblocks - 1; b > p->meet; b--) {
+        if (!p->blocks[b].inv_ok) return qgtl_set_error(QGT_ERROR_INVALID_STATE, "block %d has no inverse", b);
+        int rc = run_passes(p, &p->blocks[b], 1, state, scratch, 1, st, 0);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -651,6 +768,7 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
     rc = (p->mode == QGTL_MODE_MIXED) ? run_mixed(p, st) : run_plain(p, st);
     if (rc) { qgtl_hip_sync(); prof_reset(); return rc; }
     st->presplit = p->mode == QGTL_MODE_MIXED ? p->presplit : 0;
+    st->meet_block = p->mode == QGTL_MODE_MIXED ? p->meet : p->nblocks - 1;
     const size_t P = (size_t)p->P;
     double *Qslot = NULL;
     if (Q && P > 0) {

@@ -86,12 +86,18 @@ int qgtl_energy_gradient_device(qgtl_plan_t *p)
     }
     int t1 = 0, t2 = 0;
     TRYH(qgtl_hip_pauli_apply_c64(p->psi_final, p->hpsi, p->n, p->nterms, p->terms_dev));
-    if (P > 0) {
-        TRYH(qgtl_hip_dot_states_c64(p->D, N, (int)P, p->hpsi, p->n, p->eg_partials, &t1, p->presplit));
-        TRYH(qgtl_hip_reduce_partials_f64(p->eg_partials, (int)P, t1, p->eg_sums));
-    }
     TRYH(qgtl_hip_dot_states_c64(p->psi_final, N, 1, p->hpsi, p->n, p->eg_partials + P * tiles * 4, &t2, 0));
     TRYH(qgtl_hip_reduce_partials_f64(p->eg_partials + P * tiles * 4, 1, t2, p->eg_sums + 4 * P));
+    if (P > 0) {
+        /* the derivative states sit after block meet: <d_k|H psi> there needs
+         * H psi carried back by the same inverse blocks (the other primal
+         * buffer is free scratch after the execution) */
+        void *h = p->hpsi, *scratch = (p->psi_final == p->psi[0]) ? p->psi[1] : p->psi[0];
+        int rc = qgtl_carry_to_meet(p, &h, &scratch);
+        if (rc) return rc;
+        TRYH(qgtl_hip_dot_states_c64(p->D, N, (int)P, h, p->n, p->eg_partials, &t1, p->presplit));
+        TRYH(qgtl_hip_reduce_partials_f64(p->eg_partials, (int)P, t1, p->eg_sums));
+    }
     TRYH(qgtl_hip_energy_gradient_finalize(p->eg_sums, p->eg_sums + 4 * P, (int)P, p->eg_out));
     return 0;
 }

@@ -47,6 +47,13 @@ typedef struct {
     qgtl_pass_desc_t pass[QGTL_MAX_PASSES_PER_BLOCK];
     int pass_flips[QGTL_MAX_PASSES_PER_BLOCK];  /* 1: output goes to the other buffer */
     int mstage_kernels;                  /* 1: M-stage applied by per-gate kernels after the passes */
+    /* the inverse block (the inverse M-stage, then U^dagger), for derivative
+     * states carried backwards (n > 13, M-stage a CNOT ring or empty) */
+    int inv_ok;
+    int ninv;
+    qgtl_pass_desc_t ipass[QGTL_MAX_PASSES_PER_BLOCK];
+    int ipass_flips[QGTL_MAX_PASSES_PER_BLOCK];
+    int tab0;                            /* phase tables of pass i at tab0 + i, of ipass i at tab0 + npasses + i */
 } qgtl_block_t;
 
 typedef struct {
@@ -55,6 +62,8 @@ typedef struct {
     int op_index;
     qgtl_mat2 A;                         /* d_k (pre-M) = A psi_block (pre-M) */
     uint32_t v, w;                       /* injection masks after the M-stage */
+    qgtl_mat2 B;                         /* u_q^dagger A u_q: the block's inverse applied to d_k, a 2 x 2 on
+                                            the parameter's qubit acting on the block's input state */
 } qgtl_param_t;
 
 struct qgtl_plan {
@@ -86,6 +95,8 @@ struct qgtl_plan {
     uint64_t structure_hash;
     void *psi_final;                     /* primal after the last execute (mixed) */
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
+    int meet;                            /* D holds the derivative states after block meet (two-sided schedule:
+                                            later blocks' states are carried backwards); nblocks - 1: the end */
     /* rotation-form phase tables of every pass (block order), built on the host */
     int tabs_dirty;                      /* angles changed since the last upload */
     int tabs_passes;
@@ -122,6 +133,7 @@ int qgtl_init_if_needed(void);
 /* learned preconditioner (qgtl_mlp.c) */
 void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank);
 void qgtl_pass_phase_tables(const qgtl_pass_desc_t *p, float *out);
+int qgtl_carry_to_meet(struct qgtl_plan *p, void **state, void **scratch);
 
 /* file container of qgtl_io.c (prefix, kind header, payload, FNV-1a 64 checksum) */
 uint64_t qgtl_fnv1a(uint64_t h, const void *data, size_t n);

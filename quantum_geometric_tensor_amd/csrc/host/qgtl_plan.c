@@ -366,6 +366,59 @@ static void plan_phases(qgtl_pass_desc_t *p)
     }
 }
 
+/* High-qubit passes of a block with n > 13 (matrices u[q]): the qubits >= 13
+ * that carry a gate, in groups that leave CMIN_LOW_BITS contiguous low bits
+ * in the tile; in place. */
+static void high_passes(const qgtl_block_t *b, int n, const qgtl_mat2 *u, qgtl_pass_desc_t *out, int *flips,
+                        int *count)
+{
+    const int TB = QGTL_TILE_BITS_MAX;
+    int hq[QGTL_MAX_QUBITS], nh = 0;
+    for (int q = TB; q < n; q++)
+        if (b->has_u[q]) hq[nh++] = q;
+    const int group = TB - CMIN_LOW_BITS;
+    for (int g0 = 0; g0 < nh; g0 += group) {
+        const int len = (nh - g0 < group) ? nh - g0 : group;
+        const int c = TB - len;
+        qgtl_pass_desc_t *p = &out[*count];
+        memset(p, 0, sizeof *p);
+        p->n = n; p->m = TB; p->nlow = c;
+        int in_tile[QGTL_MAX_QUBITS] = {0};
+        for (int i = 0; i < c; i++) in_tile[i] = 1;
+        for (int i = 0; i < len; i++) {
+            p->hi_pos[i] = (int8_t)hq[g0 + i];
+            in_tile[hq[g0 + i]] = 1;
+            p->gate_bit[i] = (int8_t)(c + i);
+            set_gate(p, i, &u[hq[g0 + i]]);
+        }
+        p->ngates = len;
+        for (int q = 0; q < n; q++)
+            if (!in_tile[q]) p->rest_pos[p->nrest++] = (int8_t)q;
+        p->store = QGTL_STORE_INPLACE;
+        flips[*count] = 0;
+        (*count)++;
+    }
+}
+
+/* The pass over tile bits 0..12 (matrices u[q], q < 13) with the given load
+ * and store; returns 0 when it has nothing to do. */
+static int low_pass(const qgtl_block_t *b, int n, const qgtl_mat2 *u, qgtl_pass_desc_t *p, int store, int load)
+{
+    const int TB = QGTL_TILE_BITS_MAX;
+    memset(p, 0, sizeof *p);
+    p->n = n; p->m = TB; p->nlow = TB;
+    for (int q = TB; q < n; q++) p->rest_pos[p->nrest++] = (int8_t)q;
+    for (int q = 0; q < TB; q++)
+        if (b->has_u[q]) {
+            p->gate_bit[p->ngates] = (int8_t)q;
+            set_gate(p, p->ngates, &u[q]);
+            p->ngates++;
+        }
+    p->store = store;
+    p->load = load;
+    return p->ngates > 0 || store != QGTL_STORE_INPLACE || load != QGTL_LOAD_NATURAL;
+}
+
 static void build_passes(qgtl_block_t *b, int n)
 {
     const int TB = QGTL_TILE_BITS_MAX;
@@ -401,49 +454,39 @@ static void build_passes(qgtl_block_t *b, int n)
         }
         return;
     }
-    /* high qubits, in groups that leave CMIN_LOW_BITS contiguous low bits */
-    int hq[QGTL_MAX_QUBITS], nh = 0;
-    for (int q = TB; q < n; q++)
-        if (b->has_u[q]) hq[nh++] = q;
-    const int group = TB - CMIN_LOW_BITS;
-    for (int g0 = 0; g0 < nh; g0 += group) {
-        const int len = (nh - g0 < group) ? nh - g0 : group;
-        const int c = TB - len;
-        qgtl_pass_desc_t *p = &b->pass[b->npasses];
-        memset(p, 0, sizeof *p);
-        p->n = n; p->m = TB; p->nlow = c;
-        int in_tile[QGTL_MAX_QUBITS] = {0};
-        for (int i = 0; i < c; i++) in_tile[i] = 1;
-        for (int i = 0; i < len; i++) {
-            p->hi_pos[i] = (int8_t)hq[g0 + i];
-            in_tile[hq[g0 + i]] = 1;
-            p->gate_bit[i] = (int8_t)(c + i);
-            set_gate(p, i, &b->u[hq[g0 + i]]);
-        }
-        p->ngates = len;
-        for (int q = 0; q < n; q++)
-            if (!in_tile[q]) p->rest_pos[p->nrest++] = (int8_t)q;
-        p->store = QGTL_STORE_INPLACE;
-        b->pass_flips[b->npasses] = 0;
-        b->npasses++;
-    }
-    qgtl_pass_desc_t *p = &b->pass[b->npasses];
-    memset(p, 0, sizeof *p);
-    p->n = n; p->m = TB; p->nlow = TB;
-    for (int q = TB; q < n; q++) p->rest_pos[p->nrest++] = (int8_t)q;
-    for (int q = 0; q < TB; q++)
-        if (b->has_u[q]) {
-            p->gate_bit[p->ngates] = (int8_t)q;
-            set_gate(p, p->ngates, &b->u[q]);
-            p->ngates++;
-        }
-    p->store = b->is_ring ? QGTL_STORE_RING : QGTL_STORE_INPLACE;
-    if (!b->is_ring && b->nperm > 0) b->mstage_kernels = 1;
-    if (p->ngates > 0 || b->is_ring) {
+    high_passes(b, n, b->u, b->pass, b->pass_flips, &b->npasses);
+    if (low_pass(b, n, b->u, &b->pass[b->npasses], b->is_ring ? QGTL_STORE_RING : QGTL_STORE_INPLACE,
+                 QGTL_LOAD_NATURAL)) {
         b->pass_flips[b->npasses] = b->is_ring;
         b->npasses++;
     }
+    if (!b->is_ring && b->nperm > 0) b->mstage_kernels = 1;
     for (int i = 0; i < b->npasses; i++) plan_phases(&b->pass[i]);
+}
+
+/* The inverse of block b (n > 13): the inverse ring as a gather into the low
+ * pass, which also applies u_q^dagger on bits 0..12, then the high passes with
+ * u_q^dagger.  Blocks whose M-stage is neither the ring nor empty have none. */
+static void build_inverse_passes(qgtl_block_t *b, int n)
+{
+    b->ninv = 0;
+    b->inv_ok = 0;
+    if (n <= QGTL_TILE_BITS_MAX || (!b->is_ring && b->nperm > 0)) return;
+    qgtl_mat2 uadj[QGTL_MAX_QUBITS];
+    for (int q = 0; q < n; q++) uadj[q] = mat_adj(&b->u[q]);
+    if (low_pass(b, n, uadj, &b->ipass[0], QGTL_STORE_INPLACE, b->is_ring ? QGTL_LOAD_RING : QGTL_LOAD_NATURAL)) {
+        b->ipass_flips[0] = b->is_ring;
+        b->ninv = 1;
+    }
+    int nh = 0;
+    high_passes(b, n, uadj, &b->ipass[b->ninv], &b->ipass_flips[b->ninv], &nh);
+    b->ninv += nh;
+    for (int i = 0; i < b->ninv; i++) {
+        qgtl_pass_desc_t *p = &b->ipass[i];
+        plan_phases(p);
+        if (p->load == QGTL_LOAD_RING) choose_thread_bits(p->ph_rbits[0], 0, 1, p->gather_tbits);
+    }
+    b->inv_ok = 1;
 }
 
 static void detect_ring(qgtl_block_t *b, int n)
@@ -603,7 +646,9 @@ int qgtl_plan_build(struct qgtl_plan *plan)
         qgtl_block_t *b = &plan->blocks[bi];
         detect_ring(b, n);
         build_passes(b, n);
+        build_inverse_passes(b, n);
         for (int i = 0; i < b->npasses; i++) finish_form(&b->pass[i], plan->tune.sweep_form == 1);
+        for (int i = 0; i < b->ninv; i++) finish_form(&b->ipass[i], plan->tune.sweep_form == 1);
         for (int j = b->param_begin; j < b->param_end; j++) {
             qgtl_param_t *pp = &plan->params[order[j]];
             const int q = pp->qubit;
@@ -612,6 +657,9 @@ int qgtl_plan_build(struct qgtl_plan *plan)
             qgtl_mat2 Sadj = mat_adj(&S);
             qgtl_mat2 t = mat_mul(&S, &pp->A);
             pp->A = mat_mul(&t, &Sadj);
+            const qgtl_mat2 uadj = mat_adj(&b->u[q]);
+            t = mat_mul(&uadj, &pp->A);
+            pp->B = mat_mul(&t, &b->u[q]);
             pp->v = perm_apply(b, 1u << q);
             uint32_t w = 0;
             for (int j2 = 0; j2 < n; j2++)
@@ -628,6 +676,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
     free(u_at);
     free(g);
     plan->tabs_dirty = 1;                                     /* new angles: phase tables to rebuild */
+    plan->meet = nb - 1;
     return 0;
 }
 

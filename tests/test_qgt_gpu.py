@@ -166,6 +166,59 @@ def test_split_plane_final_states(n, tail):
         assert np.max(np.abs(out[pre][2] - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
 
 
+@pytest.mark.parametrize("n,L", [(14, 3), (15, 4), (16, 5)])
+def test_two_sided_schedule(n, L):
+    """Two-sided schedule: the derivative states of late blocks are injected at their block's
+    input and carried backwards through the inverse blocks (the CNOT ring undone by a gathering
+    load, then u^dagger), and meet the forward-carried ones after a middle block.  Q matches
+    the oracle and the all-forward schedule with fewer state-passes, and the TFIM energy and
+    gradient (H psi carried back to the meeting point) match the oracle."""
+    th, ops, oops = hea(n, L, seed=1300 + n)
+    J, h = 1.0, 0.6
+    out = {}
+    for two in (True, False):
+        p = Q.Plan(n, ops, Q.MODE_MIXED, two_sided=two)
+        p.set_hamiltonian(Q.tfim_terms(n, J, h))
+        q, st = p.execute()
+        out[two] = (q, st, *p.energy_gradient())
+        p.close()
+    assert out[False][1]["meet_block"] == L - 1
+    assert 0 <= out[True][1]["meet_block"] < L - 1
+    assert out[True][1]["sweep_state_passes"] < out[False][1]["sweep_state_passes"]
+    ref = O.qgt(n, oops)
+    for two in (True, False):
+        assert Q.rel_err(out[two][0], ref) <= MIXED_TOL, two
+    assert Q.rel_err(out[True][0], out[False][0]) <= MIXED_TOL
+    e_ref, g_ref = O.tfim_gradient(n, oops, J, h)
+    for two in (True, False):
+        assert abs(out[two][2] - e_ref) <= 1e-6 * max(1.0, abs(e_ref))
+        assert np.max(np.abs(out[two][3] - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
+
+
+def test_two_sided_meets_after_a_block_without_inverse():
+    """A block whose M-stage is neither the CNOT ring nor empty (here one CNOT and one SWAP,
+    applied by per-gate kernels) has no inverse passes: the meeting point moves to or past it,
+    and Q, energy and gradient still match the oracle (general gates: pre-phase layers in the
+    inverse passes too)."""
+    n = 14
+    gates = _mixed_circuit(n, 5, 77)
+    # replace the ring of layer 3 (gates of a layer: 2 n single-qubit + n CNOT) by two gates
+    per = 3 * n
+    ring3 = slice(3 * per + 2 * n, 4 * per)
+    gates[ring3] = [("CNOT", 0, 5, -1, 0.0), ("SWAP", 2, 13, -1, 0.0)]
+    ops, oops = Q.make_ops(gates), O.make_ops(gates)
+    p = Q.Plan(n, ops, Q.MODE_MIXED)
+    p.set_hamiltonian(Q.tfim_terms(n, 1.0, 0.9))
+    q, st = p.execute()
+    e, g = p.energy_gradient()
+    p.close()
+    assert st["meet_block"] >= 3
+    assert Q.rel_err(q, O.qgt(n, oops)) <= MIXED_TOL
+    e_ref, g_ref = O.tfim_gradient(n, oops, 1.0, 0.9)
+    assert abs(e - e_ref) <= 1e-6 * max(1.0, abs(e_ref))
+    assert np.max(np.abs(g - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
+
+
 @pytest.mark.parametrize("n", [14, 16])
 def test_lds_pass_kernel_on_register_plans(n):
     """Batched sweeps with the LDS gate-group kernel forced for full 13-bit tiles
