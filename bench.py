@@ -165,11 +165,12 @@ def main():
     N = 1 << n
 
     plan = Q.Plan(n, ops, Q.MODE_MIXED)
+    Qmix = np.empty((P, P), np.complex128)
     for _ in range(args.warmup):
-        Qmix, _ = plan.execute()
+        plan.execute(out=Qmix)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        Qmix, _ = plan.execute()
+        plan.execute(out=Qmix)
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     qgt_per_s = 1e3 / ms_per_step

@@ -92,6 +92,21 @@ typedef struct {
      * from LDS (register bits ph_rbits[0]; lanes 0..3 on independent banks
      * through the ring map) */
     int32_t gather_tbits[8];
+    /* Rotation form in the register-phase kernel: the gate on register bit j
+     * of phase p is its rotation with one factor taken out, (a, b) ->
+     * (a - t b, b + t a) with t = s / c (|c| >= |s|), or (t a - b, a + t b)
+     * with t = c / s when bit j of ph_uform[p] is set, t = ph_tan[p][j]; the
+     * product of the factors taken out (c or s of every gate) is scale.  A
+     * phase layer is (the tables at the layout's thread bits) x (its register
+     * part): reg_post[r] / reg_pre[r] is the product, over the register bits
+     * set in r, of phase[1] / phase[0] of the gate on that tile bit (last /
+     * first layout).  has_post = 0: every post phase is 1. */
+    float ph_tan[QGTL_MAX_PHASES][5];
+    int32_t ph_uform[QGTL_MAX_PHASES];
+    int32_t has_post;
+    float scale;
+    qgtl_c64_t reg_post[32];
+    qgtl_c64_t reg_pre[32];
 } qgtl_pass_desc_t;
 
 /* Derivative-state injection in the frame after a block:

@@ -156,10 +156,17 @@ class Plan:
         angles = np.ascontiguousarray(angles, np.float64)
         _check(lib().qgtl_plan_set_angles(self._h, len(angles), _p(angles)))
 
-    def execute(self, want_q=True):
+    def execute(self, want_q=True, out=None):
+        """Q (P x P complex128, parameter order) and the call's stats; out: a C-contiguous
+        complex128 P x P array to write Q into instead of a new one."""
         st = Stats()
-        Q = np.zeros((self.P, self.P), np.complex128) if want_q else None
-        _check(lib().qgtl_plan_execute(self._h, _p(Q) if want_q else None, ctypes.byref(st)))
+        if out is not None:
+            if out.dtype != np.complex128 or out.shape != (self.P, self.P) or not out.flags["C_CONTIGUOUS"]:
+                raise ValueError("out must be a C-contiguous complex128 array of shape (P, P)")
+            Q = out
+        else:
+            Q = np.empty((self.P, self.P), np.complex128) if want_q else None
+        _check(lib().qgtl_plan_execute(self._h, _p(Q) if Q is not None else None, ctypes.byref(st)))
         return Q, st.as_dict()
 
     def set_hamiltonian(self, terms):

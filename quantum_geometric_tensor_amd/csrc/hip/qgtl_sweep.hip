@@ -365,17 +365,32 @@ __device__ __forceinline__ void apply_reg(float2 (&v)[kRegs], const float2 *m) {
         if (!((r >> P) & 1)) apply2(v[r], v[r | (1 << P)], m);
 }
 
-template <int P>
-__device__ __forceinline__ void apply_reg_rot(float2 (&v)[kRegs], float c, float s) {
+// Rotation with its factor taken out: (a, b) -> (a - t b, b + t a) (UFORM =
+// false, t = s / c) or (t a - b, a + t b) (t = c / s): 2 FMA per amplitude.
+template <int P, bool UFORM>
+__device__ __forceinline__ void apply_reg_scaled(float2 (&v)[kRegs], float t) {
 #pragma unroll
-    for (int r = 0; r < kRegs; r++)
-        if (!((r >> P) & 1)) rot2(v[r], v[r | (1 << P)], c, s);
+    for (int r = 0; r < kRegs; r++) {
+        if ((r >> P) & 1) continue;
+        const float2 a = v[r], b = v[r | (1 << P)];
+        if (UFORM) {
+            v[r] = make_float2(fmaf(t, a.x, -b.x), fmaf(t, a.y, -b.y));
+            v[r | (1 << P)] = make_float2(fmaf(t, b.x, a.x), fmaf(t, b.y, a.y));
+        } else {
+            v[r] = make_float2(fmaf(-t, b.x, a.x), fmaf(-t, b.y, a.y));
+            v[r | (1 << P)] = make_float2(fmaf(t, a.x, b.x), fmaf(t, a.y, b.y));
+        }
+    }
 }
 
-// Gate of register bit J in phase ph, if any (scalar branch, compile-time pairing).
+// Gate of register bit J in phase ph, if any (scalar branches, compile-time pairing).
 template <int J>
 __device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const qgtl_pass_desc_t &d, int ph, int mask) {
-    if ((mask >> J) & 1) apply_reg_rot<J>(v, d.ph_rot[ph][J].re, d.ph_rot[ph][J].im);
+    if ((mask >> J) & 1) {
+        const float t = d.ph_tan[ph][J];
+        if ((d.ph_uform[ph] >> J) & 1) apply_reg_scaled<J, true>(v, t);
+        else apply_reg_scaled<J, false>(v, t);
+    }
 }
 
 template <int J>
@@ -432,9 +447,12 @@ struct RegLoop {
         const uint32_t y = p ^ (((p >> (kRT - 1)) & 1u) ^ par_r);
         v[R - 1] = tile[rpad(y)];
     }
-    __device__ __forceinline__ static void phase(const float2 *tab, const RegLayout &L, float2 (&v)[kRegs]) {
-        RegLoop<R - 1>::phase(tab, L, v);
-        v[R - 1] = cmul(v[R - 1], phase_at(tab, reg_index<R - 1>(L)));
+    // phase layer as (thread part pt) x (register part from the descriptor)
+    template <bool POST>
+    __device__ __forceinline__ static void phase_split(float2 pt, const qgtl_pass_desc_t &d, float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::template phase_split<POST>(pt, d, v);
+        const qgtl_c64_t q = POST ? d.reg_post[R - 1] : d.reg_pre[R - 1];
+        v[R - 1] = cmul(v[R - 1], cmul(pt, make_float2(q.re, q.im)));
     }
     // global access in pairs (register bit 0 = tile bit 0)
     __device__ __forceinline__ static void load_global(const float2 *src, uint32_t base, const uint32_t *hi_off,
@@ -497,7 +515,8 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void load_lds_ring(const float2 *, const RegLayout &, float2 (&)[kRegs], uint32_t) {}
     __device__ __forceinline__ static void load_global_ring(const float2 *, uint32_t, uint32_t, const RegLayout &,
                                                             float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void phase(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
+    template <bool POST>
+    __device__ __forceinline__ static void phase_split(float2, const qgtl_pass_desc_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global(const float2 *, uint32_t, const uint32_t *, int, uint32_t,
                                                        const RegLayout &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global(float2 *, uint32_t, const uint32_t *, int, uint32_t,
@@ -511,6 +530,18 @@ struct RegLoop<0> {
 };
 
 constexpr int kRegTileElems = 1 << kRT;
+
+// Phase layer of a rotation-form pass in layout L: the tables give the phase
+// of the thread's register-0 element (register bits 0), the descriptor the
+// register part, and scale the factors the scaled-out rotations dropped.
+template <bool POST>
+__device__ __forceinline__ void phase_layer(const float2 *tab, const RegLayout &L, const qgtl_pass_desc_t &d,
+                                            float scale, float2 (&v)[kRegs]) {
+    float2 pt = phase_at(tab, L.tdep);
+    pt.x *= scale;
+    pt.y *= scale;
+    RegLoop<kRegs>::template phase_split<POST>(pt, d, v);
+}
 
 // Output high bits of a ring store from tile r (bits 0..12 of an n-qubit
 // state): prefix_xor(r), complemented (xor ones) when output bit 12 is set;
@@ -568,7 +599,7 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     } else {
         RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
     }
-    if (rotf && d.has_pre) RegLoop<kRegs>::phase(pre_tab, Lc, v);
+    if (rotf && d.has_pre) phase_layer<false>(pre_tab, Lc, d, d.has_post ? 1.f : d.scale, v);
 
     for (int ph = 0; ph < d.nphase; ph++) {
         if (ph > 0) {
@@ -594,7 +625,7 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
             apply_phase_mat<4>(v, d, ph, mask);
         }
     }
-    if (rotf) RegLoop<kRegs>::phase(post_tab, Lc, v);
+    if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
 
     if (d.store == QGTL_STORE_RING) {
         const RingFrame f = ring_frame(d.n, r);

@@ -224,6 +224,8 @@ static void plan_free_device(struct qgtl_plan *p)
     qgtl_hip_free(p->gram_partials);
     qgtl_hip_free(p->C);
     qgtl_hip_free(p->Qdev);
+    qgtl_hip_host_free(p->Qhost);
+    p->Qhost = NULL;
     qgtl_hip_free(p->plain_partials);
     qgtl_hip_free(p->terms_dev);
     qgtl_hip_free(p->hpsi);
@@ -371,6 +373,7 @@ static int alloc_mixed(struct qgtl_plan *p)
         const size_t nt = (2 * P + 127) / 128, R = nt * 128;
         HIPCALL(qgtl_hip_malloc((void **)&p->C, R * R * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
+        HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
     }
     return 0;
 }
@@ -389,6 +392,7 @@ static int alloc_plain(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->C, P * P * 2 * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
+        HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
     }
     return 0;
 }
@@ -770,18 +774,17 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
     st->presplit = p->mode == QGTL_MODE_MIXED ? p->presplit : 0;
     st->meet_block = p->mode == QGTL_MODE_MIXED ? p->meet : p->nblocks - 1;
     const size_t P = (size_t)p->P;
-    double *Qslot = NULL;
-    if (Q && P > 0) {
-        Qslot = (double *)malloc(P * P * 2 * sizeof(double));
-        if (!Qslot) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+    /* Q comes back through the plan's pinned buffer (a direct DMA copy) */
+    double *Qslot = (Q && P > 0) ? p->Qhost : NULL;
+    if (Qslot) {
         rc = qgtl_hip_d2h(Qslot, p->Qdev, P * P * 2 * sizeof(double));
-        if (rc) { free(Qslot); return hip_fail(rc, "download Q"); }
+        if (rc) return hip_fail(rc, "download Q");
     }
     rc = qgtl_hip_sync();
-    if (rc) { free(Qslot); return hip_fail(rc, "qgtl_plan_execute"); }
+    if (rc) return hip_fail(rc, "qgtl_plan_execute");
     st->wall_ms = now_ms() - t0;
     rc = prof_collect(st);
-    if (rc) { free(Qslot); return rc; }
+    if (rc) return rc;
     if (Qslot) {
         if (p->mode == QGTL_MODE_MIXED) {
             /* slot order (creation order) -> parameter numbers */
@@ -794,7 +797,6 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
         } else {
             memcpy(Q, Qslot, P * P * 2 * sizeof(double));
         }
-        free(Qslot);
     }
     return 0;
 }

@@ -90,6 +90,7 @@ struct qgtl_plan {
     int gram_chunks;
     double *C;                           /* mixed: R x R;  plain: G (P x P complex) */
     double *Qdev;                        /* P x P complex */
+    double *Qhost;                       /* pinned host copy of Qdev (download staging) */
     double *plain_partials;
     size_t plain_partials_len;
     uint64_t structure_hash;

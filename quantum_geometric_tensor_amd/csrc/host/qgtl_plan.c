@@ -176,6 +176,23 @@ static void factor_rotation(const qgtl_mat2 *U, double *c, double *s, qgtl_cd pr
         if (fabs(-c1 - *c) <= 1e-12 && fabs(-s1 - *s) <= 1e-12) return;
         post[1] = cd(-post[1].re, -post[1].im);
     }
+    /* columns are a phase times a real column: U = R diag(pre), post = (1, 1)
+     * (the adjoint of the row case, as in the inverse of U = RZ RY) */
+    const qgtl_cd q0 = cmul(cconj(a), cc), q1 = cmul(cconj(b), d);
+    if (fabs(q0.im) <= tol && fabs(q1.im) <= tol) {
+        post[0] = post[1] = cd(1.0, 0.0);
+        pre[0] = na >= nc ? cdiv_real(a, na) : cdiv_real(cc, nc);
+        const qgtl_cd p0c = cconj(pre[0]);
+        *c = cmul(a, p0c).re;
+        *s = cmul(cc, p0c).re;
+        /* U11 = c pre1, U01 = -s pre1 */
+        pre[1] = fabs(*c) >= fabs(*s) ? cdiv_real(d, *c) : cdiv_real(cd(-b.re, -b.im), *s);
+        const qgtl_cd e01 = cmul(cd(-*s, 0.0), pre[1]), e11 = cmul(cd(*c, 0.0), pre[1]);
+        if (fabs(e01.re - b.re) <= 1e-12 && fabs(e01.im - b.im) <= 1e-12 && fabs(e11.re - d.re) <= 1e-12 &&
+            fabs(e11.im - d.im) <= 1e-12)
+            return;
+        pre[0] = pre[1] = cd(1.0, 0.0);
+    }
     /* general: pre0 = 1, c = |U00| >= 0, s = |U10| >= 0 */
     *c = na;
     *s = nc;
@@ -222,6 +239,57 @@ static void finish_form(qgtl_pass_desc_t *p, int general)
             p->has_pre = 1;
     }
     p->form = 1;
+}
+
+/* Register-kernel form of a finished rotation-form pass (see
+ * qgtl_pass_desc_t): per gate the scaled-out rotation coefficient and the
+ * factor taken out, and the register parts of the phase layers, from the
+ * block's matrices u (FP64, rounded once). */
+static void finish_rotation(qgtl_pass_desc_t *p, const qgtl_mat2 *u)
+{
+    p->has_post = 0;
+    p->scale = 1.f;
+    for (int ph = 0; ph < QGTL_MAX_PHASES; ph++) p->ph_uform[ph] = 0;
+    for (int r = 0; r < 32; r++) p->reg_post[r] = p->reg_pre[r] = c64(cd(1.0, 0.0));
+    if (p->form != 1) return;
+    double coef[QGTL_TILE_BITS_MAX], scale = 1.0;
+    int uform[QGTL_TILE_BITS_MAX], gate_of_bit[QGTL_TILE_BITS_MAX];
+    qgtl_cd rel_pre[QGTL_TILE_BITS_MAX], rel_post[QGTL_TILE_BITS_MAX];
+    for (int b = 0; b < QGTL_TILE_BITS_MAX; b++) gate_of_bit[b] = -1;
+    for (int g = 0; g < p->ngates; g++) {
+        const int tb = p->gate_bit[g];
+        const int q = tb < p->nlow ? tb : p->hi_pos[tb - p->nlow];
+        double c, sn;
+        qgtl_cd pre[2], post[2];
+        factor_rotation(&u[q], &c, &sn, pre, post);
+        uform[g] = fabs(c) < fabs(sn);
+        coef[g] = uform[g] ? c / sn : sn / c;
+        scale *= uform[g] ? sn : c;
+        rel_pre[g] = cmul(pre[1], cconj(pre[0]));
+        rel_post[g] = cmul(post[1], cconj(post[0]));
+        if (post[0].re != 1.0 || post[0].im != 0.0 || post[1].re != 1.0 || post[1].im != 0.0) p->has_post = 1;
+        gate_of_bit[tb] = g;
+    }
+    p->scale = (float)scale;
+    if (p->nphase < 1) return;
+    for (int ph = 0; ph < p->nphase; ph++)
+        for (int g = p->ph_first[ph]; g < p->ph_first[ph + 1]; g++) {
+            const int j = p->ph_greg[g];
+            p->ph_tan[ph][j] = (float)coef[g];
+            if (uform[g]) p->ph_uform[ph] |= 1 << j;
+        }
+    const int last = p->nphase - 1;
+    for (int r = 0; r < 32; r++) {
+        qgtl_cd a = cd(1.0, 0.0), b = cd(1.0, 0.0);
+        for (int i = 0; i < 5; i++) {
+            if (!((r >> i) & 1)) continue;
+            const int gl = gate_of_bit[p->ph_rbits[last][i]], g0 = gate_of_bit[p->ph_rbits[0][i]];
+            if (gl >= 0) a = cmul(a, rel_post[gl]);
+            if (g0 >= 0) b = cmul(b, rel_pre[g0]);
+        }
+        p->reg_post[r] = c64(a);
+        p->reg_pre[r] = c64(b);
+    }
 }
 
 /* LDS bank of tile index j in the register kernel's swizzled tile:
@@ -647,8 +715,18 @@ int qgtl_plan_build(struct qgtl_plan *plan)
         detect_ring(b, n);
         build_passes(b, n);
         build_inverse_passes(b, n);
-        for (int i = 0; i < b->npasses; i++) finish_form(&b->pass[i], plan->tune.sweep_form == 1);
-        for (int i = 0; i < b->ninv; i++) finish_form(&b->ipass[i], plan->tune.sweep_form == 1);
+        for (int i = 0; i < b->npasses; i++) {
+            finish_form(&b->pass[i], plan->tune.sweep_form == 1);
+            finish_rotation(&b->pass[i], b->u);
+        }
+        if (b->ninv > 0) {
+            qgtl_mat2 uadj[QGTL_MAX_QUBITS];
+            for (int q = 0; q < n; q++) uadj[q] = mat_adj(&b->u[q]);
+            for (int i = 0; i < b->ninv; i++) {
+                finish_form(&b->ipass[i], plan->tune.sweep_form == 1);
+                finish_rotation(&b->ipass[i], uadj);
+            }
+        }
         for (int j = b->param_begin; j < b->param_end; j++) {
             qgtl_param_t *pp = &plan->params[order[j]];
             const int q = pp->qubit;
