@@ -559,6 +559,7 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
         qgtl_tune_t pt;
         memset(&pt, 0, sizeof pt);
         pt.sweep_form = B->tune.sweep_form == 1 ? 1 : 0;
+        pt.sweep_schedule = 1;      /* the batch sweeps breadth-first: no inverse blocks */
         in->plan = qgtl_plan_create(n[i], nops[i], ops + op_off, QGTL_MODE_MIXED, &pt);
         if (!in->plan) {
             char msg[512];
@@ -653,10 +654,17 @@ size_t qgtl_batch_device_bytes(const qgtl_batch_t *B) { return B ? B->device_byt
 int qgtl_batch_set_angles(qgtl_batch_t *B, const double *angles)
 {
     if (!B || !angles) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch or angles");
+    /* the instances' plans are independent: rebuild them on the host cores */
+    int first_bad = B->count;
+#pragma omp parallel for schedule(dynamic, 4) reduction(min : first_bad)
     for (int i = 0; i < B->count; i++) {
         const inst_t *in = &B->inst[i];
-        int rc = qgtl_plan_set_angles(in->plan, in->nops, angles + in->op_off);
-        if (rc) return rc;
+        if (qgtl_plan_set_angles(in->plan, in->nops, angles + in->op_off) != 0 && i < first_bad) first_bad = i;
+    }
+    if (first_bad < B->count) {
+        /* again on this thread, for the error message */
+        const inst_t *in = &B->inst[first_bad];
+        return qgtl_plan_set_angles(in->plan, in->nops, angles + in->op_off);
     }
     fill_descs(B);
     return 0;

@@ -74,6 +74,7 @@ struct qgtl_plan {
     qgtl_op_t *ops;
     int nblocks;
     qgtl_block_t *blocks;
+    int blocks_cap;
     qgtl_param_t *params;                /* indexed by parameter number */
     int *slot_param;                     /* D slot -> parameter number */
     qgtl_tune_t tune;

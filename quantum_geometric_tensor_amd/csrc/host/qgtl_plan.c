@@ -596,6 +596,14 @@ static int expand(const struct qgtl_plan *plan, flat_gate_t **out, int *count)
     return 0;
 }
 
+static qgtl_block_t *open_block(qgtl_block_t *blocks, int i, int n)
+{
+    qgtl_block_t *b = &blocks[i];
+    memset(b, 0, sizeof *b);
+    for (int q = 0; q < n; q++) b->u[q] = mat_identity();
+    return b;
+}
+
 int qgtl_plan_build(struct qgtl_plan *plan)
 {
     const int n = plan->n;
@@ -638,9 +646,14 @@ int qgtl_plan_build(struct qgtl_plan *plan)
     int rc = expand(plan, &g, &ng);
     if (rc) return rc;
 
-    free(plan->blocks);
+    /* the block array is kept across rebuilds (set_angles): each block is
+     * cleared when it is opened, so only the blocks in use are touched */
+    if (!plan->blocks || plan->blocks_cap < ng + 1) {
+        free(plan->blocks);
+        plan->blocks = (qgtl_block_t *)malloc((size_t)(ng + 1) * sizeof(qgtl_block_t));
+        plan->blocks_cap = plan->blocks ? ng + 1 : 0;
+    }
     free(plan->params);
-    plan->blocks = (qgtl_block_t *)calloc((size_t)(ng + 1), sizeof(qgtl_block_t));
     plan->params = (qgtl_param_t *)calloc((size_t)(P > 0 ? P : 1), sizeof(qgtl_param_t));
     qgtl_mat2 *u_at = (qgtl_mat2 *)calloc((size_t)(P > 0 ? P : 1), sizeof(qgtl_mat2));
     int *order = (int *)calloc((size_t)(P > 0 ? P : 1), sizeof(int));   /* params in creation order */
@@ -650,8 +663,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
     }
     int nb = 0, npar = 0;
     int touched[QGTL_MAX_QUBITS];
-    qgtl_block_t *cur = &plan->blocks[0];
-    for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+    qgtl_block_t *cur = open_block(plan->blocks, 0, n);
     memset(touched, 0, sizeof touched);
     cur->param_begin = 0;
     int block_open = 0;
@@ -663,8 +675,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
             /* close the block */
             cur->param_end = npar;
             nb++;
-            cur = &plan->blocks[nb];
-            for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+            cur = open_block(plan->blocks, nb, n);
             memset(touched, 0, sizeof touched);
             cur->param_begin = npar;
             block_open = 0;
@@ -675,8 +686,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
                 /* start a fresh block rather than overflow the M-stage list */
                 cur->param_end = npar;
                 nb++;
-                cur = &plan->blocks[nb];
-                for (int q = 0; q < n; q++) cur->u[q] = mat_identity();
+                cur = open_block(plan->blocks, nb, n);
                 memset(touched, 0, sizeof touched);
                 cur->param_begin = npar;
             }
@@ -714,7 +724,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
         qgtl_block_t *b = &plan->blocks[bi];
         detect_ring(b, n);
         build_passes(b, n);
-        build_inverse_passes(b, n);
+        if (plan->mode == QGTL_MODE_MIXED && plan->tune.sweep_schedule != 1) build_inverse_passes(b, n);
         for (int i = 0; i < b->npasses; i++) {
             finish_form(&b->pass[i], plan->tune.sweep_form == 1);
             finish_rotation(&b->pass[i], b->u);
