@@ -332,20 +332,27 @@ constexpr int kRegs = 32;
 // conflict-free for 8-byte accesses.
 __device__ __forceinline__ int rpad(uint32_t j) { return (int)(j ^ (((j >> 4) ^ (j >> 8) ^ (j >> 12)) & 15u)); }
 
-// Tile index of register r for a thread whose other-bit deposit is tdep.
+// Tile index of register r for a thread whose other-bit deposit is tdep: tdep
+// | (the register bits of r).  rpad is linear over GF(2) and the thread and
+// register parts are disjoint, so the LDS word of register r is lbase ^ (xor
+// of lbit over the bits of r): one XOR with a uniform value per access.
 struct RegLayout {
-    uint32_t tdep;
-    uint32_t rb[5];
+    uint32_t tdep, lbase;
+    uint32_t rb[5], lbit[5];
 };
 
 __device__ __forceinline__ RegLayout make_layout(const int32_t *rbits, const int32_t *tbits, uint32_t t) {
     RegLayout L;
 #pragma unroll
-    for (int i = 0; i < 5; i++) L.rb[i] = 1u << rbits[i];
+    for (int i = 0; i < 5; i++) {
+        L.rb[i] = qgtl::uniform(1u << rbits[i]);
+        L.lbit[i] = (uint32_t)rpad(L.rb[i]);
+    }
     uint32_t d = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) d |= ((t >> i) & 1u) << tbits[i];
+    for (int i = 0; i < 8; i++) d |= ((t >> i) & 1u) << qgtl::uniform(tbits[i]);
     L.tdep = d;
+    L.lbase = (uint32_t)rpad(d);
     return L;
 }
 
@@ -356,6 +363,88 @@ __device__ __forceinline__ uint32_t reg_index(const RegLayout &L) {
     for (int i = 0; i < 5; i++)
         if ((R >> i) & 1) j |= L.rb[i];
     return j;
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t lds_index(const RegLayout &L) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if ((R >> i) & 1) u ^= L.lbit[i];
+    return L.lbase ^ u;
+}
+
+// Linear part of the in-tile ring map (ring_local_inverse's inverse without
+// the parity-of-r constant): prefix xor, bit 12 folded into bit 0.
+__device__ __forceinline__ uint32_t ring_lin(uint32_t x) {
+    uint32_t p = x;
+    p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
+    p &= (1u << kRT) - 1u;
+    return p ^ ((p >> (kRT - 1)) & 1u);
+}
+
+// LDS words of the ring positions of a layout's elements: element x sits at
+// rpad(ring_lin(x) ^ par_r) = base ^ (xor of bit over its register bits).
+struct RingLds {
+    uint32_t base;
+    uint32_t bit[5];
+};
+
+__device__ __forceinline__ RingLds ring_lds(const RegLayout &L, uint32_t par_r) {
+    RingLds q;
+    q.base = (uint32_t)rpad(ring_lin(L.tdep) ^ par_r);
+#pragma unroll
+    for (int i = 0; i < 5; i++) q.bit[i] = (uint32_t)rpad(ring_lin(L.rb[i]));
+    return q;
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t ring_index(const RingLds &q) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        if ((R >> i) & 1) u ^= q.bit[i];
+    return q.base ^ u;
+}
+
+// Global element offsets of a layout's register pairs (register bit 0 = tile
+// bit 0): lane + the sum of bit[i] over register bits i >= 1 of the pair; the
+// parts are disjoint bit sets of the state index.
+struct GlobalMap {
+    size_t lane;
+    uint32_t bit[5];
+};
+
+template <int R>
+__device__ __forceinline__ size_t global_index(const GlobalMap &g) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int i = 1; i < 5; i++)
+        if ((R >> i) & 1) u += g.bit[i];
+    return g.lane + u;
+}
+
+// natural placement: tile bits below nlow are state bits 0..nlow-1, the others
+// go to hi_pos (the deposit table hi_off), the tile index r to base
+__device__ __forceinline__ GlobalMap natural_map(const RegLayout &L, uint32_t base, const uint32_t *hi_off, int nlow) {
+    GlobalMap g;
+    const uint32_t lowmask = (1u << nlow) - 1u;
+    g.lane = (size_t)(base + hi_off[L.tdep >> nlow] + (L.tdep & lowmask));
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+        g.bit[i] = qgtl::uniform((L.rb[i] & lowmask) ? L.rb[i] : hi_off[L.rb[i] >> nlow]);
+    return g;
+}
+
+// ring placement of tile r (bits 0..12 of the output index y): the high part
+// of y is pre, or pre ^ ones when y_12 = 1 (a thread bit of the layout)
+__device__ __forceinline__ GlobalMap ring_map(const RegLayout &L, uint32_t pre, uint32_t ones) {
+    GlobalMap g;
+    const uint32_t yhi = ((L.tdep >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
+    g.lane = ((size_t)yhi << kRT) | L.tdep;
+#pragma unroll
+    for (int i = 0; i < 5; i++) g.bit[i] = L.rb[i];
+    return g;
 }
 
 template <int P>
@@ -422,30 +511,20 @@ template <int R>
 struct RegLoop {
     __device__ __forceinline__ static void store_lds(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs]) {
         RegLoop<R - 1>::store_lds(tile, L, v);
-        tile[rpad(reg_index<R - 1>(L))] = v[R - 1];
-    }
-    __device__ __forceinline__ static void store_lds_ring(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs],
-                                                         uint32_t par_r) {
-        RegLoop<R - 1>::store_lds_ring(tile, L, v, par_r);
-        uint32_t p = reg_index<R - 1>(L);
-        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
-        p &= (1u << kRT) - 1u;
-        const uint32_t y = p ^ (((p >> (kRT - 1)) & 1u) ^ par_r);
-        tile[rpad(y)] = v[R - 1];
+        tile[lds_index<R - 1>(L)] = v[R - 1];
     }
     __device__ __forceinline__ static void load_lds(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs]) {
         RegLoop<R - 1>::load_lds(tile, L, v);
-        v[R - 1] = tile[rpad(reg_index<R - 1>(L))];
+        v[R - 1] = tile[lds_index<R - 1>(L)];
     }
-    // element x of the tile from where store_lds_ring puts it
-    __device__ __forceinline__ static void load_lds_ring(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs],
-                                                        uint32_t par_r) {
-        RegLoop<R - 1>::load_lds_ring(tile, L, v, par_r);
-        uint32_t p = reg_index<R - 1>(L);
-        p ^= p << 1; p ^= p << 2; p ^= p << 4; p ^= p << 8;
-        p &= (1u << kRT) - 1u;
-        const uint32_t y = p ^ (((p >> (kRT - 1)) & 1u) ^ par_r);
-        v[R - 1] = tile[rpad(y)];
+    // element x of the layout to / from its ring position
+    __device__ __forceinline__ static void store_lds_ring(float2 *tile, const RingLds &q, const float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::store_lds_ring(tile, q, v);
+        tile[ring_index<R - 1>(q)] = v[R - 1];
+    }
+    __device__ __forceinline__ static void load_lds_ring(const float2 *tile, const RingLds &q, float2 (&v)[kRegs]) {
+        RegLoop<R - 1>::load_lds_ring(tile, q, v);
+        v[R - 1] = tile[ring_index<R - 1>(q)];
     }
     // phase layer as (thread part pt) x (register part from the descriptor)
     template <bool POST>
@@ -455,78 +534,36 @@ struct RegLoop {
         v[R - 1] = cmul(v[R - 1], cmul(pt, make_float2(q.re, q.im)));
     }
     // global access in pairs (register bit 0 = tile bit 0)
-    __device__ __forceinline__ static void load_global(const float2 *src, uint32_t base, const uint32_t *hi_off,
-                                                       int nlow, uint32_t lowmask, const RegLayout &L, float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::load_global(src, base, hi_off, nlow, lowmask, L, v);
-        const uint32_t j = reg_index<R - 2>(L);
-        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
-        const float4 q = *reinterpret_cast<const float4 *>(src + g);
+    __device__ __forceinline__ static void load_global(const float2 *src, const GlobalMap &g, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global(src, g, v);
+        const float4 q = *reinterpret_cast<const float4 *>(src + global_index<R - 2>(g));
         v[R - 2] = make_float2(q.x, q.y);
         v[R - 1] = make_float2(q.z, q.w);
     }
-    __device__ __forceinline__ static void store_global(float2 *dst, uint32_t base, const uint32_t *hi_off, int nlow,
-                                                        uint32_t lowmask, const RegLayout &L, const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global(dst, base, hi_off, nlow, lowmask, L, v);
-        const uint32_t j = reg_index<R - 2>(L);
-        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
-        *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
+    __device__ __forceinline__ static void store_global(float2 *dst, const GlobalMap &g, const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global(dst, g, v);
+        *reinterpret_cast<float4 *>(dst + global_index<R - 2>(g)) =
+            make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
     }
-    // the two half tiles of store_global_ring, read
-    __device__ __forceinline__ static void load_global_ring(const float2 *src, uint32_t pre, uint32_t ones,
-                                                            const RegLayout &L, float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::load_global_ring(src, pre, ones, L, v);
-        const uint32_t y = reg_index<R - 2>(L);
-        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
-        const float4 q = *reinterpret_cast<const float4 *>(src + (((size_t)yhi << kRT) | y));
-        v[R - 2] = make_float2(q.x, q.y);
-        v[R - 1] = make_float2(q.z, q.w);
-    }
-    __device__ __forceinline__ static void store_global_ring(float2 *dst, uint32_t pre, uint32_t ones,
-                                                             const RegLayout &L, const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global_ring(dst, pre, ones, L, v);
-        const uint32_t y = reg_index<R - 2>(L);
-        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
-        const size_t g = ((size_t)yhi << kRT) | y;
-        *reinterpret_cast<float4 *>(dst + g) = make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
-    }
-    // the same two stores writing the split-plane form
-    __device__ __forceinline__ static void store_global_split(uint32_t *planes, uint32_t base,
-                                                              const uint32_t *hi_off, int nlow, uint32_t lowmask,
-                                                              const RegLayout &L, const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global_split(planes, base, hi_off, nlow, lowmask, L, v);
-        const uint32_t j = reg_index<R - 2>(L);
-        const uint32_t g = base + hi_off[j >> nlow] + (j & lowmask);
-        write_split_pair(planes, g, v[R - 2], v[R - 1]);
-    }
-    __device__ __forceinline__ static void store_global_ring_split(uint32_t *planes, uint32_t pre, uint32_t ones,
-                                                                   const RegLayout &L, const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global_ring_split(planes, pre, ones, L, v);
-        const uint32_t y = reg_index<R - 2>(L);
-        const uint32_t yhi = ((y >> (kRT - 1)) & 1u) ? (pre ^ ones) : pre;
-        write_split_pair(planes, ((size_t)yhi << kRT) | y, v[R - 2], v[R - 1]);
+    // the same stores writing the split-plane form
+    __device__ __forceinline__ static void store_global_split(uint32_t *planes, const GlobalMap &g,
+                                                              const float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::store_global_split(planes, g, v);
+        write_split_pair(planes, global_index<R - 2>(g), v[R - 2], v[R - 1]);
     }
 };
 
 template <>
 struct RegLoop<0> {
     __device__ __forceinline__ static void store_lds(float2 *, const RegLayout &, const float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void store_lds_ring(float2 *, const RegLayout &, const float2 (&)[kRegs], uint32_t) {}
     __device__ __forceinline__ static void load_lds(const float2 *, const RegLayout &, float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void load_lds_ring(const float2 *, const RegLayout &, float2 (&)[kRegs], uint32_t) {}
-    __device__ __forceinline__ static void load_global_ring(const float2 *, uint32_t, uint32_t, const RegLayout &,
-                                                            float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_lds_ring(float2 *, const RingLds &, const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_lds_ring(const float2 *, const RingLds &, float2 (&)[kRegs]) {}
     template <bool POST>
     __device__ __forceinline__ static void phase_split(float2, const qgtl_pass_desc_t &, float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void load_global(const float2 *, uint32_t, const uint32_t *, int, uint32_t,
-                                                       const RegLayout &, float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void store_global(float2 *, uint32_t, const uint32_t *, int, uint32_t,
-                                                        const RegLayout &, const float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void store_global_ring(float2 *, uint32_t, uint32_t, const RegLayout &,
-                                                             const float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void store_global_split(uint32_t *, uint32_t, const uint32_t *, int, uint32_t,
-                                                              const RegLayout &, const float2 (&)[kRegs]) {}
-    __device__ __forceinline__ static void store_global_ring_split(uint32_t *, uint32_t, uint32_t, const RegLayout &,
-                                                                   const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_global(const float2 *, const GlobalMap &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global(float2 *, const GlobalMap &, const float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void store_global_split(uint32_t *, const GlobalMap &, const float2 (&)[kRegs]) {}
 };
 
 constexpr int kRegTileElems = 1 << kRT;
@@ -579,25 +616,27 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
         hi_off[h] = off;
     }
     __syncthreads();
-    const uint32_t lowmask = (1u << nlow) - 1u;
 
     float2 v[kRegs];
     const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], t);
     RegLayout Lc = L0;
-    const bool ring_load = d.load == QGTL_LOAD_RING;
+    const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
+    RingFrame f{0u, 0u, 0u};
+    if (ring_load || ring_store) f = ring_frame(d.n, r);
+    const GlobalMap gnat = natural_map(L0, base, hi_off, nlow);
+    const GlobalMap gring = (ring_load || ring_store) ? ring_map(L0, f.pre, f.ones) : gnat;
     if (ring_load) {
         // inverse ring: tile r of the output gathers from the two half tiles the
         // ring store writes tile r to (coalesced), then undoes the in-tile part
         // of the map through LDS into a layout with phase 0's register bits
-        const RingFrame f = ring_frame(d.n, r);
-        RegLoop<kRegs>::load_global_ring(src, f.pre, f.ones, L0, v);
+        RegLoop<kRegs>::load_global(src, gring, v);
         RegLoop<kRegs>::store_lds(tile, L0, v);
         __syncthreads();
         Lc = make_layout(d.ph_rbits[0], d.gather_tbits, t);
-        RegLoop<kRegs>::load_lds_ring(tile, Lc, v, f.par_r);
+        RegLoop<kRegs>::load_lds_ring(tile, ring_lds(Lc, f.par_r), v);
         __syncthreads();
     } else {
-        RegLoop<kRegs>::load_global(src, base, hi_off, nlow, lowmask, L0, v);
+        RegLoop<kRegs>::load_global(src, gnat, v);
     }
     if (rotf && d.has_pre) phase_layer<false>(pre_tab, Lc, d, d.has_post ? 1.f : d.scale, v);
 
@@ -627,15 +666,14 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     }
     if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
 
-    if (d.store == QGTL_STORE_RING) {
-        const RingFrame f = ring_frame(d.n, r);
-        RegLoop<kRegs>::store_lds_ring(tile, Lc, v, f.par_r);
+    if (ring_store) {
+        RegLoop<kRegs>::store_lds_ring(tile, ring_lds(Lc, f.par_r), v);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
         if constexpr (SPLIT)
-            RegLoop<kRegs>::store_global_ring_split(reinterpret_cast<uint32_t *>(dst), f.pre, f.ones, L0, v);
+            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), gring, v);
         else
-            RegLoop<kRegs>::store_global_ring(dst, f.pre, f.ones, L0, v);
+            RegLoop<kRegs>::store_global(dst, gring, v);
     } else {
         if (d.nphase > 1 || ring_load) {
             RegLoop<kRegs>::store_lds(tile, Lc, v);
@@ -643,9 +681,9 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
             RegLoop<kRegs>::load_lds(tile, L0, v);
         }
         if constexpr (SPLIT)
-            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), base, hi_off, nlow, lowmask, L0, v);
+            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), gnat, v);
         else
-            RegLoop<kRegs>::store_global(dst, base, hi_off, nlow, lowmask, L0, v);
+            RegLoop<kRegs>::store_global(dst, gnat, v);
     }
 }
 
