@@ -224,6 +224,17 @@ int qgtl_hip_event_record(qgtl_hip_event_t ev);
 int qgtl_hip_event_elapsed_ms(qgtl_hip_event_t start, qgtl_hip_event_t stop, float *ms);
 int qgtl_hip_event_destroy(qgtl_hip_event_t ev);
 
+/* ---- a second stream ----
+ * qgtl_hip_set_lane(1) routes the kernel launches that follow to a second
+ * stream (default priority) until qgtl_hip_set_lane(0); copies, syncs and
+ * timing events stay on the library stream.  Order between the lanes goes
+ * through events made by qgtl_hip_event_create_sync: _lane_signal records one
+ * on the current lane, _lane_wait makes the current lane wait for it. */
+int qgtl_hip_set_lane(int lane);
+int qgtl_hip_event_create_sync(qgtl_hip_event_t *ev);
+int qgtl_hip_lane_signal(qgtl_hip_event_t ev);
+int qgtl_hip_lane_wait(qgtl_hip_event_t ev);
+
 /* ---- complex64 sweeps (mixed mode) ---- */
 int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0> */
 /* Split-plane form of a final derivative state (the Gram's operand form): the

@@ -1,6 +1,8 @@
 // Device, stream, memory and event layer of the thin C ABI (qgtl_hip_abi.h).
 // One process drives one GPU through one non-blocking stream of default
-// priority; every kernel and copy of the library is ordered on it.
+// priority; every copy of the library is ordered on it, and so is every
+// kernel unless a caller routes launches to the second stream (lane 1, also
+// of default priority) for work that may overlap, ordered by events.
 #include <stdio.h>
 #include <string.h>
 
@@ -8,6 +10,8 @@
 
 namespace {
 hipStream_t g_stream = nullptr;
+hipStream_t g_side = nullptr;
+int g_lane = 0;
 int g_device = -1;
 thread_local char g_msg[256] = "";
 
@@ -23,7 +27,7 @@ int code_for(hipError_t e) {
 }  // namespace
 
 namespace qgtl {
-hipStream_t stream() { return g_stream; }
+hipStream_t stream() { return g_lane ? g_side : g_stream; }
 
 int set_error(hipError_t e, const char *what) {
     snprintf(g_msg, sizeof g_msg, "%s: %s", what, hipGetErrorString(e));
@@ -55,9 +59,12 @@ int qgtl_hip_init(int device) {
     QGTL_HIP_CHECK(hipSetDevice(device));
     if (g_stream) {
         (void)hipStreamDestroy(g_stream);
-        g_stream = nullptr;
+        (void)hipStreamDestroy(g_side);
+        g_stream = g_side = nullptr;
     }
+    g_lane = 0;
     QGTL_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    QGTL_HIP_CHECK(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
     g_device = device;
     return 0;
 }
@@ -119,6 +126,28 @@ int qgtl_hip_event_create(qgtl_hip_event_t *ev) {
 
 int qgtl_hip_event_record(qgtl_hip_event_t ev) {
     QGTL_HIP_CHECK(hipEventRecord(ev->e, g_stream));
+    return 0;
+}
+
+int qgtl_hip_event_create_sync(qgtl_hip_event_t *ev) {
+    *ev = new qgtl_hip_event_s;
+    QGTL_HIP_CHECK(hipEventCreateWithFlags(&(*ev)->e, hipEventDisableTiming));
+    return 0;
+}
+
+int qgtl_hip_set_lane(int lane) {
+    if (lane != 0 && lane != 1) return qgtl::set_error_msg(-118, "qgtl_hip_set_lane: lane is 0 or 1");
+    g_lane = lane;
+    return 0;
+}
+
+int qgtl_hip_lane_signal(qgtl_hip_event_t ev) {
+    QGTL_HIP_CHECK(hipEventRecord(ev->e, qgtl::stream()));
+    return 0;
+}
+
+int qgtl_hip_lane_wait(qgtl_hip_event_t ev) {
+    QGTL_HIP_CHECK(hipStreamWaitEvent(qgtl::stream(), ev->e, 0));
     return 0;
 }
 

@@ -294,6 +294,8 @@ void qgtl_plan_destroy(qgtl_plan_t *p)
 {
     if (!p) return;
     plan_free_device(p);
+    qgtl_hip_event_destroy(p->ev_psi);
+    qgtl_hip_event_destroy(p->ev_inj);
     free(p->tabs_host);
     free(p->ops);
     free(p->blocks);
@@ -592,7 +594,8 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
  * chunk to the meeting point, forward through blocks bi+1 .. meet or
  * (backward) through the inverses of blocks bi-1 .. meet+1, so that it ends
  * in its D slot, in split-plane form when the plan says so. */
-static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *psi, qgtl_stats_t *st, int backward)
+static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *psi, qgtl_stats_t *st, int backward,
+                      int lanes)
 {
     int flips_after = 0;
     if (!backward)
@@ -609,6 +612,7 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
         void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
         int rc = inject(p, slot + j0, cnt, psi, c_cur, st, p->presplit && direct, backward);
         if (rc) return rc;
+        if (lanes) HIPCALL(qgtl_hip_lane_signal(p->ev_inj));   /* psi may be overwritten after this */
         if (!backward) {
             for (int b2 = bi + 1; b2 <= p->meet; b2++) {
                 rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet);
@@ -625,23 +629,34 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
     return 0;
 }
 
-static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
+/* The mixed-precision QGT.  Depth-first plans run the primal chain (psi
+ * through every block) on the second stream, so that its single-state passes
+ * overlap the derivative chunks on the library stream: psi_b is published by
+ * ev_psi, and the chain waits on ev_inj (the last injection issued) before a
+ * pass overwrites a psi buffer an injection may still read.  With profiling on,
+ * everything stays on one stream (per-launch timing). */
+static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
 {
-    int rc = alloc_mixed(p);
-    if (rc) return rc;
-    rc = upload_tabs(p);
-    if (rc) return rc;
+    int rc;
     void *cur = p->psi[0], *alt = p->psi[1];
-    LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));
     const int depth_first = any_flip(p);
-    p->meet = choose_meet(p, depth_first);
-    p->presplit = presplit_ok(p, depth_first, gram_terms(p));
+    if (lanes) {
+        /* the chain starts after what the library stream holds (the phase-table upload) */
+        HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
+        HIPCALL(qgtl_hip_set_lane(1));
+        HIPCALL(qgtl_hip_lane_wait(p->ev_inj));
+    }
+    LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(cur, p->n));
     int slot = 0;
     for (int bi = 0; bi < p->nblocks; bi++) {
         const qgtl_block_t *b = &p->blocks[bi];
         const int K = b->param_end - b->param_begin;
         if (depth_first && bi > p->meet && K > 0) {
-            rc = run_chunks(p, bi, slot, K, cur, st, 1);    /* cur: the state before block bi */
+            if (lanes) {                                     /* cur: the state before block bi */
+                HIPCALL(qgtl_hip_set_lane(0));
+                HIPCALL(qgtl_hip_lane_wait(p->ev_psi));
+            }
+            rc = run_chunks(p, bi, slot, K, cur, st, 1, lanes);
             if (rc) return rc;
         }
         if (!depth_first && slot > 0) {
@@ -649,8 +664,13 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
             rc = apply_block(p, b, &dc, &da, slot, st, 0);
             if (rc) return rc;
         }
+        if (lanes) {
+            HIPCALL(qgtl_hip_set_lane(1));
+            HIPCALL(qgtl_hip_lane_wait(p->ev_inj));
+        }
         rc = apply_block(p, b, &cur, &alt, 1, st, 0);
         if (rc) return rc;
+        if (lanes) HIPCALL(qgtl_hip_lane_signal(p->ev_psi));
         if (K > 0 && !depth_first) {
             for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                 const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
@@ -658,12 +678,41 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
                 if (rc) return rc;
             }
         } else if (K > 0 && bi <= p->meet) {
-            rc = run_chunks(p, bi, slot, K, cur, st, 0);
+            if (lanes) {
+                HIPCALL(qgtl_hip_set_lane(0));
+                HIPCALL(qgtl_hip_lane_wait(p->ev_psi));
+            }
+            rc = run_chunks(p, bi, slot, K, cur, st, 0, lanes);
             if (rc) return rc;
         }
         slot += K;
     }
+    if (lanes) {
+        /* the library stream takes over psi_final (and everything after it) */
+        HIPCALL(qgtl_hip_set_lane(0));
+        HIPCALL(qgtl_hip_lane_wait(p->ev_psi));
+    }
     p->psi_final = cur;
+    return 0;
+}
+
+static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
+{
+    int rc = alloc_mixed(p);
+    if (rc) return rc;
+    rc = upload_tabs(p);
+    if (rc) return rc;
+    const int depth_first = any_flip(p);
+    p->meet = choose_meet(p, depth_first);
+    p->presplit = presplit_ok(p, depth_first, gram_terms(p));
+    const int lanes = depth_first && !g_profiling;
+    if (lanes && !p->ev_psi) {
+        HIPCALL(qgtl_hip_event_create_sync(&p->ev_psi));
+        HIPCALL(qgtl_hip_event_create_sync(&p->ev_inj));
+    }
+    rc = run_mixed_lanes(p, st, lanes);
+    if (lanes) qgtl_hip_set_lane(0);
+    if (rc) return rc;
     if (p->P == 0) return 0;
     LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p),
                                              p->presplit));

@@ -96,6 +96,7 @@ struct qgtl_plan {
     size_t plain_partials_len;
     uint64_t structure_hash;
     void *psi_final;                     /* primal after the last execute (mixed) */
+    qgtl_hip_event_t ev_psi, ev_inj;     /* ordering of the primal chain on the second stream */
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
     int meet;                            /* D holds the derivative states after block meet (two-sided schedule:
                                             later blocks' states are carried backwards); nblocks - 1: the end */
