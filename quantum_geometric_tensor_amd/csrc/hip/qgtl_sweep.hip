@@ -595,48 +595,67 @@ __device__ __forceinline__ RingFrame ring_frame(int n, uint32_t r) {
     return RingFrame{pre & ones, ones, (uint32_t)__popc(r) & 1u};
 }
 
-// SPLIT: dst receives the split-plane form of the result (a final pass).
-template <bool SPLIT = false>
-__device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
-                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
-                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr) {
-    const int nlow = d.nlow, nhi = kRT - nlow;
-    const uint32_t t = threadIdx.x;
-    const bool rotf = d.form == 1;
-    if (rotf) {
+// Per-workgroup set-up of a register pass: phase tables, the deposit table
+// hi_off of the tile bits above nlow.
+__device__ __forceinline__ void pass_setup(const qgtl_pass_desc_t &d, uint32_t *hi_off, float2 *post_tab,
+                                           float2 *pre_tab, const float2 *gtabs) {
+    const int nhi = kRT - d.nlow;
+    if (d.form == 1) {
         if (gtabs) load_phase_tables(gtabs, post_tab, pre_tab);
         else build_phase_tables(d, post_tab, pre_tab);
     }
-
-    uint32_t base = 0;
-    for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
     for (int h = threadIdx.x; h < (1 << nhi); h += kThreads) {
         uint32_t off = 0;
         for (int i = 0; i < nhi; i++) off |= ((uint32_t)(h >> i) & 1u) << d.hi_pos[i];
         hi_off[h] = off;
     }
     __syncthreads();
+}
 
-    float2 v[kRegs];
-    const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], t);
-    RegLayout Lc = L0;
+// Where tile r of a state is read from and written to, in the load/store
+// layout L0: gin for the load (the ring map for a gathering load), gout for
+// the store (the ring map for a ring store).
+struct TileMaps {
+    RingFrame f;
+    GlobalMap gin, gout;
+};
+
+__device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_t r, const RegLayout &L0,
+                                              const uint32_t *hi_off) {
+    uint32_t base = 0;
+    for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
     const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
-    RingFrame f{0u, 0u, 0u};
-    if (ring_load || ring_store) f = ring_frame(d.n, r);
-    const GlobalMap gnat = natural_map(L0, base, hi_off, nlow);
-    const GlobalMap gring = (ring_load || ring_store) ? ring_map(L0, f.pre, f.ones) : gnat;
+    TileMaps m;
+    m.f = RingFrame{0u, 0u, 0u};
+    if (ring_load || ring_store) m.f = ring_frame(d.n, r);
+    const GlobalMap gnat = natural_map(L0, base, hi_off, d.nlow);
+    const GlobalMap gring = (ring_load || ring_store) ? ring_map(L0, m.f.pre, m.f.ones) : gnat;
+    m.gin = ring_load ? gring : gnat;
+    m.gout = ring_store ? gring : gnat;
+    return m;
+}
+
+// Everything after the tile's amplitudes sit in v (layout L0, as loaded
+// through m.gin): the inverse ring's in-tile part, the phase layers and gate
+// phases, the ring or natural store.  SPLIT: dst receives the split-plane
+// form of the result (a final pass).
+template <bool SPLIT>
+__device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], const TileMaps &m,
+                                                   const RegLayout &L0, float2 *dst, float2 *tile,
+                                                   const float2 *post_tab, const float2 *pre_tab) {
+    const uint32_t t = threadIdx.x;
+    const bool rotf = d.form == 1;
+    const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
+    RegLayout Lc = L0;
     if (ring_load) {
-        // inverse ring: tile r of the output gathers from the two half tiles the
-        // ring store writes tile r to (coalesced), then undoes the in-tile part
-        // of the map through LDS into a layout with phase 0's register bits
-        RegLoop<kRegs>::load_global(src, gring, v);
+        // inverse ring: tile r of the output was gathered from the two half
+        // tiles the ring store writes tile r to (coalesced); undo the in-tile
+        // part of the map through LDS into a layout with phase 0's register bits
         RegLoop<kRegs>::store_lds(tile, L0, v);
         __syncthreads();
         Lc = make_layout(d.ph_rbits[0], d.gather_tbits, t);
-        RegLoop<kRegs>::load_lds_ring(tile, ring_lds(Lc, f.par_r), v);
+        RegLoop<kRegs>::load_lds_ring(tile, ring_lds(Lc, m.f.par_r), v);
         __syncthreads();
-    } else {
-        RegLoop<kRegs>::load_global(src, gnat, v);
     }
     if (rotf && d.has_pre) phase_layer<false>(pre_tab, Lc, d, d.has_post ? 1.f : d.scale, v);
 
@@ -667,24 +686,31 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
 
     if (ring_store) {
-        RegLoop<kRegs>::store_lds_ring(tile, ring_lds(Lc, f.par_r), v);
+        RegLoop<kRegs>::store_lds_ring(tile, ring_lds(Lc, m.f.par_r), v);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
-        if constexpr (SPLIT)
-            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), gring, v);
-        else
-            RegLoop<kRegs>::store_global(dst, gring, v);
-    } else {
-        if (d.nphase > 1 || ring_load) {
-            RegLoop<kRegs>::store_lds(tile, Lc, v);
-            __syncthreads();
-            RegLoop<kRegs>::load_lds(tile, L0, v);
-        }
-        if constexpr (SPLIT)
-            RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), gnat, v);
-        else
-            RegLoop<kRegs>::store_global(dst, gnat, v);
+    } else if (d.nphase > 1 || ring_load) {
+        RegLoop<kRegs>::store_lds(tile, Lc, v);
+        __syncthreads();
+        RegLoop<kRegs>::load_lds(tile, L0, v);
     }
+    if constexpr (SPLIT)
+        RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), m.gout, v);
+    else
+        RegLoop<kRegs>::store_global(dst, m.gout, v);
+}
+
+// One tile r of one state.
+template <bool SPLIT = false>
+__device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
+                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
+                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr) {
+    pass_setup(d, hi_off, post_tab, pre_tab, gtabs);
+    const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], threadIdx.x);
+    const TileMaps m = tile_maps(d, r, L0, hi_off);
+    float2 v[kRegs];
+    RegLoop<kRegs>::load_global(src, m.gin, v);
+    pass_compute_store<SPLIT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
 }
 
 template <bool SPLIT>
