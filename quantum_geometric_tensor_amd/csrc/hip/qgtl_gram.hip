@@ -291,7 +291,8 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
                     if (!diag) load_planes(D16, P, N, tj * (kTile / 2), k0 + kKS, pvb);
                 }
             }
-#pragma unroll 1
+            // unrolled: the next step's fragment reads issue under this step's MFMAs
+#pragma unroll
             for (int kk = 0; kk < kKS / 16; kk++) {
                 const int koff = kk * 16 + 8 * (lane >> 5);
                 frag_t fa[S][2], fb[S][2];
