@@ -195,6 +195,27 @@ def test_two_sided_schedule(n, L):
         assert np.max(np.abs(out[two][3] - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
 
 
+@pytest.mark.parametrize("form", [0, 1])
+def test_two_sided_gate_mix_inverse_passes(form):
+    """Inverse blocks of general gates (H, S, T, SX, ... factored with pre- and post-phases, or
+    general complex matrices with sweep_form=1): six layers at n = 14 meet after block 2, so the
+    states of blocks 4 and 5 run through one and two inverse blocks.  Q, energy and gradient
+    against the oracle."""
+    n = 14
+    gates = _mixed_circuit(n, 6, 321 + form)
+    ops, oops = Q.make_ops(gates), O.make_ops(gates)
+    p = Q.Plan(n, ops, Q.MODE_MIXED, sweep_form=form)
+    p.set_hamiltonian(Q.tfim_terms(n, 1.0, 0.45))
+    q, st = p.execute()
+    e, g = p.energy_gradient()
+    p.close()
+    assert 0 < st["meet_block"] < 4
+    assert Q.rel_err(q, O.qgt(n, oops)) <= MIXED_TOL
+    e_ref, g_ref = O.tfim_gradient(n, oops, 1.0, 0.45)
+    assert abs(e - e_ref) <= 1e-6 * max(1.0, abs(e_ref))
+    assert np.max(np.abs(g - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
+
+
 def test_two_sided_meets_after_a_block_without_inverse():
     """A block whose M-stage is neither the CNOT ring nor empty (here one CNOT and one SWAP,
     applied by per-gate kernels) has no inverse passes: the meeting point moves to or past it,
