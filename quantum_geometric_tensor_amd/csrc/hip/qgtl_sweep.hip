@@ -472,13 +472,34 @@ __device__ __forceinline__ void apply_reg_scaled(float2 (&v)[kRegs], float t) {
     }
 }
 
-// Gate of register bit J in phase ph, if any (scalar branches, compile-time pairing).
+// The descriptor fields one phase of a register pass reads (layout, gate
+// mask, rotation-form coefficients), loaded one phase ahead so that their
+// scalar loads are in flight during the previous phase's work.
+struct PhaseParams {
+    int32_t rbits[5], tbits[8];
+    int32_t mask, uform;
+    float tan[5];
+};
+
+__device__ __forceinline__ PhaseParams load_phase(const qgtl_pass_desc_t &d, int ph) {
+    PhaseParams q;
+#pragma unroll
+    for (int i = 0; i < 5; i++) q.rbits[i] = d.ph_rbits[ph][i];
+#pragma unroll
+    for (int i = 0; i < 8; i++) q.tbits[i] = d.ph_tbits[ph][i];
+    q.mask = d.ph_mask[ph];
+    q.uform = d.ph_uform[ph];
+#pragma unroll
+    for (int i = 0; i < 5; i++) q.tan[i] = d.ph_tan[ph][i];
+    return q;
+}
+
+// Gate of register bit J in the phase, if any (scalar branches, compile-time pairing).
 template <int J>
-__device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const qgtl_pass_desc_t &d, int ph, int mask) {
-    if ((mask >> J) & 1) {
-        const float t = d.ph_tan[ph][J];
-        if ((d.ph_uform[ph] >> J) & 1) apply_reg_scaled<J, true>(v, t);
-        else apply_reg_scaled<J, false>(v, t);
+__device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const PhaseParams &q) {
+    if ((q.mask >> J) & 1) {
+        if ((q.uform >> J) & 1) apply_reg_scaled<J, true>(v, q.tan[J]);
+        else apply_reg_scaled<J, false>(v, q.tan[J]);
     }
 }
 
@@ -659,22 +680,25 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
     }
     if (rotf && d.has_pre) phase_layer<false>(pre_tab, Lc, d, d.has_post ? 1.f : d.scale, v);
 
+    PhaseParams cur = load_phase(d, 0);
     for (int ph = 0; ph < d.nphase; ph++) {
+        PhaseParams nxt = cur;
+        if (ph + 1 < d.nphase) nxt = load_phase(d, ph + 1);
         if (ph > 0) {
-            const RegLayout Ln = make_layout(d.ph_rbits[ph], d.ph_tbits[ph], t);
+            const RegLayout Ln = make_layout(cur.rbits, cur.tbits, t);
             RegLoop<kRegs>::store_lds(tile, Lc, v);
             __syncthreads();
             RegLoop<kRegs>::load_lds(tile, Ln, v);
             __syncthreads();
             Lc = Ln;
         }
-        const int mask = d.ph_mask[ph];
+        const int mask = cur.mask;
         if (rotf) {
-            apply_phase_rot<0>(v, d, ph, mask);
-            apply_phase_rot<1>(v, d, ph, mask);
-            apply_phase_rot<2>(v, d, ph, mask);
-            apply_phase_rot<3>(v, d, ph, mask);
-            apply_phase_rot<4>(v, d, ph, mask);
+            apply_phase_rot<0>(v, cur);
+            apply_phase_rot<1>(v, cur);
+            apply_phase_rot<2>(v, cur);
+            apply_phase_rot<3>(v, cur);
+            apply_phase_rot<4>(v, cur);
         } else {
             apply_phase_mat<0>(v, d, ph, mask);
             apply_phase_mat<1>(v, d, ph, mask);
@@ -682,6 +706,7 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
             apply_phase_mat<3>(v, d, ph, mask);
             apply_phase_mat<4>(v, d, ph, mask);
         }
+        cur = nxt;
     }
     if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
 
