@@ -231,6 +231,9 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
+    // on a diagonal tile the quadrant below the diagonal is the transpose of
+    // the one above it and nothing reads it: its wave only loads and syncs
+    const bool mirror = diag && wm > wn;
 
     f32x16 acc[2][2], outer[2][2];
 #pragma unroll
@@ -294,6 +297,7 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
             // unrolled: the next step's fragment reads issue under this step's MFMAs
 #pragma unroll
             for (int kk = 0; kk < kKS / 16; kk++) {
+                if (mirror) break;
                 const int koff = kk * 16 + 8 * (lane >> 5);
                 frag_t fa[S][2], fb[S][2];
 #pragma unroll
