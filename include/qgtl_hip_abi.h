@@ -119,6 +119,14 @@ typedef struct {
     qgtl_c64_t A[QGTL_INJECT_MAX][4];
 } qgtl_inject_desc_t;
 
+/* One derivative state for a pass that forms it while loading (see
+ * qgtl_hip_pass_c64_inject): element y is M[s][s] psi[y] + M[s][1-s]
+ * psi[y ^ v], s = parity(y & w), M row-major. */
+typedef struct {
+    uint32_t v, w;
+    qgtl_c64_t M[4];
+} qgtl_inject_slot_t;
+
 /* ---- grouped launches ----
  * One launch runs the same kind of work for many independent instances.
  * Work item i covers the units [unit_begin_i, unit_begin_{i+1}) of the
@@ -254,9 +262,16 @@ int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0
 int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, size_t stride,
                       int nstates, int tile_variant, int split_out, const void *tabs);
 int qgtl_hip_pass_split_ok(const qgtl_pass_desc_t *desc);
-/* split_out = 1: the derivative states are written in split-plane form */
+/* split_out = 1: the derivative states are written in split-plane form; partials = NULL:
+ * no FP64 sums (the overlaps are taken from the Gram instead) */
 int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
                         double *partials, int *tiles_out, int split_out);
+/* the FP64 sums of qgtl_hip_inject_c64 only (the states are not written) */
+int qgtl_hip_inject_sums_c64(const qgtl_inject_desc_t *desc, const void *psi, double *partials, int *tiles_out);
+/* qgtl_hip_pass_c64 on nstates derivative states formed while loading from the
+ * one state psi: state s uses slots[s] (device array); register passes only */
+int qgtl_hip_pass_c64_inject(const qgtl_pass_desc_t *desc, const void *psi, const void *slots, void *dst,
+                             size_t stride, int nstates, int split_out, const void *tabs);
 /* out[k*4 + c] = sum over t of partials[(k*tiles + t)*4 + c], fixed order, FP64 */
 int qgtl_hip_reduce_partials_f64(const double *partials, int K, int tiles, double *out);
 int qgtl_hip_cnot_c64(void *base, size_t stride, int nstates, int n, int c, int t);
@@ -286,6 +301,11 @@ int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nc
 int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C);   /* C: R x R tiles, FP64 */
 /* Q = normalised G - a a^H from C and injection sums (P x 4 doubles). */
 int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q);
+/* the same with the overlaps taken from the Gram: C is the Gram of P + 1 states,
+ * state P the primal state at the derivative states' meeting point */
+int qgtl_hip_qgt_finalize_psi(const double *C, int P, double *Q);
+/* one complex64 state of 2^n amplitudes (n >= 6) in split-plane form */
+int qgtl_hip_c64_to_split(const void *src, void *dst, int n);
 /* MFMA lane-layout probe (diagnostics): 32 x 32 floats to out_dev */
 int qgtl_hip_debug_mfma_layout(float *out_dev);
 

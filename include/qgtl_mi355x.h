@@ -77,7 +77,10 @@ typedef struct {
                                  the derivative states of late blocks are carried backwards through the
                                  inverse blocks and all states meet after the block that minimises the
                                  sweep work; 1: every derivative state is carried forward to the end */
-    int32_t reserved[2];
+    int32_t no_fused_inject;  /* 0: a derivative-state chunk that is carried through passes is formed by its
+                                 first pass while loading (from the primal state, read through the caches),
+                                 never written or read as injected; 1: a separate injection kernel writes it */
+    int32_t reserved[1];
 } qgtl_tune_t;
 
 int qgtl_init(int device);

@@ -49,7 +49,8 @@ class Stats(ctypes.Structure):
 class Tune(ctypes.Structure):
     _fields_ = [("chunk_states", ctypes.c_int32), ("gram_chunks", ctypes.c_int32),
                 ("gram_variant", ctypes.c_int32), ("sweep_form", ctypes.c_int32), ("no_presplit", ctypes.c_int32),
-                ("sweep_schedule", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("sweep_schedule", ctypes.c_int32), ("no_fused_inject", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 _lib = None
@@ -142,10 +143,11 @@ class Plan:
     """A compiled circuit with device buffers, re-run with new angles."""
 
     def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0, sweep_form=0,
-                 presplit=True, two_sided=True):
+                 presplit=True, two_sided=True, fused_inject=True):
         self.ops = np.ascontiguousarray(ops, OP_DTYPE).copy()
         self.n = n
-        tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form, 0 if presplit else 1, 0 if two_sided else 1)
+        tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form, 0 if presplit else 1, 0 if two_sided else 1,
+                    0 if fused_inject else 1)
         h = lib().qgtl_plan_create(n, len(self.ops), _p(self.ops), mode, ctypes.byref(tune))
         if not h:
             raise QGTLError(-1, lib().qgtl_last_error().decode(errors="replace"))

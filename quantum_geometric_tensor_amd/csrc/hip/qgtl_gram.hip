@@ -447,6 +447,43 @@ __device__ __forceinline__ void qgt_finalize_elem(const double *__restrict__ C, 
     }
 }
 
+// The same with the overlaps read from the Gram: the primal state at the
+// states' meeting point is Gram state P (Z rows 2P, 2P+1), so
+// <d_k|psi> = G_kP, |d_k|^2 = G_kk and |psi|^2 = G_PP.
+__device__ __forceinline__ void qgt_finalize_psi_elem(const double *__restrict__ C, int R, int P,
+                                                      double *__restrict__ Q, size_t idx) {
+    const int k = (int)(idx / P), l = (int)(idx % P);
+    if (k > l) return;
+    const int m = P;   // the primal state
+    const double gre = C[(size_t)(2 * k) * R + 2 * l] + C[(size_t)(2 * k + 1) * R + 2 * l + 1];
+    const double gim = C[(size_t)(2 * k) * R + 2 * l + 1] - C[(size_t)(2 * k + 1) * R + 2 * l];
+    const double gkk = C[(size_t)(2 * k) * R + 2 * k] + C[(size_t)(2 * k + 1) * R + 2 * k + 1];
+    const double gll = C[(size_t)(2 * l) * R + 2 * l] + C[(size_t)(2 * l + 1) * R + 2 * l + 1];
+    const double gmm = C[(size_t)(2 * m) * R + 2 * m] + C[(size_t)(2 * m + 1) * R + 2 * m + 1];
+    const double akr0 = C[(size_t)(2 * k) * R + 2 * m] + C[(size_t)(2 * k + 1) * R + 2 * m + 1];
+    const double aki0 = C[(size_t)(2 * k) * R + 2 * m + 1] - C[(size_t)(2 * k + 1) * R + 2 * m];
+    const double alr0 = C[(size_t)(2 * l) * R + 2 * m] + C[(size_t)(2 * l + 1) * R + 2 * m + 1];
+    const double ali0 = C[(size_t)(2 * l) * R + 2 * m + 1] - C[(size_t)(2 * l + 1) * R + 2 * m];
+    const double sg = 0.25 / sqrt(gkk * gll);
+    const double sk = 0.5 / sqrt(gkk * gmm), sl = 0.5 / sqrt(gll * gmm);
+    const double akr = akr0 * sk, aki = aki0 * sk, alr = alr0 * sl, ali = ali0 * sl;
+    const double pr = akr * alr + aki * ali, pi = aki * alr - akr * ali;
+    const double qr = gre * sg - pr, qi = (k == l) ? 0.0 : gim * sg - pi;
+    Q[2 * ((size_t)k * P + l)] = qr;
+    Q[2 * ((size_t)k * P + l) + 1] = qi;
+    if (k != l) {
+        Q[2 * ((size_t)l * P + k)] = qr;
+        Q[2 * ((size_t)l * P + k) + 1] = -qi;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void qgt_finalize_psi_kernel(const double *__restrict__ C, int R, int P,
+                                                                    double *__restrict__ Q) {
+    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < (size_t)P * P;
+         idx += (size_t)gridDim.x * kThreads)
+        qgt_finalize_psi_elem(C, R, P, Q, idx);
+}
+
 __global__ __launch_bounds__(kThreads) void qgt_finalize_kernel(const double *__restrict__ C, int R,
                                                                 const double *__restrict__ inj, int P,
                                                                 double *__restrict__ Q) {
@@ -534,6 +571,16 @@ int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C) {
     unsigned grid = (unsigned)((total + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(gram_reduce_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), partials, nt, nchunks, C);
     QGTL_LAUNCH_CHECK("gram_reduce");
+    return 0;
+}
+
+int qgtl_hip_qgt_finalize_psi(const double *C, int P, double *Q) {
+    const int nt = (2 * (P + 1) + kTile - 1) / kTile;
+    const size_t work = (size_t)P * P;
+    unsigned grid = (unsigned)((work + kThreads - 1) / kThreads);
+    if (grid > 4096) grid = 4096;
+    hipLaunchKernelGGL(qgt_finalize_psi_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), C, nt * kTile, P, Q);
+    QGTL_LAUNCH_CHECK("qgt_finalize_psi");
     return 0;
 }
 

@@ -566,6 +566,25 @@ struct RegLoop {
         *reinterpret_cast<float4 *>(dst + global_index<R - 2>(g)) =
             make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
     }
+    // derivative state formed while loading: element y of the state is
+    // M[s][s] psi[y] + M[s][1-s] psi[y ^ v], s = parity(y & w), the same
+    // arithmetic as the injection kernel (psi is shared by every state of the
+    // launch and read through the caches)
+    __device__ __forceinline__ static void load_global_inject(const float2 *psi, const GlobalMap &g,
+                                                              const qgtl_inject_slot_t &q, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global_inject(psi, g, q, v);
+        const size_t a = global_index<R - 2>(g);
+        const float4 qa = *reinterpret_cast<const float4 *>(psi + a);
+        const float4 qb = *reinterpret_cast<const float4 *>(psi + ((a ^ (size_t)q.v) & ~(size_t)1));
+        const bool odd = (q.v & 1u) != 0;
+        const float2 b0 = odd ? make_float2(qb.z, qb.w) : make_float2(qb.x, qb.y);
+        const float2 b1 = odd ? make_float2(qb.x, qb.y) : make_float2(qb.z, qb.w);
+        const bool s0 = (__popc((uint32_t)a & q.w) & 1) != 0, s1 = s0 != ((q.w & 1u) != 0);
+        const float2 m00 = make_float2(q.M[0].re, q.M[0].im), m01 = make_float2(q.M[1].re, q.M[1].im);
+        const float2 m10 = make_float2(q.M[2].re, q.M[2].im), m11 = make_float2(q.M[3].re, q.M[3].im);
+        v[R - 2] = cmadd(cmadd(make_float2(0.f, 0.f), s0 ? m11 : m00, make_float2(qa.x, qa.y)), s0 ? m10 : m01, b0);
+        v[R - 1] = cmadd(cmadd(make_float2(0.f, 0.f), s1 ? m11 : m00, make_float2(qa.z, qa.w)), s1 ? m10 : m01, b1);
+    }
     // the same stores writing the split-plane form
     __device__ __forceinline__ static void store_global_split(uint32_t *planes, const GlobalMap &g,
                                                               const float2 (&v)[kRegs]) {
@@ -583,6 +602,8 @@ struct RegLoop<0> {
     template <bool POST>
     __device__ __forceinline__ static void phase_split(float2, const qgtl_pass_desc_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global(const float2 *, const GlobalMap &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_global_inject(const float2 *, const GlobalMap &,
+                                                              const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global(float2 *, const GlobalMap &, const float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global_split(uint32_t *, const GlobalMap &, const float2 (&)[kRegs]) {}
 };
@@ -725,16 +746,19 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         RegLoop<kRegs>::store_global(dst, m.gout, v);
 }
 
-// One tile r of one state.
+// One tile r of one state.  inj != nullptr: src is the primal state the
+// derivative state *inj is injected from while loading.
 template <bool SPLIT = false>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
-                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr) {
+                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr,
+                                               const qgtl_inject_slot_t *inj = nullptr) {
     pass_setup(d, hi_off, post_tab, pre_tab, gtabs);
     const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], threadIdx.x);
     const TileMaps m = tile_maps(d, r, L0, hi_off);
     float2 v[kRegs];
-    RegLoop<kRegs>::load_global(src, m.gin, v);
+    if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
+    else RegLoop<kRegs>::load_global(src, m.gin, v);
     pass_compute_store<SPLIT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
 }
 
@@ -750,10 +774,33 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
                           blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
 }
 
+// The first pass of a derivative-state chunk with the injection folded into
+// its load: state s of the launch is slot inj[s] applied to psi (one state,
+// shared), so the injected states are never written to or read from HBM.
+template <bool SPLIT>
+__global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pass_desc_t d,
+                                                                   const float2 *__restrict__ psi,
+                                                                   const qgtl_inject_slot_t *__restrict__ inj,
+                                                                   float2 *dst_all, size_t stride,
+                                                                   const float2 *__restrict__ gtabs) {
+    __shared__ float2 tile[kRegTileElems];
+    __shared__ uint32_t hi_off[1 << kHiTableBits];
+    __shared__ float2 tabs[2 * kPhaseTab];
+    // grid (states, tiles): the states of one tile are dispatched together and
+    // read the same primal tile (and partner tile) through the caches
+    const qgtl_inject_slot_t q = inj[blockIdx.x];
+    pass_regs_body<SPLIT>(d, psi, dst_all + (size_t)blockIdx.x * stride, blockIdx.y, tile, hi_off, tabs,
+                          tabs + kPhaseTab, gtabs, &q);
+}
+
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
 // FP64 per-tile sums of <d|psi>, |d|^2 and |psi|^2.
 constexpr int kInjectPerThread = 8;
 constexpr int kInjectTile = kThreads * kInjectPerThread;
+// single-plan injections: 32 amplitudes per thread, so the loads of a
+// workgroup are in flight together and its FP64 reduction is amortised
+constexpr int kInjectPerThreadPlan = 32;
+constexpr int kInjectTilePlan = kThreads * kInjectPerThreadPlan;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -763,7 +810,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Tile `tile` of derivative state k; its four sums go to out[0..3].  SPLIT:
 // dst receives the split-plane form (four fp16 planes of N values).
-template <bool SPLIT = false>
+template <bool SPLIT = false, bool STORE = true, int PER = kInjectPerThread, bool SUMS = true>
 __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const float2 *__restrict__ psi,
                                             float2 *dst, int k, uint32_t tile, double *out,
                                             double (*red)[4]) {
@@ -773,14 +820,16 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
     const float2 a10 = make_float2(d.A[k][2].re, d.A[k][2].im), a11 = make_float2(d.A[k][3].re, d.A[k][3].im);
     double sr = 0.0, si = 0.0, sd = 0.0, sp = 0.0;
 #pragma unroll
-    for (int i = 0; i < kInjectPerThread; i++) {
-        const uint32_t y = tile * kInjectTile + i * kThreads + threadIdx.x;
+    for (int i = 0; i < PER; i++) {
+        const uint32_t y = tile * (PER * kThreads) + i * kThreads + threadIdx.x;
         if (y < N) {
             const float2 a = psi[y], b = psi[y ^ v];
             const bool s = (__popc(y & w) & 1) != 0;
             const float2 mss = s ? a11 : a00, mso = s ? a10 : a01;
             const float2 o = cmadd(cmadd(make_float2(0.f, 0.f), mss, a), mso, b);
-            if constexpr (SPLIT) {
+            if constexpr (!STORE) {
+                // sums only: the state itself is formed by the first pass that reads it
+            } else if constexpr (SPLIT) {
                 uint32_t h, l;                        // (Re, Im) pieces of this amplitude
                 qgtl::split_f16_pair(o.x * qgtl::kSplitScale, o.y * qgtl::kSplitScale, h, l);
                 uint16_t *pl = reinterpret_cast<uint16_t *>(dst) + (size_t)(y >> 6) * 256 + (y & 63);
@@ -791,13 +840,16 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
             } else {
                 dst[y] = o;
             }
-            const double dr = o.x, di = o.y, pr = a.x, pi = a.y;
-            sr += dr * pr + di * pi;
-            si += dr * pi - di * pr;
-            sd += dr * dr + di * di;
-            sp += pr * pr + pi * pi;
+            if constexpr (SUMS) {
+                const double dr = o.x, di = o.y, pr = a.x, pi = a.y;
+                sr += dr * pr + di * pi;
+                si += dr * pi - di * pr;
+                sd += dr * dr + di * di;
+                sp += pr * pr + pi * pi;
+            }
         }
     }
+    if constexpr (!SUMS) return;
     sr = wave_sum(sr); si = wave_sum(si); sd = wave_sum(sd); sp = wave_sum(sp);
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
@@ -811,15 +863,25 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
     }
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool STORE = true, bool SUMS = true>
 __global__ __launch_bounds__(kThreads) void inject_c64_kernel(const qgtl_inject_desc_t d,
                                                               const float2 *__restrict__ psi,
                                                               float2 *dst_all, size_t stride,
                                                               double *partials) {
     __shared__ double red[kThreads / 64][4];
     const int k = blockIdx.y;
-    inject_body<SPLIT>(d, psi, dst_all + (size_t)k * stride, k, blockIdx.x,
-                       partials + ((size_t)k * gridDim.x + blockIdx.x) * 4, red);
+    inject_body<SPLIT, STORE, kInjectPerThreadPlan, SUMS>(
+        d, psi, STORE ? dst_all + (size_t)k * stride : nullptr, k, blockIdx.x,
+        SUMS ? partials + ((size_t)k * gridDim.x + blockIdx.x) * 4 : nullptr, red);
+}
+
+// One complex64 state in split-plane form (the Gram's operand form).
+__global__ __launch_bounds__(kThreads) void c64_to_split_kernel(const float2 *__restrict__ src, uint32_t *planes,
+                                                                size_t N) {
+    for (size_t g = 2 * ((size_t)blockIdx.x * kThreads + threadIdx.x); g < N; g += 2 * (size_t)gridDim.x * kThreads) {
+        const float4 q = *reinterpret_cast<const float4 *>(src + g);
+        write_split_pair(planes, g, make_float2(q.x, q.y), make_float2(q.z, q.w));
+    }
 }
 
 // out[c] = sum over t of pk[t * 4 + c] (fixed tree order, FP64).
@@ -1166,20 +1228,77 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
     return 0;
 }
 
+int qgtl_hip_c64_to_split(const void *src, void *dst, int n) {
+    const size_t N = (size_t)1 << n;
+    if (n < 6) return qgtl::set_error_msg(-118, "qgtl_hip_c64_to_split: needs at least 64 amplitudes");
+    size_t blocks = (N / 2 + kThreads - 1) / kThreads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(c64_to_split_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, qgtl::stream(),
+                       (const float2 *)src, (uint32_t *)dst, N);
+    QGTL_LAUNCH_CHECK("c64_to_split");
+    return 0;
+}
+
+int qgtl_hip_inject_sums_c64(const qgtl_inject_desc_t *desc, const void *psi, double *partials, int *tiles_out) {
+    if (desc->K < 1 || desc->K > QGTL_INJECT_MAX || desc->n < 1 || desc->n > QGTL_MAX_QUBITS)
+        return qgtl::set_error_msg(-118, "qgtl_hip_inject_sums_c64: bad descriptor");
+    const uint32_t N = 1u << desc->n;
+    const unsigned tiles = (N + kInjectTilePlan - 1) / kInjectTilePlan;
+    *tiles_out = (int)tiles;
+    hipLaunchKernelGGL((inject_c64_kernel<false, false>), dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0,
+                       qgtl::stream(), *desc, (const float2 *)psi, (float2 *)nullptr, (size_t)0, partials);
+    QGTL_LAUNCH_CHECK("inject_sums_c64");
+    return 0;
+}
+
+int qgtl_hip_pass_c64_inject(const qgtl_pass_desc_t *desc, const void *psi, const void *slots, void *dst,
+                             size_t stride, int nstates, int split_out, const void *tabs) {
+    if (desc->m != kRT || desc->nlow < 1 || desc->nlow > kRT || kRT - desc->nlow > kHiTableBits ||
+        desc->ngates > kRT || desc->m + desc->nrest != desc->n || nstates < 1 || !regs_pass(desc, 0) ||
+        stride < ((size_t)1 << desc->n))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: needs a register pass over separate states");
+    if (desc->nrest > 15 || nstates > 65535)
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: at most 2^15 tiles per state");
+    if (desc->store == QGTL_STORE_RING && (desc->nlow != desc->m || desc->n <= desc->m))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: ring store needs the low tile of a wider state");
+    if (desc->load != QGTL_LOAD_NATURAL &&
+        (desc->load != QGTL_LOAD_RING || desc->nlow != kRT || desc->n <= kRT || desc->store != QGTL_STORE_INPLACE ||
+         !tbits_ok(desc, desc->gather_tbits)))
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: bad ring gather");
+    const dim3 grid((unsigned)nstates, 1u << desc->nrest);
+    const float2 *gt = (const float2 *)tabs;
+    const qgtl_inject_slot_t *sl = (const qgtl_inject_slot_t *)slots;
+    if (split_out)
+        hipLaunchKernelGGL(pass_c64_inject_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, sl, (float2 *)dst, stride, gt);
+    else
+        hipLaunchKernelGGL(pass_c64_inject_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, sl, (float2 *)dst, stride, gt);
+    QGTL_LAUNCH_CHECK("pass_c64_inject");
+    return 0;
+}
+
 int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *dst, size_t stride,
                         double *partials, int *tiles_out, int split_out) {
     if (desc->K < 1 || desc->K > QGTL_INJECT_MAX || desc->n < 1 || desc->n > QGTL_MAX_QUBITS)
         return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: bad descriptor");
     const uint32_t N = 1u << desc->n;
     if (split_out && stride < N) return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: states overlap");
-    const unsigned tiles = (N + kInjectTile - 1) / kInjectTile;
+    const unsigned tiles = (N + kInjectTilePlan - 1) / kInjectTilePlan;
     *tiles_out = (int)tiles;
-    if (split_out)
-        hipLaunchKernelGGL(inject_c64_kernel<true>, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0,
-                           qgtl::stream(), *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
+    const dim3 grid(tiles, (unsigned)desc->K);
+    if (split_out && partials)
+        hipLaunchKernelGGL(inject_c64_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, (float2 *)dst, stride, partials);
+    else if (partials)
+        hipLaunchKernelGGL(inject_c64_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, (float2 *)dst, stride, partials);
+    else if (split_out)   /* no sums: the overlaps come from elsewhere */
+        hipLaunchKernelGGL((inject_c64_kernel<true, true, false>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, (float2 *)dst, stride, partials);
     else
-        hipLaunchKernelGGL(inject_c64_kernel<false>, dim3(tiles, (unsigned)desc->K), dim3(kThreads), 0,
-                           qgtl::stream(), *desc, (const float2 *)psi, (float2 *)dst, stride, partials);
+        hipLaunchKernelGGL((inject_c64_kernel<false, true, false>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+                           (const float2 *)psi, (float2 *)dst, stride, partials);
     QGTL_LAUNCH_CHECK("inject_c64");
     return 0;
 }

@@ -219,6 +219,8 @@ static void plan_free_device(struct qgtl_plan *p)
     qgtl_hip_free(p->tabs_dev);
     p->tabs_dev = NULL;
     p->tabs_passes = 0;
+    qgtl_hip_free(p->inj_slots_dev);
+    p->inj_slots_dev = NULL;
     qgtl_hip_free(p->inj_partials);
     qgtl_hip_free(p->inj_sums);
     qgtl_hip_free(p->gram_partials);
@@ -294,6 +296,7 @@ void qgtl_plan_destroy(qgtl_plan_t *p)
 {
     if (!p) return;
     plan_free_device(p);
+    free(p->inj_slots_host);
     qgtl_hip_event_destroy(p->ev_psi);
     qgtl_hip_event_destroy(p->ev_inj);
     free(p->tabs_host);
@@ -331,7 +334,7 @@ static int chunk_states(const struct qgtl_plan *p)
 static int gram_chunks(const struct qgtl_plan *p)
 {
     if (p->tune.gram_chunks > 0) return (p->tune.gram_chunks + 7) / 8 * 8;
-    const size_t T = (size_t)qgtl_hip_gram_tiles(p->P);
+    const size_t T = (size_t)qgtl_hip_gram_tiles(p->P + 1);
     size_t want = p->N / 2048;
     const size_t minw = (1024 + T - 1) / T;
     if (want < minw) want = minw;
@@ -360,7 +363,9 @@ static int alloc_mixed(struct qgtl_plan *p)
     HIPCALL(qgtl_hip_malloc(&p->psi[0], N * 8));
     HIPCALL(qgtl_hip_malloc(&p->psi[1], N * 8));
     if (P > 0) {
-        HIPCALL(qgtl_hip_malloc(&p->D, P * N * 8));
+        /* one slot more than P: depth-first plans place the primal state at the
+         * meeting point there, so the Gram also yields the overlaps and norms */
+        HIPCALL(qgtl_hip_malloc(&p->D, (P + 1) * N * 8));
         if (any_flip(p)) {
             p->tmp_states = (size_t)chunk_states(p);
             HIPCALL(qgtl_hip_malloc(&p->tmp, p->tmp_states * N * 8));
@@ -370,9 +375,9 @@ static int alloc_mixed(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_partials, p->inj_partials_len * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
         p->gram_chunks = gram_chunks(p);
-        const size_t T = (size_t)qgtl_hip_gram_tiles(p->P);
+        const size_t T = (size_t)qgtl_hip_gram_tiles(p->P + 1);
         HIPCALL(qgtl_hip_malloc((void **)&p->gram_partials, (size_t)p->gram_chunks * T * 128 * 128 * sizeof(float)));
-        const size_t nt = (2 * P + 127) / 128, R = nt * 128;
+        const size_t nt = (2 * (P + 1) + 127) / 128, R = nt * 128;
         HIPCALL(qgtl_hip_malloc((void **)&p->C, R * R * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
         HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
@@ -435,6 +440,29 @@ static int upload_tabs(struct qgtl_plan *p)
             qgtl_pass_phase_tables(&p->blocks[bi].ipass[i], p->tabs_host + per * (size_t)k);
     }
     HIPCALL(qgtl_hip_h2d(p->tabs_dev, p->tabs_host, sizeof(float) * per * (size_t)total));
+    if (p->P > 0) {
+        const size_t P = (size_t)p->P;
+        if (!p->inj_slots_dev) {
+            free(p->inj_slots_host);
+            p->inj_slots_host = (qgtl_inject_slot_t *)malloc(2 * P * sizeof(qgtl_inject_slot_t));
+            if (!p->inj_slots_host) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+            HIPCALL(qgtl_hip_malloc(&p->inj_slots_dev, 2 * P * sizeof(qgtl_inject_slot_t)));
+        }
+        for (size_t s = 0; s < P; s++) {
+            const qgtl_param_t *pp = &p->params[p->slot_param[s]];
+            qgtl_inject_slot_t *f = &p->inj_slots_host[s], *bk = &p->inj_slots_host[P + s];
+            f->v = pp->v;
+            f->w = pp->w;
+            bk->v = bk->w = 1u << pp->qubit;
+            for (int e = 0; e < 4; e++) {
+                f->M[e].re = (float)pp->A.m[e].re;
+                f->M[e].im = (float)pp->A.m[e].im;
+                bk->M[e].re = (float)pp->B.m[e].re;
+                bk->M[e].im = (float)pp->B.m[e].im;
+            }
+        }
+        HIPCALL(qgtl_hip_h2d(p->inj_slots_dev, p->inj_slots_host, 2 * P * sizeof(qgtl_inject_slot_t)));
+    }
     p->tabs_dirty = 0;
     return 0;
 }
@@ -444,7 +472,7 @@ static int upload_tabs(struct qgtl_plan *p)
  * two.  split_final: the last pass writes the split-plane form, always to
  * *alt (it cannot run in place). */
 static int run_passes(struct qgtl_plan *p, const qgtl_block_t *b, int inv, void **cur, void **alt, int nstates,
-                      qgtl_stats_t *st, int split_final)
+                      qgtl_stats_t *st, int split_final, const void *inj_psi, const void *inj_slots)
 {
     const qgtl_pass_desc_t *passes = inv ? b->ipass : b->pass;
     const int *flips = inv ? b->ipass_flips : b->pass_flips;
@@ -453,7 +481,11 @@ static int run_passes(struct qgtl_plan *p, const qgtl_block_t *b, int inv, void 
         const int split = split_final && i == np - 1;
         const int flip = flips[i] || split;
         void *dst = flip ? *alt : *cur;
-        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&passes[i], *cur, dst, p->N, nstates, 0, split, pass_tabs(p, b, i, inv)));
+        if (i == 0 && inj_psi)   /* the chunk's states are formed from inj_psi while this pass loads */
+            LAUNCH(K_SWEEP, qgtl_hip_pass_c64_inject(&passes[0], inj_psi, inj_slots, dst, p->N, nstates, split,
+                                                     pass_tabs(p, b, 0, inv)));
+        else
+            LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&passes[i], *cur, dst, p->N, nstates, 0, split, pass_tabs(p, b, i, inv)));
         if (st) {
             st->sweep_launches++;
             st->sweep_state_passes += nstates;
@@ -466,9 +498,9 @@ static int run_passes(struct qgtl_plan *p, const qgtl_block_t *b, int inv, void 
 
 /* Apply block b (its passes, then any per-gate M-stage kernels). */
 static int apply_block(struct qgtl_plan *p, const qgtl_block_t *b, void **cur, void **alt, int nstates,
-                       qgtl_stats_t *st, int split_final)
+                       qgtl_stats_t *st, int split_final, const void *inj_psi, const void *inj_slots)
 {
-    int rc = run_passes(p, b, 0, cur, alt, nstates, st, split_final);
+    int rc = run_passes(p, b, 0, cur, alt, nstates, st, split_final, inj_psi, inj_slots);
     if (rc) return rc;
     if (b->mstage_kernels)
         for (int i = 0; i < b->nperm; i++) {
@@ -567,7 +599,7 @@ static int presplit_ok(const struct qgtl_plan *p, int depth_first, int terms)
  * form when split): after the block from its output, or (backward) at the
  * block's input from the state before it. */
 static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *dst, qgtl_stats_t *st, int split,
-                  int backward)
+                  int backward, int store)
 {
     qgtl_inject_desc_t d;
     memset(&d, 0, sizeof d);
@@ -584,7 +616,15 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
         }
     }
     int tiles = 0;
-    LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles, split));
+    if (p->psi_in_gram) {
+        /* overlaps and norms come from the Gram: only the states are formed */
+        if (store) LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, NULL, &tiles, split));
+        return 0;
+    }
+    if (store)
+        LAUNCH(K_INJECT, qgtl_hip_inject_c64(&d, psi, dst, p->N, p->inj_partials, &tiles, split));
+    else   /* sums only: the chunk's first pass forms the states while loading */
+        LAUNCH(K_INJECT, qgtl_hip_inject_sums_c64(&d, psi, p->inj_partials, &tiles));
     LAUNCH(K_INJECT, qgtl_hip_reduce_partials_f64(p->inj_partials, cnt, tiles, p->inj_sums + 4 * (size_t)s0));
     return 0;
 }
@@ -603,6 +643,13 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
     else
         for (int b2 = bi - 1; b2 > p->meet; b2--) flips_after += block_flips(p, b2, 1);
     const int direct = backward ? bi == p->meet + 1 : bi == p->meet;   /* nothing to carry */
+    /* a carried chunk is formed by its first pass while loading, when that is a register pass */
+    int fuse = !direct && !p->tune.no_fused_inject && p->inj_slots_dev && p->n <= 28;
+    if (fuse) {
+        const qgtl_block_t *fb = &p->blocks[backward ? bi - 1 : bi + 1];
+        const int np = backward ? fb->ninv : fb->npasses;
+        fuse = np >= 1 && qgtl_hip_pass_split_ok(backward ? &fb->ipass[0] : &fb->pass[0]);
+    }
     const int nch = (K + (int)p->tmp_states - 1) / (int)p->tmp_states;
     const int C = (K + nch - 1) / nch;
     for (int j0 = 0; j0 < K; j0 += C) {
@@ -610,18 +657,27 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
         void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
         void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
         void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
-        int rc = inject(p, slot + j0, cnt, psi, c_cur, st, p->presplit && direct, backward);
+        int rc = inject(p, slot + j0, cnt, psi, c_cur, st, p->presplit && direct, backward, !fuse);
         if (rc) return rc;
-        if (lanes) HIPCALL(qgtl_hip_lane_signal(p->ev_inj));   /* psi may be overwritten after this */
+        const void *ipsi = fuse ? psi : NULL;
+        const void *islots = fuse ? (const char *)p->inj_slots_dev +
+                                        (size_t)((backward ? p->P : 0) + slot + j0) * sizeof(qgtl_inject_slot_t)
+                                  : NULL;
+        /* psi may be overwritten once its last reader of this chunk is issued */
+        if (lanes && !fuse) HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
         if (!backward) {
             for (int b2 = bi + 1; b2 <= p->meet; b2++) {
-                rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet);
+                rc = apply_block(p, &p->blocks[b2], &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet,
+                                 b2 == bi + 1 ? ipsi : NULL, islots);
                 if (rc) return rc;
+                if (lanes && fuse && b2 == bi + 1) HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
             }
         } else {
             for (int b2 = bi - 1; b2 > p->meet; b2--) {
-                rc = run_passes(p, &p->blocks[b2], 1, &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet + 1);
+                rc = run_passes(p, &p->blocks[b2], 1, &c_cur, &c_alt, cnt, st, p->presplit && b2 == p->meet + 1,
+                                b2 == bi - 1 ? ipsi : NULL, islots);
                 if (rc) return rc;
+                if (lanes && fuse && b2 == bi - 1) HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
             }
         }
         if (c_cur != dslot) return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: chunk ended off its slot");
@@ -661,20 +717,26 @@ static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
         }
         if (!depth_first && slot > 0) {
             void *dc = p->D, *da = NULL;
-            rc = apply_block(p, b, &dc, &da, slot, st, 0);
+            rc = apply_block(p, b, &dc, &da, slot, st, 0, NULL, NULL);
             if (rc) return rc;
         }
         if (lanes) {
             HIPCALL(qgtl_hip_set_lane(1));
             HIPCALL(qgtl_hip_lane_wait(p->ev_inj));
         }
-        rc = apply_block(p, b, &cur, &alt, 1, st, 0);
+        rc = apply_block(p, b, &cur, &alt, 1, st, 0, NULL, NULL);
         if (rc) return rc;
+        if (p->psi_in_gram && bi == p->meet) {
+            /* the primal state at the meeting point joins the Gram as state P */
+            void *slotP = (char *)p->D + (size_t)p->P * p->N * 8;
+            if (p->presplit) LAUNCH(K_OTHER, qgtl_hip_c64_to_split(cur, slotP, p->n));
+            else LAUNCH(K_OTHER, qgtl_hip_copy_f32x4(slotP, cur, p->N / 2));
+        }
         if (lanes) HIPCALL(qgtl_hip_lane_signal(p->ev_psi));
         if (K > 0 && !depth_first) {
             for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                 const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
-                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st, 0, 0);
+                rc = inject(p, slot + j0, cnt, cur, (char *)p->D + (size_t)(slot + j0) * p->N * 8, st, 0, 0, 1);
                 if (rc) return rc;
             }
         } else if (K > 0 && bi <= p->meet) {
@@ -705,6 +767,7 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     const int depth_first = any_flip(p);
     p->meet = choose_meet(p, depth_first);
     p->presplit = presplit_ok(p, depth_first, gram_terms(p));
+    p->psi_in_gram = depth_first && p->P > 0 && p->N >= 64;
     const int lanes = depth_first && !g_profiling;
     if (lanes && !p->ev_psi) {
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_psi));
@@ -714,10 +777,12 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     if (lanes) qgtl_hip_set_lane(0);
     if (rc) return rc;
     if (p->P == 0) return 0;
-    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, p->P, p->N, p->gram_partials, p->gram_chunks, gram_terms(p),
+    const int Pg = p->P + (p->psi_in_gram ? 1 : 0);
+    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, Pg, p->N, p->gram_partials, p->gram_chunks, gram_terms(p),
                                              p->presplit));
-    LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, p->P, p->gram_chunks, p->C));
-    LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
+    LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, Pg, p->gram_chunks, p->C));
+    if (p->psi_in_gram) LAUNCH(K_FINAL, qgtl_hip_qgt_finalize_psi(p->C, p->P, p->Qdev));
+    else LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
     if (st) {
         st->gram_launches += 2;
         st->gram_flops += 4.0 * (double)p->N * p->P * (p->P + 1);
@@ -733,7 +798,7 @@ int qgtl_carry_to_meet(struct qgtl_plan *p, void **state, void **scratch)
     qgtl_stats_t *st = NULL;
     for (int b = p->nblocks - 1; b > p->meet; b--) {
         if (!p->blocks[b].inv_ok) return qgtl_set_error(QGT_ERROR_INVALID_STATE, "block %d has no inverse", b);
-        int rc = run_passes(p, &p->blocks[b], 1, state, scratch, 1, st, 0);
+        int rc = run_passes(p, &p->blocks[b], 1, state, scratch, 1, st, 0, NULL, NULL);
         if (rc) return rc;
     }
     return 0;
@@ -924,7 +989,7 @@ int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *ps
         if (rc) { qgtl_hip_free(out); qgtl_plan_destroy(p); return hip_fail(rc, "statevector buffer"); }
         void *cur = p->psi[0], *alt = p->psi[1];
         rc = qgtl_hip_fill_zero_state_c64(cur, n);
-        for (int b = 0; !rc && b < p->nblocks; b++) rc = apply_block(p, &p->blocks[b], &cur, &alt, 1, st, 0);
+        for (int b = 0; !rc && b < p->nblocks; b++) rc = apply_block(p, &p->blocks[b], &cur, &alt, 1, st, 0, NULL, NULL);
         if (!rc) rc = qgtl_hip_c64_to_c128(cur, out, N);
     } else {
         rc = qgtl_hip_fill_zero_state_c128(out, n);
@@ -961,7 +1026,10 @@ int qgtl_debug_mfma_layout(float *host_out)
 int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int *R_out)
 {
     if (!p || p->mode != QGTL_MODE_MIXED || !p->C) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "no mixed-mode result");
-    const size_t R = (size_t)((2 * p->P + 127) / 128) * 128;
+    /* depth-first plans: C is the Gram of P + 1 states (the last one the primal
+     * state) and the injection sums are not computed */
+    const int Pg = p->P + (p->psi_in_gram ? 1 : 0);
+    const size_t R = (size_t)((2 * Pg + 127) / 128) * 128;
     if (R_out) *R_out = (int)R;
     if (C_out) HIPCALL(qgtl_hip_d2h(C_out, p->C, R * R * sizeof(double)));
     if (inj_out) HIPCALL(qgtl_hip_d2h(inj_out, p->inj_sums, (size_t)p->P * 4 * sizeof(double)));

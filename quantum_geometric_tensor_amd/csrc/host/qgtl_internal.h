@@ -98,6 +98,8 @@ struct qgtl_plan {
     void *psi_final;                     /* primal after the last execute (mixed) */
     qgtl_hip_event_t ev_psi, ev_inj;     /* ordering of the primal chain on the second stream */
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
+    int psi_in_gram;                     /* D slot P holds the primal state at the meeting point; the Gram of
+                                            P + 1 states gives the overlaps and norms (depth-first plans) */
     int meet;                            /* D holds the derivative states after block meet (two-sided schedule:
                                             later blocks' states are carried backwards); nblocks - 1: the end */
     /* rotation-form phase tables of every pass (block order), built on the host */
@@ -105,6 +107,10 @@ struct qgtl_plan {
     int tabs_passes;
     float *tabs_host;                    /* tabs_passes x 2 x QGTL_PHASE_TAB complex64 */
     void *tabs_dev;
+    /* per D slot: the injection as a load-time formula, forward (slots 0..P-1)
+     * and backward (P..2P-1), rebuilt with the tables */
+    qgtl_inject_slot_t *inj_slots_host;
+    void *inj_slots_dev;
     /* Hamiltonian for energy / gradient */
     int nterms;
     void *terms_dev;

@@ -216,6 +216,31 @@ def test_two_sided_gate_mix_inverse_passes(form):
     assert np.max(np.abs(g - g_ref)) <= 1e-6 * max(1.0, np.max(np.abs(g_ref)))
 
 
+@pytest.mark.parametrize("n,L,mix", [(14, 5, False), (16, 6, False), (15, 6, True)])
+def test_fused_injection_bit_identical(n, L, mix):
+    """Carried derivative-state chunks are formed by their first pass while it loads the primal
+    state (forward: the next block's high pass; backward: the gathering load of the inverse
+    block), with the same arithmetic as the injection kernel, whose sums-only variant still
+    provides <d|psi>, |d|^2, |psi|^2: Q, E and grad are bit-identical to injecting into memory
+    first, and Q matches the oracle."""
+    if mix:
+        gates = _mixed_circuit(n, L, 555)
+        ops, oops = Q.make_ops(gates), O.make_ops(gates)
+    else:
+        _, ops, oops = hea(n, L, seed=1700 + n)
+    out = {}
+    for fused in (True, False):
+        p = Q.Plan(n, ops, Q.MODE_MIXED, fused_inject=fused)
+        p.set_hamiltonian(Q.tfim_terms(n, 1.0, 0.8))
+        q, st = p.execute()
+        out[fused] = (q, *p.energy_gradient())
+        p.close()
+    assert np.array_equal(out[True][0], out[False][0])
+    assert out[True][1] == out[False][1]
+    assert np.array_equal(out[True][2], out[False][2])
+    assert Q.rel_err(out[True][0], O.qgt(n, oops)) <= MIXED_TOL
+
+
 def test_two_sided_meets_after_a_block_without_inverse():
     """A block whose M-stage is neither the CNOT ring nor empty (here one CNOT and one SWAP,
     applied by per-gate kernels) has no inverse passes: the meeting point moves to or past it,
