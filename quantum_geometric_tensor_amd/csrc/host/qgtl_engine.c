@@ -489,7 +489,10 @@ static int run_passes(struct qgtl_plan *p, const qgtl_block_t *b, int inv, void 
         if (st) {
             st->sweep_launches++;
             st->sweep_state_passes += nstates;
-            st->sweep_bytes += (double)nstates * (double)p->N * 16.0;
+            /* a pass that forms its states while loading writes 8 B per amplitude
+             * and reads the one primal state (twice: the element and its partner) */
+            if (i == 0 && inj_psi) st->sweep_bytes += (double)p->N * (8.0 * nstates + 16.0);
+            else st->sweep_bytes += (double)nstates * (double)p->N * 16.0;
         }
         if (flip) { void *t = *cur; *cur = *alt; *alt = t; }
     }
