@@ -903,7 +903,9 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
     rc = prof_collect(st);
     if (rc) return rc;
     if (Qslot) {
-        if (p->mode == QGTL_MODE_MIXED) {
+        int identity = 1;
+        for (size_t a = 0; a < P && identity; a++) identity = p->slot_param[a] == (int)a;
+        if (p->mode == QGTL_MODE_MIXED && !identity) {
             /* slot order (creation order) -> parameter numbers */
             for (size_t a = 0; a < P; a++)
                 for (size_t b = 0; b < P; b++) {
