@@ -460,6 +460,10 @@ static int upload_tabs(struct qgtl_plan *p)
                 bk->M[e].re = (float)pp->B.m[e].re;
                 bk->M[e].im = (float)pp->B.m[e].im;
             }
+            /* a diagonal M needs no partner: v = 0 makes the partner the element itself
+             * (the same line, and a zero coefficient) */
+            if (pp->A.m[1].re == 0.0 && pp->A.m[1].im == 0.0 && pp->A.m[2].re == 0.0 && pp->A.m[2].im == 0.0) f->v = 0;
+            if (pp->B.m[1].re == 0.0 && pp->B.m[1].im == 0.0 && pp->B.m[2].re == 0.0 && pp->B.m[2].im == 0.0) bk->v = 0;
         }
         HIPCALL(qgtl_hip_h2d(p->inj_slots_dev, p->inj_slots_host, 2 * P * sizeof(qgtl_inject_slot_t)));
     }
@@ -612,6 +616,8 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
         const qgtl_param_t *pp = &p->params[p->slot_param[s0 + j]];
         const qgtl_mat2 *M = backward ? &pp->B : &pp->A;
         d.v[j] = backward ? 1u << pp->qubit : pp->v;
+        if (M->m[1].re == 0.0 && M->m[1].im == 0.0 && M->m[2].re == 0.0 && M->m[2].im == 0.0)
+            d.v[j] = 0;   /* diagonal: no partner (the element's own line, zero coefficient) */
         d.w[j] = backward ? 1u << pp->qubit : pp->w;
         for (int e = 0; e < 4; e++) {
             d.A[j][e].re = (float)M->m[e].re;
