@@ -777,20 +777,35 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
 // The first pass of a derivative-state chunk with the injection folded into
 // its load: state s of the launch is slot inj[s] applied to psi (one state,
 // shared), so the injected states are never written to or read from HBM.
+constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dispatched to them round-robin
+
 template <bool SPLIT>
 __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pass_desc_t d,
                                                                    const float2 *__restrict__ psi,
                                                                    const qgtl_inject_slot_t *__restrict__ inj,
                                                                    float2 *dst_all, size_t stride,
-                                                                   const float2 *__restrict__ gtabs) {
+                                                                   const float2 *__restrict__ gtabs,
+                                                                   uint32_t nstates, uint32_t ntiles) {
     __shared__ float2 tile[kRegTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
-    // grid (states, tiles): the states of one tile are dispatched together and
-    // read the same primal tile (and partner tile) through the caches
-    const qgtl_inject_slot_t q = inj[blockIdx.x];
-    pass_regs_body<SPLIT>(d, psi, dst_all + (size_t)blockIdx.x * stride, blockIdx.y, tile, hi_off, tabs,
-                          tabs + kPhaseTab, gtabs, &q);
+    // one-dimensional grid of nstates x ntiles workgroups.  Workgroup b runs on
+    // XCD b % 8 (round-robin dispatch); with ntiles a multiple of 8, XCD x takes
+    // the tiles t = x (mod 8) in order, all states of a tile back to back, so
+    // each XCD's L2 fetches a primal tile (and its partner tile) once instead of
+    // every XCD fetching every tile
+    const uint32_t b = blockIdx.x;
+    uint32_t s, t;
+    if ((ntiles & (kXcd - 1)) == 0) {
+        const uint32_t slot = b / kXcd;
+        s = slot % nstates;
+        t = (slot / nstates) * kXcd + b % kXcd;
+    } else {
+        s = b % nstates;
+        t = b / nstates;
+    }
+    const qgtl_inject_slot_t q = inj[s];
+    pass_regs_body<SPLIT>(d, psi, dst_all + (size_t)s * stride, t, tile, hi_off, tabs, tabs + kPhaseTab, gtabs, &q);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -1259,21 +1274,24 @@ int qgtl_hip_pass_c64_inject(const qgtl_pass_desc_t *desc, const void *psi, cons
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: needs a register pass over separate states");
     if (desc->nrest > 15 || nstates > 65535)
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: at most 2^15 tiles per state");
+    if (((size_t)nstates << desc->nrest) * kThreads >= ((size_t)1 << 32))   // one-dimensional grid of work-items
+        return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: states x tiles must stay below 2^24");
     if (desc->store == QGTL_STORE_RING && (desc->nlow != desc->m || desc->n <= desc->m))
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: ring store needs the low tile of a wider state");
     if (desc->load != QGTL_LOAD_NATURAL &&
         (desc->load != QGTL_LOAD_RING || desc->nlow != kRT || desc->n <= kRT || desc->store != QGTL_STORE_INPLACE ||
          !tbits_ok(desc, desc->gather_tbits)))
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64_inject: bad ring gather");
-    const dim3 grid((unsigned)nstates, 1u << desc->nrest);
+    const uint32_t ntiles = 1u << desc->nrest;
+    const dim3 grid((unsigned)nstates * ntiles);   // <= 65535 * 2^15 workgroups
     const float2 *gt = (const float2 *)tabs;
     const qgtl_inject_slot_t *sl = (const qgtl_inject_slot_t *)slots;
     if (split_out)
         hipLaunchKernelGGL(pass_c64_inject_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)psi, sl, (float2 *)dst, stride, gt);
+                           (const float2 *)psi, sl, (float2 *)dst, stride, gt, (uint32_t)nstates, ntiles);
     else
         hipLaunchKernelGGL(pass_c64_inject_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)psi, sl, (float2 *)dst, stride, gt);
+                           (const float2 *)psi, sl, (float2 *)dst, stride, gt, (uint32_t)nstates, ntiles);
     QGTL_LAUNCH_CHECK("pass_c64_inject");
     return 0;
 }
