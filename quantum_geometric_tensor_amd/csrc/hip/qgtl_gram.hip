@@ -232,8 +232,11 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     // on a diagonal tile the quadrant below the diagonal is the transpose of
-    // the one above it and nothing reads it: its wave only loads and syncs
+    // the one above it and nothing reads it: its wave only loads and syncs;
+    // the waves on the diagonal skip their 32x32 block below it the same way
+    // (the only below-diagonal entries read, C[2k+1][2k], sit in diagonal blocks)
     const bool mirror = diag && wm > wn;
+    const bool lower_sub = diag && wm == wn;
 
     f32x16 acc[2][2], outer[2][2];
 #pragma unroll
@@ -313,6 +316,7 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
                 for (int a = 0; a < 2; a++)
 #pragma unroll
                     for (int b = 0; b < 2; b++) {
+                        if (a > b && lower_sub) continue;
                         // smallest terms first
                         if (PT == 6) {
                             acc[a][b] = Frag<F16>::mma(fa[1][a], fb[1][b], acc[a][b]);
