@@ -186,7 +186,10 @@ __device__ __forceinline__ void load_planes(const uint16_t *__restrict__ D16, in
     const uint16_t *blk = D16 + (k0 >> 6) * 256 + 64 * pl + 8 * c;
 #pragma unroll
     for (int i = 0; i < kPlaneLoads; i++) {
-        const int s = (s0 + 8 * i + sl < P) ? s0 + 8 * i + sl : P - 1;
+        int s = (s0 + 8 * i + sl < P) ? s0 + 8 * i + sl : P - 1;
+        // an opaque copy: the clamped address is formed at every stage instead of
+        // being hoisted out of the loop as eight 64-bit registers (which spilled)
+        __asm__ volatile("" : "+v"(s));
         v[i] = *reinterpret_cast<const u32x4 *>(blk + (size_t)s * 4 * N);
     }
 }
