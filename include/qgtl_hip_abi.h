@@ -218,6 +218,7 @@ int qgtl_hip_device_count(void);
 void *qgtl_hip_stream(void);                          /* the library's stream (default priority) */
 int qgtl_hip_malloc(void **ptr, size_t bytes);
 int qgtl_hip_free(void *ptr);
+int qgtl_hip_mem_free(size_t *free_bytes);            /* free device memory on the current device */
 int qgtl_hip_host_alloc(void **ptr, size_t bytes);     /* pinned host memory (hipHostMalloc) */
 int qgtl_hip_host_free(void *ptr);
 int qgtl_hip_h2d(void *dst, const void *src, size_t bytes);   /* async on the library stream */

@@ -83,6 +83,13 @@ int qgtl_hip_free(void *ptr) {
     return 0;
 }
 
+int qgtl_hip_mem_free(size_t *free_bytes) {
+    size_t total = 0;
+    *free_bytes = 0;
+    QGTL_HIP_CHECK(hipMemGetInfo(free_bytes, &total));
+    return 0;
+}
+
 int qgtl_hip_host_alloc(void **ptr, size_t bytes) {
     *ptr = nullptr;
     if (bytes == 0) return 0;

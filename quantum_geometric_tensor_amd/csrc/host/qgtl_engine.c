@@ -330,15 +330,23 @@ static int chunk_states(const struct qgtl_plan *p)
 /* Split-K chunks of ~2048 amplitudes, so that one chunk of up to 256 states
  * fits the 4 MB L2 of the XCD its tiles run on (measured at n = 20, P = 240:
  * 104 chunks 3.18 ms, 256 chunks 2.63 ms, 512 chunks 2.49 ms, 1024 chunks
- * 2.64 ms), at least ~1024 workgroups, and at most 1 GB of fp32 partials. */
-static int gram_chunks(const struct qgtl_plan *p)
+ * 2.64 ms), at least ~1024 workgroups, and fp32 partials of at most 1 GB or,
+ * for larger state sets, 1/16 of the states' bytes, within half of the free
+ * device memory (free_bytes, read after the states were allocated): longer
+ * chunks let the tile pairs that share rows drift apart, so their re-reads
+ * miss L2 (n = 26, P = 416: 584 chunks 243 ms, 4096 chunks 227 ms, 8192 chunks
+ * 215 ms). */
+static int gram_chunks(const struct qgtl_plan *p, size_t free_bytes)
 {
     if (p->tune.gram_chunks > 0) return (p->tune.gram_chunks + 7) / 8 * 8;
     const size_t T = (size_t)qgtl_hip_gram_tiles(p->P + 1);
     size_t want = p->N / 2048;
     const size_t minw = (1024 + T - 1) / T;
     if (want < minw) want = minw;
-    const size_t cap = ((size_t)1 << 30) / (T * 128 * 128 * sizeof(float));
+    size_t budget = ((size_t)p->P + 1) * p->N * 8 / 16;
+    if (budget > free_bytes / 2) budget = free_bytes / 2;
+    if (budget < ((size_t)1 << 30)) budget = (size_t)1 << 30;
+    const size_t cap = budget / (T * 128 * 128 * sizeof(float));
     if (want > cap) want = cap;
     const size_t maxc = p->N / 256;
     if (want > maxc) want = maxc;
@@ -374,7 +382,9 @@ static int alloc_mixed(struct qgtl_plan *p)
         p->inj_partials_len = (size_t)QGTL_INJECT_MAX * tiles * 4;
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_partials, p->inj_partials_len * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
-        p->gram_chunks = gram_chunks(p);
+        size_t free_bytes = 0;
+        HIPCALL(qgtl_hip_mem_free(&free_bytes));
+        p->gram_chunks = gram_chunks(p, free_bytes);
         const size_t T = (size_t)qgtl_hip_gram_tiles(p->P + 1);
         HIPCALL(qgtl_hip_malloc((void **)&p->gram_partials, (size_t)p->gram_chunks * T * 128 * 128 * sizeof(float)));
         const size_t nt = (2 * (P + 1) + 127) / 128, R = nt * 128;
