@@ -585,6 +585,18 @@ struct RegLoop {
         v[R - 2] = cmadd(cmadd(make_float2(0.f, 0.f), s0 ? m11 : m00, make_float2(qa.x, qa.y)), s0 ? m10 : m01, b0);
         v[R - 1] = cmadd(cmadd(make_float2(0.f, 0.f), s1 ? m11 : m00, make_float2(qa.z, qa.w)), s1 ? m10 : m01, b1);
     }
+    // the same for a diagonal matrix (q.v == 0): element y is M[s][s] psi[y], the
+    // partner term being an exact zero -- no partner load, one multiply-add
+    __device__ __forceinline__ static void load_global_inject_diag(const float2 *psi, const GlobalMap &g,
+                                                                   const qgtl_inject_slot_t &q, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global_inject_diag(psi, g, q, v);
+        const size_t a = global_index<R - 2>(g);
+        const float4 qa = *reinterpret_cast<const float4 *>(psi + a);
+        const bool s0 = (__popc((uint32_t)a & q.w) & 1) != 0, s1 = s0 != ((q.w & 1u) != 0);
+        const float2 m00 = make_float2(q.M[0].re, q.M[0].im), m11 = make_float2(q.M[3].re, q.M[3].im);
+        v[R - 2] = cmadd(make_float2(0.f, 0.f), s0 ? m11 : m00, make_float2(qa.x, qa.y));
+        v[R - 1] = cmadd(make_float2(0.f, 0.f), s1 ? m11 : m00, make_float2(qa.z, qa.w));
+    }
     // the same stores writing the split-plane form
     __device__ __forceinline__ static void store_global_split(uint32_t *planes, const GlobalMap &g,
                                                               const float2 (&v)[kRegs]) {
@@ -604,6 +616,8 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void load_global(const float2 *, const GlobalMap &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global_inject(const float2 *, const GlobalMap &,
                                                               const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_global_inject_diag(const float2 *, const GlobalMap &,
+                                                                   const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global(float2 *, const GlobalMap &, const float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void store_global_split(uint32_t *, const GlobalMap &, const float2 (&)[kRegs]) {}
 };
@@ -757,7 +771,8 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], threadIdx.x);
     const TileMaps m = tile_maps(d, r, L0, hi_off);
     float2 v[kRegs];
-    if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
+    if (inj && inj->v == 0) RegLoop<kRegs>::load_global_inject_diag(src, m.gin, *inj, v);
+    else if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
     else RegLoop<kRegs>::load_global(src, m.gin, v);
     pass_compute_store<SPLIT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
 }
@@ -833,6 +848,29 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
     const uint32_t v = d.v[k], w = d.w[k];
     const float2 a00 = make_float2(d.A[k][0].re, d.A[k][0].im), a01 = make_float2(d.A[k][1].re, d.A[k][1].im);
     const float2 a10 = make_float2(d.A[k][2].re, d.A[k][2].im), a11 = make_float2(d.A[k][3].re, d.A[k][3].im);
+    if constexpr (SPLIT && STORE && !SUMS) {
+        // two adjacent amplitudes per thread: one 16-byte load per operand and the
+        // four planes stored as 4-byte pairs (lanes cover 64-amplitude blocks), with
+        // the arithmetic of the element loop below
+        const bool odd = (v & 1u) != 0, wodd = (w & 1u) != 0;
+#pragma unroll
+        for (int i = 0; i < PER / 2; i++) {
+            const uint32_t y = tile * (PER * kThreads) + 2 * (i * kThreads + threadIdx.x);
+            if (y < N) {
+                const float4 qa = *reinterpret_cast<const float4 *>(psi + y);
+                const float4 qb = *reinterpret_cast<const float4 *>(psi + ((y ^ v) & ~1u));
+                const float2 b0 = odd ? make_float2(qb.z, qb.w) : make_float2(qb.x, qb.y);
+                const float2 b1 = odd ? make_float2(qb.x, qb.y) : make_float2(qb.z, qb.w);
+                const bool s0 = (__popc(y & w) & 1) != 0, s1 = s0 != wodd;
+                const float2 o0 = cmadd(cmadd(make_float2(0.f, 0.f), s0 ? a11 : a00, make_float2(qa.x, qa.y)),
+                                        s0 ? a10 : a01, b0);
+                const float2 o1 = cmadd(cmadd(make_float2(0.f, 0.f), s1 ? a11 : a00, make_float2(qa.z, qa.w)),
+                                        s1 ? a10 : a01, b1);
+                write_split_pair(reinterpret_cast<uint32_t *>(dst), y, o0, o1);
+            }
+        }
+        return;
+    }
     double sr = 0.0, si = 0.0, sd = 0.0, sp = 0.0;
 #pragma unroll
     for (int i = 0; i < PER; i++) {
@@ -1302,6 +1340,8 @@ int qgtl_hip_inject_c64(const qgtl_inject_desc_t *desc, const void *psi, void *d
         return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: bad descriptor");
     const uint32_t N = 1u << desc->n;
     if (split_out && stride < N) return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: states overlap");
+    if (split_out && desc->n < 6)   /* the planes are laid out in blocks of 64 amplitudes */
+        return qgtl::set_error_msg(-118, "qgtl_hip_inject_c64: split-plane output needs n >= 6");
     const unsigned tiles = (N + kInjectTilePlan - 1) / kInjectTilePlan;
     *tiles_out = (int)tiles;
     const dim3 grid(tiles, (unsigned)desc->K);
