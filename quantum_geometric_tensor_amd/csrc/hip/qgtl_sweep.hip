@@ -848,57 +848,42 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
     const uint32_t v = d.v[k], w = d.w[k];
     const float2 a00 = make_float2(d.A[k][0].re, d.A[k][0].im), a01 = make_float2(d.A[k][1].re, d.A[k][1].im);
     const float2 a10 = make_float2(d.A[k][2].re, d.A[k][2].im), a11 = make_float2(d.A[k][3].re, d.A[k][3].im);
-    if constexpr (SPLIT && STORE && !SUMS) {
-        // two adjacent amplitudes per thread: one 16-byte load per operand and the
-        // four planes stored as 4-byte pairs (lanes cover 64-amplitude blocks), with
-        // the arithmetic of the element loop below
-        const bool odd = (v & 1u) != 0, wodd = (w & 1u) != 0;
-#pragma unroll
-        for (int i = 0; i < PER / 2; i++) {
-            const uint32_t y = tile * (PER * kThreads) + 2 * (i * kThreads + threadIdx.x);
-            if (y < N) {
-                const float4 qa = *reinterpret_cast<const float4 *>(psi + y);
-                const float4 qb = *reinterpret_cast<const float4 *>(psi + ((y ^ v) & ~1u));
-                const float2 b0 = odd ? make_float2(qb.z, qb.w) : make_float2(qb.x, qb.y);
-                const float2 b1 = odd ? make_float2(qb.x, qb.y) : make_float2(qb.z, qb.w);
-                const bool s0 = (__popc(y & w) & 1) != 0, s1 = s0 != wodd;
-                const float2 o0 = cmadd(cmadd(make_float2(0.f, 0.f), s0 ? a11 : a00, make_float2(qa.x, qa.y)),
-                                        s0 ? a10 : a01, b0);
-                const float2 o1 = cmadd(cmadd(make_float2(0.f, 0.f), s1 ? a11 : a00, make_float2(qa.z, qa.w)),
-                                        s1 ? a10 : a01, b1);
-                write_split_pair(reinterpret_cast<uint32_t *>(dst), y, o0, o1);
-            }
-        }
-        return;
-    }
+    // two adjacent amplitudes per thread: one 16-byte load per operand, 16-byte
+    // (complex64) or 4-byte-pair (split-plane) stores; the FP64 sums run over the
+    // thread's amplitudes in a fixed order
+    static_assert(PER % 2 == 0, "amplitude pairs");
+    const bool odd = (v & 1u) != 0, wodd = (w & 1u) != 0;
     double sr = 0.0, si = 0.0, sd = 0.0, sp = 0.0;
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const uint32_t y = tile * (PER * kThreads) + i * kThreads + threadIdx.x;
-        if (y < N) {
-            const float2 a = psi[y], b = psi[y ^ v];
-            const bool s = (__popc(y & w) & 1) != 0;
-            const float2 mss = s ? a11 : a00, mso = s ? a10 : a01;
-            const float2 o = cmadd(cmadd(make_float2(0.f, 0.f), mss, a), mso, b);
+    for (int i = 0; i < PER / 2; i++) {
+        const uint32_t y = tile * (PER * kThreads) + 2 * (i * kThreads + threadIdx.x);
+        if (y < N) {                                          // N even: y + 1 < N too
+            const float4 qa = *reinterpret_cast<const float4 *>(psi + y);
+            const float4 qb = *reinterpret_cast<const float4 *>(psi + ((y ^ v) & ~1u));
+            const float2 a0 = make_float2(qa.x, qa.y), a1 = make_float2(qa.z, qa.w);
+            const float2 b0 = odd ? make_float2(qb.z, qb.w) : make_float2(qb.x, qb.y);
+            const float2 b1 = odd ? make_float2(qb.x, qb.y) : make_float2(qb.z, qb.w);
+            const bool s0 = (__popc(y & w) & 1) != 0, s1 = s0 != wodd;
+            const float2 o0 = cmadd(cmadd(make_float2(0.f, 0.f), s0 ? a11 : a00, a0), s0 ? a10 : a01, b0);
+            const float2 o1 = cmadd(cmadd(make_float2(0.f, 0.f), s1 ? a11 : a00, a1), s1 ? a10 : a01, b1);
             if constexpr (!STORE) {
                 // sums only: the state itself is formed by the first pass that reads it
             } else if constexpr (SPLIT) {
-                uint32_t h, l;                        // (Re, Im) pieces of this amplitude
-                qgtl::split_f16_pair(o.x * qgtl::kSplitScale, o.y * qgtl::kSplitScale, h, l);
-                uint16_t *pl = reinterpret_cast<uint16_t *>(dst) + (size_t)(y >> 6) * 256 + (y & 63);
-                pl[0] = (uint16_t)h;
-                pl[64] = (uint16_t)l;
-                pl[128] = (uint16_t)(h >> 16);
-                pl[192] = (uint16_t)(l >> 16);
+                write_split_pair(reinterpret_cast<uint32_t *>(dst), y, o0, o1);
             } else {
-                dst[y] = o;
+                *reinterpret_cast<float4 *>(dst + y) = make_float4(o0.x, o0.y, o1.x, o1.y);
             }
             if constexpr (SUMS) {
-                const double dr = o.x, di = o.y, pr = a.x, pi = a.y;
-                sr += dr * pr + di * pi;
-                si += dr * pi - di * pr;
-                sd += dr * dr + di * di;
-                sp += pr * pr + pi * pi;
+                const double dr0 = o0.x, di0 = o0.y, pr0 = a0.x, pi0 = a0.y;
+                const double dr1 = o1.x, di1 = o1.y, pr1 = a1.x, pi1 = a1.y;
+                sr += dr0 * pr0 + di0 * pi0;
+                si += dr0 * pi0 - di0 * pr0;
+                sd += dr0 * dr0 + di0 * di0;
+                sp += pr0 * pr0 + pi0 * pi0;
+                sr += dr1 * pr1 + di1 * pi1;
+                si += dr1 * pi1 - di1 * pr1;
+                sd += dr1 * dr1 + di1 * di1;
+                sp += pr1 * pr1 + pi1 * pi1;
             }
         }
     }
