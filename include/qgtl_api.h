@@ -136,6 +136,23 @@ qgt_error_t quantum_state_normalize(quantum_state_t *state);
 qgt_error_t quantum_state_fidelity(float *fidelity, const quantum_state_t *a, const quantum_state_t *b);
 /* :86 */
 bool quantum_state_is_normalized(const quantum_state_t *state);
+/* :93-94 (declared, never defined in QGTL): a complex64 copy in device memory for
+ * hardware.type == HARDWARE_TYPE_GPU (HARDWARE_TYPE_CPU moves it back; other types
+ * QGT_ERROR_NOT_IMPLEMENTED); while it is there the device copy is authoritative */
+qgt_error_t quantum_state_to_device(quantum_state_t *state, quantum_hardware_t hardware);
+/* :95-96: copies the device values into coordinates and frees the device copy */
+qgt_error_t quantum_state_from_device(quantum_state_t *state, quantum_hardware_t hardware);
+/* :97-98 */
+bool quantum_state_is_on_device(const quantum_state_t *state, quantum_hardware_t hardware);
+
+/* ---- files: include/quantum_geometric/core/quantum_geometric_tensor.h ---- */
+/* :144-145 (declared, never defined in QGTL): type, rank, dimensions, flags and the
+ * total_elements components in the library's checksummed container (magic QGTLTEN1) */
+qgt_error_t geometric_tensor_save(const quantum_geometric_tensor_t *tensor, const char *filename);
+/* :146-147: a new tensor allocated the way QGTL's geometric_tensor_create allocates one
+ * (aligned_alloc(64) struct and components, malloc'd dimensions), so QGTL's
+ * geometric_tensor_destroy frees it */
+qgt_error_t geometric_tensor_load(quantum_geometric_tensor_t **tensor, const char *filename);
 
 #ifdef __cplusplus
 }

@@ -354,6 +354,25 @@ int qgtl_hip_pc_transform(const float *y, const double *D, int P, int r, double 
 int qgtl_hip_reciprocal(const double *D, int P, double *d);
 int qgtl_hip_pinv_apply_c128(const void *V, const double *ev, const void *g, void *x, int P, double cutoff);
 
+/* ---- whole-vector operations on complex64 vectors of any length (state container, compute backend) ----
+ * partials: device scratch of qgtl_hip_vec_partials_len() doubles; out4: device, 4 doubles */
+int qgtl_hip_vec_partials_len(void);
+/* out4 = Re <a|b>, Im <a|b>, |a|^2, |b|^2 (FP64, fixed summation order) */
+int qgtl_hip_vec_dotc_c64(const void *a, const void *b, size_t n, double *partials, double *out4);
+/* out4[0] = sum |psi_i|^2 obs_i, out4[1] = |psi|^2 (obs: n real values) */
+int qgtl_hip_vec_expect_diag_c64(const void *psi, const float *obs, size_t n, double *partials, double *out4);
+int qgtl_hip_vec_scale_c64(void *x, size_t n, float s);
+/* y = U x, U row-major m x ncols complex64, FP64 accumulation (y must not alias x) */
+int qgtl_hip_vec_cgemv_c64(const void *U, const void *x, void *y, size_t m, size_t ncols);
+/* C = A B, row-major complex64 (A m x n, B n x k), FP32 accumulation */
+int qgtl_hip_vec_cgemm_c64(const void *A, const void *B, void *C, size_t m, size_t n, size_t k);
+/* the kind of memory behind a pointer: 1 device memory of this process, 0 otherwise */
+int qgtl_hip_is_device_ptr(const void *p);
+int qgtl_hip_memset(void *dst, int value, size_t bytes);
+/* a copy between any two of host, pinned and device memory (direction inferred), on the library stream */
+int qgtl_hip_copy(void *dst, const void *src, size_t bytes);
+int qgtl_hip_device_total_mem(size_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,11 @@ int qgtl_plan_set_angles(qgtl_plan_t *plan, int nops, const double *angles);
 int qgtl_plan_execute(qgtl_plan_t *plan, double *Q, qgtl_stats_t *stats);
 int qgtl_plan_num_params(const qgtl_plan_t *plan);
 void qgtl_plan_destroy(qgtl_plan_t *plan);
+/* The overlaps a_k = <d_k psi|psi> (2P doubles: Re, Im; parameter order) that the
+ * last mixed-mode execute subtracted in Q = G - a a^H (the quantity QGTL forms in
+ * quantum_geometric_tensor_network.c:1139-1166), normalised by the exact
+ * ||d_k psi|| = 1/2 and ||psi|| = 1 as the QGT is. */
+int qgtl_plan_overlaps(qgtl_plan_t *plan, double *a);
 
 /* Release every cached device buffer of the calling process. */
 void qgtl_release_cache(void);
@@ -197,6 +202,12 @@ int qgtl_plan_energy_gradient(qgtl_plan_t *plan, double *energy, double *grad);
  * network created by create_quantum_geometric_tensor_network. */
 struct quantum_geometric_tensor_network;
 int qgtl_network_qgt_f64(const struct quantum_geometric_tensor_network *qgtn, double *Q, int *P);
+
+/* The device copy (complex64, `dimension` values) of a quantum_state_t moved to
+ * the GPU with quantum_state_to_device, or NULL: for callers that write or read
+ * it with their own kernels or the compute backend's memcpy. */
+struct quantum_state_t;
+void *qgtl_state_device_ptr(const struct quantum_state_t *state);
 
 /* ---- natural-gradient solves (FP64 on the GPU) ----
  * (A + lambda I) x = b for `batch` real symmetric P x P systems stored one

@@ -191,6 +191,111 @@ typedef struct quantum_state_t {
     size_t confidence_size;
 } quantum_state_t;
 
+/* include/quantum_geometric/hardware/hardware_capabilities.h:28-54 */
+typedef struct {
+    size_t size;
+    size_t line_size;
+    size_t associativity;
+    bool inclusive;
+} CacheConfig;
+
+typedef struct {
+    uint32_t feature_flags;
+    size_t num_physical_cores;
+    size_t num_logical_cores;
+    size_t vector_register_width;
+    size_t max_vector_elements;
+    CacheConfig cache_config[3];
+    bool has_fma;
+    bool has_avx;
+    bool has_avx2;
+    bool has_avx512;
+    bool has_neon;
+    bool has_sve;
+    bool has_amx;
+    size_t page_size;
+    size_t huge_page_size;
+    bool supports_huge_pages;
+} SystemCapabilities;
+
+/* include/quantum_geometric/hardware/quantum_hardware_types.h:120-151 */
+typedef struct QuantumHardwareCapabilities {
+    bool supports_gpu;
+    bool supports_metal;
+    bool supports_cuda;
+    bool supports_openmp;
+    bool supports_mpi;
+    bool supports_distributed;
+    bool supports_feedback;
+    bool supports_reset;
+    bool supports_gates;
+    bool supports_measurement;
+    bool supports_annealing;
+    bool supports_conditional;
+    bool supports_parallel;
+    bool supports_error_correction;
+    uint32_t max_qubits;
+    uint32_t max_gates;
+    uint32_t max_depth;
+    uint32_t max_shots;
+    uint32_t max_parallel_jobs;
+    size_t memory_size;
+    double coherence_time;
+    double gate_time;
+    double readout_time;
+    double *connectivity;
+    size_t connectivity_size;
+    size_t num_gates;
+    char **available_gates;
+    void *extensions;
+    void *device_specific;
+    void *backend_specific;
+} QuantumHardwareCapabilities;
+
+/* include/quantum_geometric/hardware/quantum_hardware_types.h:264-277 (passed by value to
+ * quantum_state_to_device / _from_device / _is_on_device; this library reads `type`) */
+typedef struct {
+    HardwareType type;
+    uint32_t num_qubits;
+    uint32_t num_classical_bits;
+    SystemCapabilities sys_caps;
+    QuantumHardwareCapabilities caps;
+    char device_name[256];
+    void *device_data;
+    uint32_t device_id;
+    void *device_handle;
+    void *context;
+    void *command_queue;
+} quantum_hardware_t;
+
+/* include/quantum_geometric/core/quantum_geometric_types.h:232-250 */
+typedef enum {
+    GEOMETRIC_TENSOR_SCALAR, GEOMETRIC_TENSOR_VECTOR, GEOMETRIC_TENSOR_COVECTOR, GEOMETRIC_TENSOR_BIVECTOR,
+    GEOMETRIC_TENSOR_TRIVECTOR, GEOMETRIC_TENSOR_UNITARY, GEOMETRIC_TENSOR_HERMITIAN, GEOMETRIC_TENSOR_SYMMETRIC,
+    GEOMETRIC_TENSOR_CUSTOM
+} geometric_tensor_type_t;
+
+/* include/quantum_geometric/core/quantum_geometric_types.h:290-298 */
+typedef enum { QGT_MEM_STANDARD = 0, QGT_MEM_HUGE_PAGES, QGT_MEM_PINNED, QGT_MEM_UNIFIED } qgt_memory_type_t;
+
+/* include/quantum_geometric/core/quantum_geometric_types.h:331-353 */
+typedef struct quantum_geometric_tensor_t {
+    geometric_tensor_type_t type;
+    size_t *dimensions;
+    size_t rank;
+    size_t dimension;
+    ComplexFloat *components;
+    size_t num_spins;
+    void *auxiliary_data;
+    bool is_symmetric;
+    bool is_unitary;
+    bool is_hermitian;
+    HardwareType hardware;
+    qgt_memory_type_t mem_type;
+    size_t total_elements;
+    size_t aligned_elements;
+} quantum_geometric_tensor_t;
+
 /* include/quantum_geometric/core/quantum_geometric_types.h:371-395 */
 typedef enum {
     GEOMETRIC_METRIC_EUCLIDEAN, GEOMETRIC_METRIC_MINKOWSKI, GEOMETRIC_METRIC_FUBINI_STUDY,

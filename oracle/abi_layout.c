@@ -14,8 +14,12 @@
 #include "quantum_geometric/core/quantum_geometric_curvature.h"
 #include "quantum_geometric/core/quantum_geometric_gradient.h"
 #include "quantum_geometric/core/quantum_state.h"
+#include "quantum_geometric/core/quantum_geometric_types.h"
+#include "quantum_geometric/hardware/quantum_hardware_types.h"
+#include "quantum_geometric/supercomputer/compute_backend.h"
 #else
 #include "qgtl_api.h"
+#include "qgtl_compute_backend.h"
 #endif
 
 #define SZ(T) printf("sizeof %s %zu\n", #T, sizeof(T))
@@ -64,6 +68,42 @@ int main(void)
     OFF(natural_gradient_config_t, regularization_param); OFF(natural_gradient_config_t, condition_threshold);
     OFF(natural_gradient_config_t, use_adaptive_regularization);
     OFF(natural_gradient_config_t, use_pseudoinverse_fallback); OFF(natural_gradient_config_t, singular_value_cutoff);
+    SZ(CacheConfig); SZ(SystemCapabilities); SZ(QuantumHardwareCapabilities);
+    OFF(SystemCapabilities, cache_config); OFF(SystemCapabilities, has_fma); OFF(SystemCapabilities, page_size);
+    OFF(QuantumHardwareCapabilities, max_qubits); OFF(QuantumHardwareCapabilities, memory_size);
+    OFF(QuantumHardwareCapabilities, backend_specific);
+    SZ(quantum_hardware_t);
+    OFF(quantum_hardware_t, type); OFF(quantum_hardware_t, sys_caps); OFF(quantum_hardware_t, caps);
+    OFF(quantum_hardware_t, device_name); OFF(quantum_hardware_t, device_data); OFF(quantum_hardware_t, device_id);
+    OFF(quantum_hardware_t, command_queue);
+    SZ(quantum_geometric_tensor_t);
+    OFF(quantum_geometric_tensor_t, type); OFF(quantum_geometric_tensor_t, dimensions);
+    OFF(quantum_geometric_tensor_t, rank); OFF(quantum_geometric_tensor_t, dimension);
+    OFF(quantum_geometric_tensor_t, components); OFF(quantum_geometric_tensor_t, num_spins);
+    OFF(quantum_geometric_tensor_t, auxiliary_data); OFF(quantum_geometric_tensor_t, is_symmetric);
+    OFF(quantum_geometric_tensor_t, is_hermitian); OFF(quantum_geometric_tensor_t, hardware);
+    OFF(quantum_geometric_tensor_t, mem_type); OFF(quantum_geometric_tensor_t, total_elements);
+    OFF(quantum_geometric_tensor_t, aligned_elements);
+    SZ(ComputeBackendOps);
+    OFF(ComputeBackendOps, init); OFF(ComputeBackendOps, memcpy); OFF(ComputeBackendOps, wait_event);
+    OFF(ComputeBackendOps, quantum_unitary); OFF(ComputeBackendOps, quantum_normalize);
+    OFF(ComputeBackendOps, quantum_tensor_contract); OFF(ComputeBackendOps, quantum_gradient);
+    OFF(ComputeBackendOps, quantum_inner_product); OFF(ComputeBackendOps, quantum_expectation);
+    OFF(ComputeBackendOps, barrier); OFF(ComputeBackendOps, reduce_scatter); OFF(ComputeBackendOps, execute);
+    OFF(ComputeBackendOps, reset_metrics);
+    SZ(ComputeBackendInfo); OFF(ComputeBackendInfo, name); OFF(ComputeBackendInfo, priority); OFF(ComputeBackendInfo, ops);
+    SZ(ComputeQuantumOp); OFF(ComputeQuantumOp, output_data); OFF(ComputeQuantumOp, parameters);
+    OFF(ComputeQuantumOp, param_size); OFF(ComputeQuantumOp, dims); OFF(ComputeQuantumOp, num_dims);
+    OFF(ComputeQuantumOp, num_parameters);
+    SZ(ComputeExecutionPlan); OFF(ComputeExecutionPlan, workspace); OFF(ComputeExecutionPlan, requires_sync);
+    SZ(ComputeMetrics); OFF(ComputeMetrics, flops); OFF(ComputeMetrics, current_memory_bytes);
+    OFF(ComputeMetrics, error_rate);
+    SZ(ComputeDistributedConfig); OFF(ComputeDistributedConfig, preferred_backend);
+    OFF(ComputeDistributedConfig, backend_config);
+    VAL(COMPUTE_BACKEND_CUDA); VAL(COMPUTE_MEM_DEVICE); VAL(COMPUTE_MEM_PINNED); VAL(COMPUTE_DTYPE_COMPLEX128);
+    VAL(COMPUTE_ERROR_NOT_IMPLEMENTED); VAL(COMPUTE_ERROR_INTERNAL); VAL(QUANTUM_OP_EXPECTATION);
+    VAL(QUANTUM_OP_DENSITY_MATRIX); VAL(COMPUTE_REDUCE_AVG); VAL(GEOMETRIC_TENSOR_HERMITIAN); VAL(QGT_MEM_UNIFIED);
+    VAL(HARDWARE_TYPE_QPU);
     VAL(GATE_TYPE_X); VAL(GATE_TYPE_Y); VAL(GATE_TYPE_Z); VAL(GATE_TYPE_H); VAL(GATE_TYPE_S); VAL(GATE_TYPE_T);
     VAL(GATE_TYPE_RX); VAL(GATE_TYPE_RY); VAL(GATE_TYPE_RZ); VAL(GATE_TYPE_CNOT); VAL(GATE_TYPE_CY);
     VAL(GATE_TYPE_CZ); VAL(GATE_TYPE_SWAP); VAL(GATE_TYPE_U1); VAL(GATE_TYPE_PHASE); VAL(GATE_TYPE_SDG);

@@ -178,6 +178,15 @@ class Plan:
         L.qgtl_plan_set_hamiltonian.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _check(L.qgtl_plan_set_hamiltonian(self._h, len(terms), _p(terms)))
 
+    def overlaps(self):
+        """a_k = <d_k psi|psi> (complex128, parameter order) that the last mixed-mode
+        execute() subtracted in Q = G - a a^H."""
+        a = np.zeros(2 * self.P, np.float64)
+        L = lib()
+        L.qgtl_plan_overlaps.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _check(L.qgtl_plan_overlaps(self._h, _p(a)))
+        return a.view(np.complex128)
+
     def energy_gradient(self):
         """E and grad_k = 2 Re <d_k psi|H|psi> from the last execute()."""
         e = np.zeros(1, np.float64)

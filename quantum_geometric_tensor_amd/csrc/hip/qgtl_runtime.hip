@@ -206,6 +206,35 @@ void qgtl_hip_graph_destroy(qgtl_hip_graph_t graph) {
     delete graph;
 }
 
+int qgtl_hip_is_device_ptr(const void *p) {
+    if (!p) return 0;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // an unregistered host pointer: not an error of the caller
+        return 0;
+    }
+    return a.type == hipMemoryTypeDevice ? 1 : 0;
+}
+
+// Fills on the library stream with a kernel-free HIP call; used only outside
+// captured graphs (the compute-backend memset slot).
+int qgtl_hip_memset(void *dst, int value, size_t bytes) {
+    if (bytes) QGTL_HIP_CHECK(hipMemsetAsync(dst, value, bytes, g_stream));
+    return 0;
+}
+
+int qgtl_hip_copy(void *dst, const void *src, size_t bytes) {
+    if (bytes) QGTL_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, g_stream));
+    return 0;
+}
+
+int qgtl_hip_device_total_mem(size_t *bytes) {
+    size_t free_b = 0;
+    *bytes = 0;
+    QGTL_HIP_CHECK(hipMemGetInfo(&free_b, bytes));
+    return 0;
+}
+
 int qgtl_hip_cu_count(void) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;

@@ -78,10 +78,13 @@ quantum_gate_t *create_quantum_gate(gate_type_t type, const size_t *qubits, size
     g->type = type;
     g->num_qubits = num_qubits;
     g->target_qubits = (size_t *)malloc(num_qubits * sizeof(size_t));
+    /* the unified array of QGTL's circuit interface (quantum_types.h:181) holds the same qubits */
+    g->qubits = (size_t *)malloc(num_qubits * sizeof(size_t));
     const size_t dim = (size_t)1 << num_qubits;
     g->matrix = (ComplexFloat *)malloc(dim * dim * sizeof(ComplexFloat));
-    if (!g->target_qubits || !g->matrix) { destroy_quantum_gate(g); return NULL; }
+    if (!g->target_qubits || !g->qubits || !g->matrix) { destroy_quantum_gate(g); return NULL; }
     memcpy(g->target_qubits, qubits, num_qubits * sizeof(size_t));
+    memcpy(g->qubits, qubits, num_qubits * sizeof(size_t));
     g->is_controlled = (type == GATE_TYPE_CNOT || type == GATE_TYPE_CZ);
     if (rot) {
         g->is_parameterized = true;
@@ -104,9 +107,13 @@ quantum_gate_t *copy_quantum_gate(const quantum_gate_t *gate)
     g->parameters = NULL; g->matrix = NULL; g->custom_data = NULL;
     const size_t dim = (size_t)1 << gate->num_qubits;
     g->target_qubits = (size_t *)malloc((gate->num_qubits ? gate->num_qubits : 1) * sizeof(size_t));
+    g->qubits = (size_t *)malloc((gate->num_qubits ? gate->num_qubits : 1) * sizeof(size_t));
     g->matrix = (ComplexFloat *)malloc(dim * dim * sizeof(ComplexFloat));
-    if (!g->target_qubits || !g->matrix) { destroy_quantum_gate(g); return NULL; }
-    if (gate->target_qubits) memcpy(g->target_qubits, gate->target_qubits, gate->num_qubits * sizeof(size_t));
+    if (!g->target_qubits || !g->qubits || !g->matrix) { destroy_quantum_gate(g); return NULL; }
+    if (gate->target_qubits) {
+        memcpy(g->target_qubits, gate->target_qubits, gate->num_qubits * sizeof(size_t));
+        memcpy(g->qubits, gate->target_qubits, gate->num_qubits * sizeof(size_t));
+    }
     if (gate->matrix) memcpy(g->matrix, gate->matrix, dim * dim * sizeof(ComplexFloat));
     else fill_gate_matrix(g);
     if (gate->num_controls && gate->control_qubits) {
@@ -137,6 +144,7 @@ void destroy_quantum_gate(quantum_gate_t *gate)
     if (!gate) return;
     free(gate->matrix);
     free(gate->target_qubits);
+    free(gate->qubits);
     free(gate->control_qubits);
     free(gate->parameters);
     free(gate);
@@ -366,8 +374,56 @@ static int network_qgt(const quantum_geometric_tensor_network_t *q, const double
     return P;
 }
 
+/* One gate of a flat circuit built by QGTL's circuit API
+ * (src/quantum_geometric/core/quantum_circuit_operations.c:786-1135).  Those
+ * builders set type, num_qubits, the unified qubits[] array (control first for
+ * CNOT and CZ), parameters / num_parameters and custom_data only; the gate
+ * struct comes from malloc, so target_qubits, control_qubits, matrix and
+ * is_parameterized are indeterminate and are not read.  A rotation (RX, RY,
+ * RZ) with a parameter is a parameterised gate, as quantum_circuit_execute
+ * applies it (:1166-1177); quantum_circuit_phase's GATE_S carries an angle
+ * that quantum_circuit_execute ignores (it applies RZ(pi/2), :1162-1165), so it
+ * is S here; X, Y, Z, S differ from what quantum_circuit_execute applies
+ * (RX(pi), RY(pi), RZ(pi), RZ(pi/2)) only by a global phase. */
+static bool apply_flat_gate(quantum_geometric_tensor_network_t *q, const quantum_gate_t *gate)
+{
+    const size_t nq = gate->num_qubits;
+    if (nq < 1 || nq > 2 || !gate->qubits) {
+        set_qgtn_error("Invalid gate structure");
+        return false;
+    }
+    for (size_t t = 0; t < nq; t++)
+        if (gate->qubits[t] >= q->num_qubits) {
+            set_qgtn_error("qubit index %zu out of range", gate->qubits[t]);
+            return false;
+        }
+    const int rot = gate->type == GATE_TYPE_RX || gate->type == GATE_TYPE_RY || gate->type == GATE_TYPE_RZ;
+    const int two = gate->type == GATE_TYPE_CNOT || gate->type == GATE_TYPE_CZ || gate->type == GATE_TYPE_CY ||
+                    gate->type == GATE_TYPE_SWAP;
+    const int one = gate->type == GATE_TYPE_H || gate->type == GATE_TYPE_X || gate->type == GATE_TYPE_Y ||
+                    gate->type == GATE_TYPE_Z || gate->type == GATE_TYPE_S || gate->type == GATE_TYPE_T ||
+                    gate->type == GATE_TYPE_I || rot;
+    if ((two && nq != 2) || (one && nq != 1) || (!two && !one)) {
+        set_qgtn_error("gate type %d on %zu qubits is not supported in a flat circuit", (int)gate->type, nq);
+        return false;
+    }
+    if (rot && (gate->num_parameters < 1 || !gate->parameters)) {
+        set_qgtn_error("rotation gate without its angle");
+        return false;
+    }
+    quantum_gate_t *g = create_quantum_gate(gate->type, gate->qubits, nq, rot ? gate->parameters : NULL, rot ? 1 : 0);
+    if (!g) {
+        set_qgtn_error("Failed to create quantum gate");
+        return false;
+    }
+    const bool ok = apply_quantum_gate(q, g, g->target_qubits, nq);
+    destroy_quantum_gate(g);
+    return ok;
+}
+
 bool apply_quantum_circuit(quantum_geometric_tensor_network_t *q, const quantum_circuit_t *circuit)
 {
+    QGTL_LOCKED();
     if (!q || !circuit) {
         set_qgtn_error("Invalid arguments to apply_quantum_circuit");
         return false;
@@ -385,13 +441,12 @@ bool apply_quantum_circuit(quantum_geometric_tensor_network_t *q, const quantum_
             if (!apply_quantum_gate(q, gate, gate->target_qubits, gate->num_qubits)) return false;
         }
     }
-    /* also accept a flat gate list when the circuit has no layers */
+    /* a flat gate list (no layers), as QGTL's circuit builders make it */
     if (circuit->num_layers == 0)
         for (size_t g = 0; g < circuit->num_gates; g++) {
             const quantum_gate_t *gate = circuit->gates ? circuit->gates[g] : NULL;
             if (!gate) continue;
-            if (!gate->target_qubits) { set_qgtn_error("Invalid gate structure"); return false; }
-            if (!apply_quantum_gate(q, gate, gate->target_qubits, gate->num_qubits)) return false;
+            if (!apply_flat_gate(q, gate)) return false;
         }
     /* refresh the network's state (QGTL keeps it in circuit->state) */
     ComplexFloat *sv = NULL;
@@ -413,6 +468,7 @@ bool apply_quantum_circuit(quantum_geometric_tensor_network_t *q, const quantum_
 
 bool get_quantum_state(const quantum_geometric_tensor_network_t *q, ComplexFloat **state_vector, size_t *dimension)
 {
+    QGTL_LOCKED();
     if (!q || !state_vector || !dimension || !q->circuit) {
         set_qgtn_error("Invalid arguments to get_quantum_state");
         return false;
@@ -442,6 +498,7 @@ bool get_quantum_state(const quantum_geometric_tensor_network_t *q, ComplexFloat
 bool compute_quantum_geometric_tensor(const quantum_geometric_tensor_network_t *q, size_t i, size_t j,
                                       ComplexFloat *result)
 {
+    QGTL_LOCKED();
     if (!q || !result) {
         set_qgtn_error("Invalid arguments to compute_quantum_geometric_tensor");
         return false;
@@ -549,6 +606,7 @@ static int network_qgt_for(const quantum_geometric_tensor_network_t *q, size_t n
 qgt_error_t geometric_compute_fubini_study_metric(quantum_geometric_metric_t *metric,
                                                   const quantum_geometric_tensor_network_t *qgtn, size_t num_params)
 {
+    QGTL_LOCKED();
     CHECK_NULL(metric);
     CHECK_NULL(qgtn);
     CHECK_ARG(num_params > 0);
@@ -581,6 +639,7 @@ qgt_error_t geometric_compute_fubini_study_element(const quantum_geometric_tenso
 qgt_error_t geometric_compute_berry_curvature(quantum_geometric_curvature_t *curvature,
                                               const quantum_geometric_tensor_network_t *qgtn, size_t num_params)
 {
+    QGTL_LOCKED();
     CHECK_NULL(curvature);
     CHECK_NULL(qgtn);
     CHECK_ARG(num_params > 0);
@@ -629,6 +688,7 @@ qgt_error_t geometric_compose_qgt(ComplexFloat *qgt, const quantum_geometric_met
 qgt_error_t geometric_compute_full_qgt(ComplexFloat *qgt, const quantum_geometric_tensor_network_t *qgtn,
                                        size_t num_params)
 {
+    QGTL_LOCKED();
     CHECK_NULL(qgt);
     CHECK_NULL(qgtn);
     CHECK_ARG(num_params > 0);
@@ -647,6 +707,7 @@ qgt_error_t geometric_compute_full_qgt(ComplexFloat *qgt, const quantum_geometri
 /* FP64 view of the cached QGT (additive): copies P x P complex128. */
 int qgtl_network_qgt_f64(const quantum_geometric_tensor_network_t *qgtn, double *Q_out, int *P_out)
 {
+    QGTL_LOCKED();
     const double *Q = NULL;
     const int P = network_qgt(qgtn, &Q);
     if (P < 0) return P;

@@ -508,6 +508,7 @@ static int rebuild(struct qgtl_batch *B)
 /* ------------------------------------------------------------ public API */
 void qgtl_batch_destroy(qgtl_batch_t *B)
 {
+    QGTL_LOCKED();
     if (!B) return;
     qgtl_hip_graph_destroy(B->graph);
     for (int i = 0; i < B->count; i++) qgtl_plan_destroy(B->inst[i].plan);
@@ -543,6 +544,7 @@ static qgtl_batch_t *fail_create(struct qgtl_batch *B) { qgtl_batch_destroy(B); 
 qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const qgtl_op_t *ops,
                                 const qgtl_batch_tune_t *tune)
 {
+    QGTL_LOCKED();
     if (count < 1 || !n || !nops || !ops) {
         qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "empty batch");
         return NULL;
@@ -653,6 +655,7 @@ size_t qgtl_batch_device_bytes(const qgtl_batch_t *B) { return B ? B->device_byt
 
 int qgtl_batch_set_angles(qgtl_batch_t *B, const double *angles)
 {
+    QGTL_LOCKED();
     if (!B || !angles) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch or angles");
     /* the instances' plans are independent: rebuild them on the host cores */
     int first_bad = B->count;
@@ -689,6 +692,7 @@ static int apply_sweep_form(struct qgtl_batch *B)
 
 int qgtl_batch_set_tune(qgtl_batch_t *B, const qgtl_batch_tune_t *tune)
 {
+    QGTL_LOCKED();
     if (!B || !tune) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch or tune");
     B->tune = *tune;
     const int rc = apply_sweep_form(B);
@@ -730,6 +734,7 @@ static int run_device(struct qgtl_batch *B, double *device_ms)
 
 int qgtl_batch_execute(qgtl_batch_t *B, double *Q, qgtl_batch_stats_t *st)
 {
+    QGTL_LOCKED();
     if (!B) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch");
     const double t0 = now_ms();
     const int dirty = B->descs_dirty;
@@ -798,6 +803,7 @@ static int time_tune(struct qgtl_batch *B, const qgtl_batch_tune_t *t, int reps,
 
 int qgtl_batch_autotune(qgtl_batch_t *B, int reps, const char *json_path, qgtl_batch_tune_t *best_out)
 {
+    QGTL_LOCKED();
     if (!B) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch");
     const int c = B->cus > 0 ? B->cus : 256;
     qgtl_batch_tune_t best = B->tune;

@@ -18,6 +18,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
+#include <pthread.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -51,12 +52,34 @@ static int hip_fail(int rc, const char *where)
         if (_rc) return hip_fail(_rc, #call);                  \
     } while (0)
 
+/* ------------------------------------------------------------------ lock */
+static pthread_mutex_t g_lock;
+static pthread_once_t g_lock_once = PTHREAD_ONCE_INIT;
+
+static void lock_init(void)
+{
+    pthread_mutexattr_t a;
+    pthread_mutexattr_init(&a);
+    pthread_mutexattr_settype(&a, PTHREAD_MUTEX_RECURSIVE);
+    pthread_mutex_init(&g_lock, &a);
+    pthread_mutexattr_destroy(&a);
+}
+
+void qgtl_lock(void)
+{
+    pthread_once(&g_lock_once, lock_init);
+    pthread_mutex_lock(&g_lock);
+}
+
+void qgtl_unlock(void) { pthread_mutex_unlock(&g_lock); }
+
 /* ------------------------------------------------------------------ init */
 static int g_inited = 0;
 static int g_profiling = 0;
 
 int qgtl_init(int device)
 {
+    QGTL_LOCKED();
     int rc = qgtl_hip_init(device);
     if (rc) return hip_fail(rc, "qgtl_init");
     g_inited = 1;
@@ -72,7 +95,11 @@ static int ensure_init(void)
 
 int qgtl_device_count(void) { return qgtl_hip_device_count(); }
 
-void qgtl_set_profiling(int enable) { g_profiling = enable ? 1 : 0; }
+void qgtl_set_profiling(int enable)
+{
+    QGTL_LOCKED();
+    g_profiling = enable ? 1 : 0;
+}
 
 /* ------------------------------------------------------------- profiling */
 enum { K_SWEEP = 0, K_INJECT, K_GRAM_MAIN, K_GRAM_AUX, K_FINAL, K_OTHER, K_KINDS };
@@ -257,6 +284,7 @@ static void plan_free_device(struct qgtl_plan *p)
 
 qgtl_plan_t *qgtl_plan_create(int n, int nops, const qgtl_op_t *ops, int mode, const qgtl_tune_t *tune)
 {
+    QGTL_LOCKED();
     if (mode != QGTL_MODE_MIXED && mode != QGTL_MODE_FP64_PLAIN) {
         qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "unknown mode %d", mode);
         return NULL;
@@ -287,6 +315,7 @@ int qgtl_plan_num_params(const qgtl_plan_t *p) { return p ? p->P : -1; }
 
 int qgtl_plan_set_angles(qgtl_plan_t *p, int nops, const double *angles)
 {
+    QGTL_LOCKED();
     if (!p || !angles || nops != p->nops) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "angle count mismatch");
     for (int i = 0; i < nops; i++) p->ops[i].angle = angles[i];
     return qgtl_plan_build(p);
@@ -294,6 +323,7 @@ int qgtl_plan_set_angles(qgtl_plan_t *p, int nops, const double *angles)
 
 void qgtl_plan_destroy(qgtl_plan_t *p)
 {
+    QGTL_LOCKED();
     if (!p) return;
     plan_free_device(p);
     free(p->inj_slots_host);
@@ -892,6 +922,7 @@ static int run_plain(struct qgtl_plan *p, qgtl_stats_t *st)
 
 int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
 {
+    QGTL_LOCKED();
     if (!p) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null plan");
     int rc = ensure_init();
     if (rc) return rc;
@@ -978,6 +1009,7 @@ static qgtl_plan_t *cached_plan(int n, int nops, const qgtl_op_t *ops, int mode)
 
 void qgtl_release_cache(void)
 {
+    QGTL_LOCKED();
     for (int i = 0; i < PLAN_CACHE; i++) {
         qgtl_plan_destroy(g_cache[i]);
         g_cache[i] = NULL;
@@ -986,6 +1018,7 @@ void qgtl_release_cache(void)
 
 int qgtl_qgt(int n, int nops, const qgtl_op_t *ops, int mode, double *Q, qgtl_stats_t *stats)
 {
+    QGTL_LOCKED();
     int rc = ensure_init();
     if (rc) return rc;
     qgtl_plan_t *p = cached_plan(n, nops, ops, mode);
@@ -995,6 +1028,7 @@ int qgtl_qgt(int n, int nops, const qgtl_op_t *ops, int mode, double *Q, qgtl_st
 
 int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *psi_out)
 {
+    QGTL_LOCKED();
     int rc = ensure_init();
     if (rc) return rc;
     qgtl_plan_t *p = qgtl_plan_create(n, nops, ops, mode, NULL);
@@ -1030,6 +1064,7 @@ int qgtl_statevector(int n, int nops, const qgtl_op_t *ops, int mode, double *ps
 
 int qgtl_debug_mfma_layout(float *host_out)
 {
+    QGTL_LOCKED();
     int rc = ensure_init();
     if (rc) return rc;
     void *d = NULL;
@@ -1046,6 +1081,7 @@ int qgtl_debug_mfma_layout(float *host_out)
  * (P x 4: Re/Im <d|psi>, |d|^2, |psi|^2, slot order). */
 int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int *R_out)
 {
+    QGTL_LOCKED();
     if (!p || p->mode != QGTL_MODE_MIXED || !p->C) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "no mixed-mode result");
     /* depth-first plans: C is the Gram of P + 1 states (the last one the primal
      * state) and the injection sums are not computed */
@@ -1058,10 +1094,59 @@ int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int 
     return 0;
 }
 
+/* The overlaps a_k = <d_k psi|psi> (2P doubles: Re, Im, parameter order) that
+ * the last mixed-mode execute subtracted (Q_kl = G_kl - a_k conj(a_l)), with
+ * the same normalisation by ||d_k psi|| = 1/2 and ||psi|| = 1: from the FP64
+ * reduction of the Gram's row of the primal state on depth-first plans, from
+ * the FP64 injection sums otherwise. */
+int qgtl_plan_overlaps(qgtl_plan_t *p, double *a)
+{
+    QGTL_LOCKED();
+    if (!p || !a) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_plan_overlaps: null argument");
+    if (p->mode != QGTL_MODE_MIXED || !p->C || p->P < 1)
+        return qgtl_set_error(QGT_ERROR_INVALID_STATE, "qgtl_plan_overlaps: no mixed-mode result");
+    const size_t P = (size_t)p->P;
+    if (p->psi_in_gram) {
+        const size_t R = ((2 * (P + 1) + 127) / 128) * 128;
+        double *C = (double *)malloc(R * R * sizeof(double));
+        if (!C) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+        int rc = qgtl_hip_d2h(C, p->C, R * R * sizeof(double));
+        if (!rc) rc = qgtl_hip_sync();
+        if (rc) { free(C); return hip_fail(rc, "qgtl_plan_overlaps"); }
+        const size_t m = P;
+        const double gmm = C[(2 * m) * R + 2 * m] + C[(2 * m + 1) * R + 2 * m + 1];
+        for (size_t s = 0; s < P; s++) {
+            const double gkk = C[(2 * s) * R + 2 * s] + C[(2 * s + 1) * R + 2 * s + 1];
+            const double re = C[(2 * s) * R + 2 * m] + C[(2 * s + 1) * R + 2 * m + 1];
+            const double im = C[(2 * s) * R + 2 * m + 1] - C[(2 * s + 1) * R + 2 * m];
+            const double sc = 0.5 / sqrt(gkk * gmm);
+            const size_t k = (size_t)p->slot_param[s];
+            a[2 * k] = re * sc;
+            a[2 * k + 1] = im * sc;
+        }
+        free(C);
+        return 0;
+    }
+    double *inj = (double *)malloc(P * 4 * sizeof(double));
+    if (!inj) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+    int rc = qgtl_hip_d2h(inj, p->inj_sums, P * 4 * sizeof(double));
+    if (!rc) rc = qgtl_hip_sync();
+    if (rc) { free(inj); return hip_fail(rc, "qgtl_plan_overlaps"); }
+    for (size_t s = 0; s < P; s++) {
+        const double sc = 0.5 / sqrt(inj[4 * s + 2] * inj[4 * s + 3]);
+        const size_t k = (size_t)p->slot_param[s];
+        a[2 * k] = inj[4 * s] * sc;
+        a[2 * k + 1] = inj[4 * s + 1] * sc;
+    }
+    free(inj);
+    return 0;
+}
+
 /* Diagnostics: one plain copy of `bytes` (a multiple of 16) between two
  * fresh buffers, the known-size reference for HBM traffic counters. */
 int qgtl_debug_copy_kernel(size_t bytes, double *ms)
 {
+    QGTL_LOCKED();
     if (bytes == 0 || bytes % 16) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "copy size must be a multiple of 16");
     int rc = ensure_init();
     if (rc) return rc;
@@ -1087,4 +1172,8 @@ int qgtl_debug_copy_kernel(size_t bytes, double *ms)
     return 0;
 }
 
-int qgtl_init_if_needed(void) { return ensure_init(); }
+int qgtl_init_if_needed(void)
+{
+    QGTL_LOCKED();
+    return ensure_init();
+}

@@ -53,6 +53,7 @@ int qgtl_tfim_terms(int n, double J, double h, qgtl_pauli_term_t *terms)
 
 int qgtl_plan_set_hamiltonian(qgtl_plan_t *p, int nterms, const qgtl_pauli_term_t *terms)
 {
+    QGTL_LOCKED();
     if (!p || nterms < 1 || !terms) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "empty Hamiltonian");
     const uint32_t full = (p->n >= 32) ? 0xffffffffu : ((1u << p->n) - 1u);
     for (int t = 0; t < nterms; t++)
@@ -104,6 +105,7 @@ int qgtl_energy_gradient_device(qgtl_plan_t *p)
 
 int qgtl_plan_energy_gradient(qgtl_plan_t *p, double *energy, double *grad)
 {
+    QGTL_LOCKED();
     int rc0 = qgtl_energy_gradient_device(p);
     if (rc0) return rc0;
     const size_t P = (size_t)p->P;

@@ -134,6 +134,19 @@ uint64_t qgtl_structure_hash(int n, int nops, const qgtl_op_t *ops, int mode);
 /* errors (qgtl_engine.c) */
 int qgtl_set_error(int code, const char *fmt, ...);
 
+/* One process-wide recursive lock (qgtl_engine.c) held by every exported
+ * entry point that touches the device or state shared between calls: the
+ * library's streams and lane selector, the plan cache behind qgtl_qgt and the
+ * QGTL network calls, the solvers' scratch buffers and the profiling events.
+ * Calls from several host threads therefore run one after another on the GPU,
+ * each with its own inputs and outputs.  QGTL_LOCKED() at the top of a
+ * function holds the lock until the function returns. */
+void qgtl_lock(void);
+void qgtl_unlock(void);
+static inline void qgtl_unlock_scope(int *unused) { (void)unused; qgtl_unlock(); }
+#define QGTL_LOCKED() \
+    int qgtl_lock_guard_ __attribute__((cleanup(qgtl_unlock_scope), unused)) = (qgtl_lock(), 0)
+
 
 
 /* engine (qgtl_engine.c) */

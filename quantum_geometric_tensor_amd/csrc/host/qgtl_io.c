@@ -3,12 +3,13 @@
  *
  * QGTL declares geometric_tensor_save / geometric_tensor_load
  * (include/quantum_geometric/core/quantum_geometric_tensor.h:144-147) with no
- * implementation and no file format; these are the library's own formats for
- * what the QGT path produces, so that generated QGT datasets round-trip
- * (MLP checkpoints: qgtl_mlp_save / qgtl_mlp_load in qgtl_mlp.c, same container).
+ * implementation and no file format.  Both are exported here over the
+ * library's own container (kind "QGTLTEN1"), beside the formats for what the
+ * QGT path produces, so that generated QGT datasets round-trip (MLP
+ * checkpoints: qgtl_mlp_save / qgtl_mlp_load in qgtl_mlp.c, same container).
  *
  * Container (little-endian, every field fixed width):
- *   char     magic[8]        "QGTLQGT1" | "QGTLSET1" | "QGTLMLP1"
+ *   char     magic[8]        "QGTLQGT1" | "QGTLSET1" | "QGTLMLP1" | "QGTLTEN1"
  *   uint32   version         1
  *   uint32   header_bytes    size of the kind-specific header that follows
  *   uint64   payload_bytes
@@ -22,6 +23,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "qgtl_api.h"
 #include "qgtl_internal.h"
 
 #define QGTL_IO_VERSION 1u
@@ -235,4 +237,103 @@ int qgtl_dataset_load(const char *path, int S, int F, int P, float *X, float *As
     void *pieces[2] = {X, As};
     const size_t sizes[2] = {sizeof(float) * (size_t)S * (size_t)F, sizeof(float) * (size_t)S * (size_t)P * (size_t)P};
     return qgtl_io_read_payload(f, path, &h, sizeof h, pb, ck, 2, pieces, sizes);
+}
+
+/* ---- QGTL's generic tensor container (quantum_geometric_tensor.h:144-147) ----
+ * Kind header: type, rank, flags (bit 0 symmetric, 1 unitary, 2 hermitian),
+ * hardware, mem_type, dimension, num_spins, total_elements; payload: the rank
+ * dimensions (uint64) and total_elements components (ComplexFloat). */
+typedef struct {
+    int32_t type, flags, hardware, mem_type;
+    uint64_t rank, dimension, num_spins, total_elements;
+} tensor_hdr_t;
+
+#define QGTL_TENSOR_MAX_RANK 64
+
+qgt_error_t geometric_tensor_save(const quantum_geometric_tensor_t *t, const char *filename)
+{
+    if (!t || !filename || t->rank == 0 || t->rank > QGTL_TENSOR_MAX_RANK || !t->dimensions)
+        return (qgt_error_t)qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "geometric_tensor_save: bad arguments");
+    size_t total = 1;
+    for (size_t i = 0; i < t->rank; i++) {
+        if (t->dimensions[i] == 0 || total > ((size_t)1 << 40) / t->dimensions[i])
+            return (qgt_error_t)qgtl_set_error(QGT_ERROR_INVALID_DIMENSION, "geometric_tensor_save: bad dimensions");
+        total *= t->dimensions[i];
+    }
+    /* QGTL's geometric_tensor_create sets total_elements to the product of the dimensions */
+    if (t->total_elements != total || !t->components)
+        return (qgt_error_t)qgtl_set_error(QGT_ERROR_INVALID_STATE, "geometric_tensor_save: components do not match "
+                                           "the dimensions");
+    const tensor_hdr_t h = {(int32_t)t->type, (t->is_symmetric ? 1 : 0) | (t->is_unitary ? 2 : 0) |
+                                                  (t->is_hermitian ? 4 : 0),
+                            (int32_t)t->hardware, (int32_t)t->mem_type, t->rank, t->dimension, t->num_spins, total};
+    uint64_t dims[QGTL_TENSOR_MAX_RANK];
+    for (size_t i = 0; i < t->rank; i++) dims[i] = t->dimensions[i];
+    const void *pieces[2] = {dims, t->components};
+    const size_t sizes[2] = {sizeof(uint64_t) * t->rank, sizeof(ComplexFloat) * total};
+    const int rc = qgtl_io_write(filename, "QGTLTEN1", &h, sizeof h, 2, pieces, sizes);
+    return (qgt_error_t)rc;
+}
+
+qgt_error_t geometric_tensor_load(quantum_geometric_tensor_t **tensor, const char *filename)
+{
+    if (!tensor || !filename)
+        return (qgt_error_t)qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "geometric_tensor_load: bad arguments");
+    *tensor = NULL;
+    tensor_hdr_t h;
+    uint64_t pb = 0, ck = 0;
+    FILE *f = NULL;
+    int rc = qgtl_io_open(filename, "QGTLTEN1", &h, sizeof h, &pb, &ck, &f);
+    if (rc) return (qgt_error_t)rc;
+    if (h.rank == 0 || h.rank > QGTL_TENSOR_MAX_RANK || h.total_elements == 0 ||
+        h.total_elements > ((uint64_t)1 << 40) ||
+        pb != sizeof(uint64_t) * h.rank + sizeof(ComplexFloat) * h.total_elements) {
+        fclose(f);
+        return (qgt_error_t)qgtl_set_error(QGT_ERROR_VALIDATION_FAILED, "%s: inconsistent tensor header", filename);
+    }
+    const size_t total = (size_t)h.total_elements;
+    const size_t aligned = (total + 7) & ~(size_t)7;   /* QGTL's create pads to 8 elements */
+    quantum_geometric_tensor_t *t = (quantum_geometric_tensor_t *)aligned_alloc(64, (sizeof *t + 63) / 64 * 64);
+    uint64_t *dims64 = (uint64_t *)malloc(sizeof(uint64_t) * h.rank);
+    size_t *dims = (size_t *)malloc(sizeof(size_t) * h.rank);
+    ComplexFloat *comps = (ComplexFloat *)aligned_alloc(64, (aligned * sizeof(ComplexFloat) + 63) / 64 * 64);
+    if (!t || !dims64 || !dims || !comps) {
+        fclose(f);
+        free(t); free(dims64); free(dims); free(comps);
+        return (qgt_error_t)qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+    }
+    void *pieces[2] = {dims64, comps};
+    const size_t sizes[2] = {sizeof(uint64_t) * h.rank, sizeof(ComplexFloat) * total};
+    rc = qgtl_io_read_payload(f, filename, &h, sizeof h, pb, ck, 2, pieces, sizes);
+    size_t prod = 1;
+    for (uint64_t i = 0; !rc && i < h.rank; i++) {
+        dims[i] = (size_t)dims64[i];
+        if (dims[i] == 0 || prod > total / dims[i]) rc = QGT_ERROR_VALIDATION_FAILED;
+        else prod *= dims[i];
+    }
+    if (!rc && prod != total) rc = QGT_ERROR_VALIDATION_FAILED;
+    free(dims64);
+    if (rc) {
+        free(t); free(dims); free(comps);
+        if (rc == QGT_ERROR_VALIDATION_FAILED) qgtl_set_error(rc, "%s: dimensions do not match the components", filename);
+        return (qgt_error_t)rc;
+    }
+    for (size_t i = total; i < aligned; i++) comps[i] = (ComplexFloat){0.f, 0.f};
+    memset(t, 0, sizeof *t);
+    t->type = (geometric_tensor_type_t)h.type;
+    t->dimensions = dims;
+    t->rank = (size_t)h.rank;
+    t->dimension = (size_t)h.dimension;
+    t->components = comps;
+    t->num_spins = (size_t)h.num_spins;
+    t->auxiliary_data = NULL;
+    t->is_symmetric = (h.flags & 1) != 0;
+    t->is_unitary = (h.flags & 2) != 0;
+    t->is_hermitian = (h.flags & 4) != 0;
+    t->hardware = (HardwareType)h.hardware;
+    t->mem_type = (qgt_memory_type_t)h.mem_type;
+    t->total_elements = total;
+    t->aligned_elements = aligned;
+    *tensor = t;
+    return QGT_SUCCESS;
 }

@@ -99,6 +99,7 @@ static int ensure_batch(struct qgtl_mlp *m, int B)
 
 void qgtl_mlp_destroy(qgtl_mlp_t *m)
 {
+    QGTL_LOCKED();
     if (!m) return;
     free_batch(m);
     qgtl_hip_free(m->w);
@@ -113,6 +114,7 @@ void qgtl_mlp_destroy(qgtl_mlp_t *m)
 
 qgtl_mlp_t *qgtl_mlp_create(int in_dim, int hidden, int hidden_layers, int P, int rank, uint64_t seed)
 {
+    QGTL_LOCKED();
     if (in_dim < 1 || hidden < 1 || hidden_layers < 1 || hidden_layers >= MLP_MAX_LAYERS || P < 1 || P > 1024 ||
         rank < 0 || rank > 16 || P * rank > 4096) {
         qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "mlp: bad shape (hidden layers 1..7, P <= 1024, rank <= 16)");
@@ -168,6 +170,7 @@ void qgtl_mlp_shape(const qgtl_mlp_t *m, int *in_dim, int *P, int *rank)
 
 int qgtl_mlp_set_params(qgtl_mlp_t *m, const float *params)
 {
+    QGTL_LOCKED();
     if (!m || !params) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null mlp or params");
     MTRY(qgtl_hip_h2d(m->w, params, m->nparams * sizeof(float)));
     MTRY(qgtl_hip_fill_f32(m->m, m->nparams, 0.f));
@@ -180,6 +183,7 @@ int qgtl_mlp_set_params(qgtl_mlp_t *m, const float *params)
 
 int qgtl_mlp_get_params(const qgtl_mlp_t *m, float *params)
 {
+    QGTL_LOCKED();
     if (!m || !params) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null mlp or params");
     MTRY(qgtl_hip_d2h(params, m->w, m->nparams * sizeof(float)));
     MTRY(qgtl_hip_sync());
@@ -188,6 +192,7 @@ int qgtl_mlp_get_params(const qgtl_mlp_t *m, float *params)
 
 int qgtl_mlp_set_data(qgtl_mlp_t *m, int S, const float *X, const float *As)
 {
+    QGTL_LOCKED();
     if (!m || S < 1 || !X || !As) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "empty training set");
     qgtl_hip_free(m->X_all);
     qgtl_hip_free(m->As_all);
@@ -261,6 +266,7 @@ static int loss_grad_dev(struct qgtl_mlp *m, int B, const int *idx, double *loss
 
 int qgtl_mlp_loss_grad(qgtl_mlp_t *m, int B, const int *idx, double *loss, float *grad)
 {
+    QGTL_LOCKED();
     if (!m) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null mlp");
     int rc = loss_grad_dev(m, B, idx, loss);
     if (rc) return rc;
@@ -273,6 +279,7 @@ int qgtl_mlp_loss_grad(qgtl_mlp_t *m, int B, const int *idx, double *loss, float
 
 int qgtl_mlp_train_step(qgtl_mlp_t *m, int B, const int *idx, float lr, double *loss)
 {
+    QGTL_LOCKED();
     if (!m) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null mlp");
     int rc = loss_grad_dev(m, B, idx, loss);
     if (rc) return rc;
@@ -300,6 +307,7 @@ int qgtl_mlp_forward_device(qgtl_mlp_t *m, const float **y_dev)
 
 int qgtl_mlp_forward(qgtl_mlp_t *m, int B, const float *X, float *Y)
 {
+    QGTL_LOCKED();
     if (!m || B < 1 || !X || !Y) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "bad forward arguments");
     int rc = ensure_batch(m, B);
     if (rc) return rc;
@@ -327,6 +335,7 @@ int qgtl_pc_features(int P, const double *theta, const double *D, double field, 
 
 int qgtl_mlp_preconditioner(qgtl_mlp_t *m, const float *x, const double *D, double *d, double *U)
 {
+    QGTL_LOCKED();
     if (!m || !x || !D || !d) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "bad preconditioner arguments");
     float *y = (float *)malloc(sizeof(float) * (size_t)m->O);
     if (!y) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
@@ -352,6 +361,7 @@ typedef struct {
 
 int qgtl_mlp_save(const qgtl_mlp_t *m, const char *path, int with_optimizer)
 {
+    QGTL_LOCKED();
     if (!m || !path) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_mlp_save: null argument");
     const size_t nb = m->nparams * sizeof(float);
     const int k = with_optimizer ? 3 : 1;
@@ -374,6 +384,7 @@ int qgtl_mlp_save(const qgtl_mlp_t *m, const char *path, int with_optimizer)
 
 qgtl_mlp_t *qgtl_mlp_load(const char *path)
 {
+    QGTL_LOCKED();
     mlp_hdr_t hd;
     uint64_t pb = 0, ck = 0;
     FILE *f = NULL;

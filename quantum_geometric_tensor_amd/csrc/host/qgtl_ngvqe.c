@@ -108,6 +108,7 @@ out:
 int qgtl_ng_step(qgtl_plan_t *p, double *theta, const qgtl_ng_config_t *cfg, qgtl_mlp_t *mlp, double field,
                  qgtl_ng_info_t *info)
 {
+    QGTL_LOCKED();
     if (!p || !theta || !cfg) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_ng_step: null argument");
     if (p->mode != QGTL_MODE_MIXED) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_ng_step: mixed-mode plan");
     if (p->nterms < 1) return qgtl_set_error(QGT_ERROR_INVALID_STATE, "qgtl_ng_step: no Hamiltonian set");

@@ -42,6 +42,7 @@ static int scratch(int slot, size_t bytes, void **out)
 int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double lambda, int method, double tol,
                    int maxit, const double *d, const double *U, int rank, double *x, qgtl_solve_info_t *info)
 {
+    QGTL_LOCKED();
     if (P < 1 || P > 1024 || batch < 1 || !A || !b || !x) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64: bad arguments");
     if (method == QGTL_SOLVE_PCG && (rank < 0 || rank > 64 || (rank > 0 && !U)))
         return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64: low-rank factor needs 0 <= rank <= 64");
@@ -126,6 +127,7 @@ int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double la
 /* Hermitian eigen-decomposition on the GPU (complex128 in, ascending). */
 int qgtl_hermitian_eig(int P, const double *A, double *evals, double *V)
 {
+    QGTL_LOCKED();
     if (P < 1 || P > 1024 || !A || !evals) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_hermitian_eig: bad arguments");
     TRY(qgtl_init_if_needed());
     const size_t PP = (size_t)P * P;
@@ -197,6 +199,7 @@ static int singular_values(const double *M, size_t P, double *sv, int *hermitian
 
 bool compute_complex_matrix_condition_number(const ComplexFloat *matrix, size_t dimension, float *condition_number)
 {
+    QGTL_LOCKED();
     if (!matrix || !condition_number || dimension == 0 || dimension > 1024) return false;
     if (qgtl_init_if_needed()) return false;
     double *M = to_c128(matrix, dimension * dimension);
@@ -254,6 +257,7 @@ static int pinv_solve(const double *M, const double *g, double *x, size_t P, dou
 bool compute_pseudoinverse_natural_gradient(const ComplexFloat *gradient, const ComplexFloat *metric,
                                             ComplexFloat *natural_gradient, size_t dimension, float singular_cutoff)
 {
+    QGTL_LOCKED();
     if (!gradient || !metric || !natural_gradient || dimension == 0 || dimension > 1024) return false;
     if (qgtl_init_if_needed()) return false;
     double *M = to_c128(metric, dimension * dimension), *g = to_c128(gradient, dimension);
@@ -269,6 +273,7 @@ bool compute_regularized_natural_gradient(const ComplexFloat *gradient, const Co
                                           ComplexFloat *natural_gradient, size_t dimension,
                                           const natural_gradient_config_t *config)
 {
+    QGTL_LOCKED();
     if (!gradient || !metric || !natural_gradient || !config || dimension == 0 || dimension > 1024) return false;
     if (qgtl_init_if_needed()) return false;
     double lambda = config->regularization_param;

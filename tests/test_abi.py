@@ -36,11 +36,11 @@ def declared_functions(path):
     return names
 
 
-@pytest.mark.parametrize("header", ["qgtl_api.h", "qgtl_mi355x.h", "qgtl_hip_abi.h"])
+@pytest.mark.parametrize("header", ["qgtl_api.h", "qgtl_mi355x.h", "qgtl_hip_abi.h", "qgtl_compute_backend.h"])
 def test_all_declared_symbols_exported(header):
     lib = Q.lib()
     names = declared_functions(os.path.join(INCLUDE, header))
-    assert len(names) > 5
+    assert len(names) >= 2
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, f"{header}: not exported: {missing}"
 
@@ -54,6 +54,15 @@ def test_abi_layout_matches_qgtl_headers():
     b = subprocess.run([ours], stdout=subprocess.PIPE, text=True, check=True).stdout
     assert a.count("\n") > 80
     assert a == b
+
+
+def test_compute_backend_registers_with_qgtl_registry():
+    """Loading the library registers the HIP backend through QGTL's compute_register_backend
+    (here a stand-in defined by the test program), once, with every vtable slot filled."""
+    prog = os.path.join(ROOT, "tests", "c", "_build", "test_backend_register")
+    res = subprocess.run([prog], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout
+    assert "HIP" in res.stdout
 
 
 def test_plan_validation_errors():

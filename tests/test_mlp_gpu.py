@@ -4,8 +4,10 @@ the FP64 CPU oracle (oracle/mlp_oracle.c).
 The GPU rounds weights and every GEMM operand (activations, back-propagated
 gradients) to bf16 and accumulates in fp32; the oracle gets the same
 bf16-rounded weights and inputs and computes everything else in FP64.  Bars:
-forward output normwise 2e-2, loss 1e-2 relative, parameter gradient
-normwise 5e-2 with cosine similarity >= 0.998, one Adam step to 1e-6.
+forward output normwise 6e-3, loss 1e-2 relative, parameter gradient
+normwise 1e-2 with cosine similarity >= 0.9999, one Adam step to 1e-6 (the bars
+are about 3x the measured bf16 errors: forward 2.0e-3, gradient 3.3e-3,
+cosine 0.999997).
 """
 import numpy as np
 import pytest
@@ -60,7 +62,7 @@ def test_mlp_forward_matches_oracle():
     Yo = O.mlp_forward(m.in_dim, m.hidden, m.hidden_layers, m.out_dim, oracle_params(m), bf16(X))
     err = np.max(np.abs(Y - Yo)) / np.max(np.abs(Yo))
     print(f"forward normwise err {err:.2e}")
-    assert err <= 2e-2
+    assert err <= 6e-3
     m.close()
 
 
@@ -79,7 +81,7 @@ def test_mlp_loss_and_gradient_match_oracle():
     cos = float(np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)))
     print(f"loss {loss:.6f} oracle {lo:.6f}; grad normwise err {gerr:.2e} cos {cos:.6f}")
     assert abs(loss - lo) <= 1e-2 * max(1.0, abs(lo))
-    assert gerr <= 5e-2 and cos >= 0.998
+    assert gerr <= 1e-2 and cos >= 0.9999
     m.close()
 
 

@@ -279,21 +279,26 @@ def test_lds_pass_kernel_on_register_plans(n):
 
 @pytest.mark.parametrize("method", [Q.SOLVE_CHOLESKY, Q.SOLVE_JACOBI_PCG, Q.SOLVE_CG])
 def test_ng_step_device_resident(method):
-    """One natural-gradient step on the device equals theta - eta (Re Q + lambda I)^-1 grad E
-    assembled on the host from the same plan (QGT, energy and gradient on the GPU), and E
-    matches the oracle's TFIM energy."""
+    """One natural-gradient step on the device against the FP64 oracle's step: the oracle's
+    Q, energy and gradient, A = Re Q + lambda I and its Cholesky solve on the host
+    (update_natural_gradient, classical_optimization_engine.c:390-416).  The GPU's Q
+    carries the mixed-precision error (<= 1e-6 normwise), which the solve amplifies by at
+    most cond(A) <= (1/4 + lambda) / lambda: the step is held to 1e-6 cond(A) relative."""
     n, L, J, h, lam, eta = 6, 3, 1.0, 0.9, 1e-3, 0.05
     th, ops, oops = hea(n, L, seed=321)
+    e_ref, g_ref = O.tfim_gradient(n, oops, J, h)
+    A = O.qgt(n, oops).real + lam * np.eye(len(th))
+    x_ref = O.cholesky_solve(A, g_ref)
     p = Q.Plan(n, ops, Q.MODE_MIXED)
     p.set_hamiltonian(Q.tfim_terms(n, J, h))
-    Qm, _ = p.execute()
-    e_ref, g = p.energy_gradient()
-    A = Qm.real + lam * np.eye(len(th))
-    expect = th - eta * np.linalg.solve(A, g)
     theta = th.copy()
     info = Q.ng_step(p, theta, lam, eta, method, 1e-11, 20000, None, h)
     p.close()
-    assert abs(info["energy"] - O.tfim_gradient(n, oops, J, h)[0]) <= 1e-6
-    tol = 1e-9 if method == Q.SOLVE_CHOLESKY else 1e-6   # iterative: residual 1e-11 times the condition number
-    assert np.max(np.abs(theta - expect)) <= tol * np.max(np.abs(expect - th))
     assert info["solve_status"] == 0
+    assert abs(info["energy"] - e_ref) <= 1e-6
+    assert abs(info["grad_norm"] - np.linalg.norm(g_ref)) <= 1e-6 * np.linalg.norm(g_ref)
+    x = (th - theta) / eta
+    cond = np.linalg.cond(A)
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    print(f"method {method}: step rel err {err:.2e} (cond(A) = {cond:.0f})")
+    assert err <= 1e-6 * cond
