@@ -120,6 +120,71 @@ def large_leg(Q, reps=2):
             "gram_tflops": st["gram_flops"] / (st["gram_kernel_ms"] * 1e-3) / 1e12 if st["gram_kernel_ms"] else None}
 
 
+def _cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def cpu_leg(Q, n, L, theta, Qmix, full=False):
+    """The FP64 C oracle as the CPU baseline, in the reference's loop structure:
+    derivative states materialised once (exact, complex128), then the per-element
+    loop over mu <= nu with three length-2^n dot products per element
+    (quantum_geometric_curvature.c:300-317 over quantum_geometric_tensor_network.c:1127-1166)
+    and the P x P loop (differential_geometry.c:2832-2858).  The n=20 loops are timed on a
+    bounded sample of rows (1 warm-up + 5 repetitions, median) and scaled by element
+    count; the n=10 instance is timed whole (median of 5).  full=True adds the
+    single-thread n=20 derivative states (minutes) and the n=20 P x P loop."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = _cpu_threads()
+    P = 2 * n * L
+    total_el = {"elementwise": P * (P + 1) // 2, "pairwise": P * P}
+    oops = O.hea_ops(n, L, theta)
+    psi, D, deriv_all = O.derivative_states_timed(n, oops, threads)
+
+    def sampled(variant, nthreads, rows, reps=6):
+        t, el = O.qgt_loop_time(psi, D, variant, nthreads, rows, reps)
+        med = float(np.median(t[1:]))
+        return med, el, med * total_el[variant] / el
+
+    rows = min(P, threads)
+    med, el, loop_all = sampled("elementwise", threads, rows)
+    per_qgt = deriv_all + loop_all
+    out = {"value": 1.0 / per_qgt, "unit": "QGT/s", "cores": threads, "kind": "port",
+           "sample": (f"n={n} L={L} P={P}, FP64 C oracle on {threads} threads: derivative states once "
+                      f"({deriv_all:.2f} s) + the per-element loop mu<=nu with 3 dot products per element, rows "
+                      f"0..{rows - 1} ({el} of {total_el['elementwise']} elements) {med:.2f} s (median of 5) scaled "
+                      f"by element count: {per_qgt:.1f} s per QGT"),
+           "seconds_per_qgt": per_qgt, "derivative_states_s": deriv_all, "loop_s_scaled": loop_all}
+    med1, el1, loop1 = sampled("elementwise", 1, 1, reps=4)
+    single = {"threads": 1, "loop_s_scaled": loop1, "sample": f"row 0 ({el1} elements) {med1:.2f} s, median of 3"}
+    if full:
+        _, _, d1 = O.derivative_states_timed(n, oops, 1, psi, D)
+        single["derivative_states_s"] = d1
+        single["seconds_per_qgt"] = d1 + loop1
+        medp, elp, loopp = sampled("pairwise", threads, rows)
+        out["pairwise_all_threads"] = {"seconds_per_qgt": deriv_all + loopp, "loop_s_scaled": loopp,
+                                       "sample": f"rows 0..{rows - 1} ({elp} elements) {medp:.2f} s, median of 5"}
+    out["elementwise_single_thread"] = single
+    del D
+    # the n=10, L=4 instance whole: both loop forms, one thread and all threads
+    n1, L1 = 10, 4
+    ops1 = O.hea_ops(n1, L1, O.uniform_angles(0, 2 * n1 * L1))
+    P1 = 2 * n1 * L1
+    small = {}
+    for nt in (1, threads):
+        ds = []
+        for _ in range(6):
+            psi1, D1, d = O.derivative_states_timed(n1, ops1, nt)
+            ds.append(d)
+        for variant in ("elementwise", "pairwise"):
+            t, _ = O.qgt_loop_time(psi1, D1, variant, nt, P1, 6)
+            small[f"{variant}_{nt}_threads_s"] = float(np.median(ds[1:])) + float(np.median(t[1:]))
+    out["n10_L4_whole"] = small
+    # accuracy of the mixed GPU result against the oracle's full matrix (hoisted form, same values)
+    Qcpu = O.qgt_hoisted(n, oops)
+    return out, Q.rel_err(Qmix, Qcpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +192,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline with the single-thread n=20 derivative states and the n=20 P x P loop (minutes)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="time only the mixed-precision path (for profiler runs): no FP64 legs")
     ap.add_argument("--no-batched", action="store_true", help="skip the 512-instance batched leg")
@@ -210,17 +277,7 @@ def main():
     cpu = None
     err_cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib as O
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        oops = O.hea_ops(n, L, theta)
-        c0 = time.perf_counter()
-        Qcpu = O.qgt_hoisted(n, oops)
-        c1 = time.perf_counter()
-        err_cpu = Q.rel_err(Qmix, Qcpu)
-        cpu = {"value": 1.0 / (c1 - c0), "unit": "QGT/s", "cores": threads, "kind": "port",
-               "sample": f"one full n={n} L={L} P={P} QGT by the FP64 C oracle (derivative states once, "
-                         f"hoisted overlaps, upper triangle), OpenMP {threads} threads, {c1 - c0:.1f} s"}
+        cpu, err_cpu = cpu_leg(Q, n, L, theta, Qmix, full=args.cpu_full)
 
     line = {
         "metric": "QGT/s at n=20,P=240: mixed-precision vs FP64 plain on GPU, rel err <=1e-6",

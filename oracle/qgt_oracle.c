@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -295,11 +296,13 @@ static void dotc(size_t dim, const double *a, const double *b, double *re, doubl
     *im = si;
 }
 
-/* differential_geometry.c:2832-2858: every (i, j), three sums per element. */
-void oracle_qgt_pairwise(size_t dim, int P, const double *psi, const double *D, double *Q)
+/* differential_geometry.c:2832-2858: every (i, j), three sums per element;
+ * rows [r0, r1) of the loop on `threads` OpenMP threads (0: the default). */
+static void qgt_pairwise_rows(size_t dim, int P, const double *psi, const double *D, double *Q, int r0, int r1,
+                              int threads)
 {
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < P; i++) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int i = r0; i < r1; i++) {
         const double *di = D + 2 * dim * (size_t)i;
         for (int j = 0; j < P; j++) {
             const double *dj = D + 2 * dim * (size_t)j;
@@ -313,12 +316,18 @@ void oracle_qgt_pairwise(size_t dim, int P, const double *psi, const double *D, 
     }
 }
 
-/* quantum_geometric_tensor_network.c:1127-1175 per element (mu <= nu), with the
- * Hermitian fill of quantum_geometric_curvature.c:300-317. */
-void oracle_qgt_elementwise(size_t dim, int P, const double *psi, const double *D, double *Q)
+void oracle_qgt_pairwise(size_t dim, int P, const double *psi, const double *D, double *Q)
 {
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int mu = 0; mu < P; mu++) {
+    qgt_pairwise_rows(dim, P, psi, D, Q, 0, P, 0);
+}
+
+/* quantum_geometric_tensor_network.c:1127-1175 per element (mu <= nu), with the
+ * Hermitian fill of quantum_geometric_curvature.c:300-317; rows [r0, r1). */
+static void qgt_elementwise_rows(size_t dim, int P, const double *psi, const double *D, double *Q, int r0, int r1,
+                                 int threads)
+{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : omp_get_max_threads())
+    for (int mu = r0; mu < r1; mu++) {
         const double *dm = D + 2 * dim * (size_t)mu;
         for (int nu = mu; nu < P; nu++) {
             const double *dn = D + 2 * dim * (size_t)nu;
@@ -337,6 +346,11 @@ void oracle_qgt_elementwise(size_t dim, int P, const double *psi, const double *
             }
         }
     }
+}
+
+void oracle_qgt_elementwise(size_t dim, int P, const double *psi, const double *D, double *Q)
+{
+    qgt_elementwise_rows(dim, P, psi, D, Q, 0, P, 0);
 }
 
 /* Same value as the elementwise form with the P projections hoisted. */
@@ -879,4 +893,43 @@ int oracle_qgt_hoisted(int n, int nops, const oracle_op_t *ops, double *Q)
     }
     free(psi); free(D); free(a);
     return rc;
+}
+
+/* ----------------------------------------------- CPU baseline timing */
+static double wall_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int oracle_derivative_states_timed(int n, int nops, const oracle_op_t *ops, int threads, double *psi, double *D,
+                                   double *seconds)
+{
+    const int saved = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+    const double t0 = wall_s();
+    const int rc = oracle_derivative_states(n, nops, ops, psi, D);
+    *seconds = wall_s() - t0;
+    omp_set_num_threads(saved);
+    return rc;
+}
+
+int oracle_qgt_loop_time(size_t dim, int P, const double *psi, const double *D, int variant, int threads, int rows,
+                         int reps, double *sample_s, long *elements)
+{
+    if (P < 1 || rows < 1 || rows > P || reps < 1 || (variant != 0 && variant != 1)) return -1;
+    double *Q = (double *)malloc(2 * sizeof(double) * (size_t)P * (size_t)P);
+    if (!Q) return -1;
+    long el = 0;
+    for (int r = 0; r < rows; r++) el += variant == 0 ? (long)(P - r) : (long)P;
+    *elements = el;
+    for (int k = 0; k < reps; k++) {
+        const double t0 = wall_s();
+        if (variant == 0) qgt_elementwise_rows(dim, P, psi, D, Q, 0, rows, threads);
+        else qgt_pairwise_rows(dim, P, psi, D, Q, 0, rows, threads);
+        sample_s[k] = wall_s() - t0;
+    }
+    free(Q);
+    return 0;
 }

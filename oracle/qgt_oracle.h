@@ -96,6 +96,17 @@ void oracle_qgt_elementwise(size_t dim, int P, const double *psi, const double *
 int  oracle_qgt(int n, int nops, const oracle_op_t *ops, double *Q);
 /* Same values, overlaps hoisted and upper triangle only (timed CPU baseline). */
 int  oracle_qgt_hoisted(int n, int nops, const oracle_op_t *ops, double *Q);
+/* CPU baseline timing: the derivative states on `threads` OpenMP threads
+ * (<= 0: the default) with their wall time in *seconds; and rows [0, rows) of
+ * the per-element loop (variant 0: mu <= nu with three dot products per
+ * element, the structure of quantum_geometric_curvature.c:300-317 over
+ * quantum_geometric_tensor_network.c:1127-1166) or of the P x P loop (variant
+ * 1: differential_geometry.c:2832-2858) over given states, `reps` times,
+ * seconds in sample_s[]; *elements = Q elements one repetition computes. */
+int  oracle_derivative_states_timed(int n, int nops, const oracle_op_t *ops, int threads, double *psi, double *D,
+                                    double *seconds);
+int  oracle_qgt_loop_time(size_t dim, int P, const double *psi, const double *D, int variant, int threads, int rows,
+                          int reps, double *sample_s, long *elements);
 /* Q[r][c] for the listed rows/cols. */
 int  oracle_qgt_block(int n, int nops, const oracle_op_t *ops, int nr, const int *rows,
                       int nc, const int *cols, double *Q);

@@ -657,17 +657,18 @@ int qgtl_batch_set_angles(qgtl_batch_t *B, const double *angles)
 {
     QGTL_LOCKED();
     if (!B || !angles) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "null batch or angles");
-    /* the instances' plans are independent: rebuild them on the host cores */
+    /* the instances' plans are independent: rebuild them on the host cores (the
+     * workers must not take the library lock this thread holds) */
     int first_bad = B->count;
 #pragma omp parallel for schedule(dynamic, 4) reduction(min : first_bad)
     for (int i = 0; i < B->count; i++) {
         const inst_t *in = &B->inst[i];
-        if (qgtl_plan_set_angles(in->plan, in->nops, angles + in->op_off) != 0 && i < first_bad) first_bad = i;
+        if (qgtl_plan_set_angles_nolock(in->plan, in->nops, angles + in->op_off) != 0 && i < first_bad) first_bad = i;
     }
     if (first_bad < B->count) {
         /* again on this thread, for the error message */
         const inst_t *in = &B->inst[first_bad];
-        return qgtl_plan_set_angles(in->plan, in->nops, angles + in->op_off);
+        return qgtl_plan_set_angles_nolock(in->plan, in->nops, angles + in->op_off);
     }
     fill_descs(B);
     return 0;

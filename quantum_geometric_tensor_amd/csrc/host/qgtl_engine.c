@@ -313,12 +313,17 @@ qgtl_plan_t *qgtl_plan_create(int n, int nops, const qgtl_op_t *ops, int mode, c
 
 int qgtl_plan_num_params(const qgtl_plan_t *p) { return p ? p->P : -1; }
 
-int qgtl_plan_set_angles(qgtl_plan_t *p, int nops, const double *angles)
+int qgtl_plan_set_angles_nolock(qgtl_plan_t *p, int nops, const double *angles)
 {
-    QGTL_LOCKED();
     if (!p || !angles || nops != p->nops) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "angle count mismatch");
     for (int i = 0; i < nops; i++) p->ops[i].angle = angles[i];
     return qgtl_plan_build(p);
+}
+
+int qgtl_plan_set_angles(qgtl_plan_t *p, int nops, const double *angles)
+{
+    QGTL_LOCKED();
+    return qgtl_plan_set_angles_nolock(p, nops, angles);
 }
 
 void qgtl_plan_destroy(qgtl_plan_t *p)

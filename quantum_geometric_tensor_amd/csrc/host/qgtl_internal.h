@@ -147,6 +147,12 @@ static inline void qgtl_unlock_scope(int *unused) { (void)unused; qgtl_unlock();
 #define QGTL_LOCKED() \
     int qgtl_lock_guard_ __attribute__((cleanup(qgtl_unlock_scope), unused)) = (qgtl_lock(), 0)
 
+/* qgtl_plan_set_angles without taking the lock (host-only plan rebuild): for a
+ * caller that holds the lock and rebuilds several plans on worker threads, one
+ * thread per plan -- a worker taking the lock the caller holds would wait for
+ * the caller, which waits for the workers. */
+int qgtl_plan_set_angles_nolock(qgtl_plan_t *p, int nops, const double *angles);
+
 
 
 /* engine (qgtl_engine.c) */

@@ -40,6 +40,8 @@ def lib():
         L.oracle_qgt.argtypes = [ip, ip, vp, vp]
         L.oracle_qgt_hoisted.argtypes = [ip, ip, vp, vp]
         L.oracle_qgt_block.argtypes = [ip, ip, vp, ip, vp, ip, vp, vp]
+        L.oracle_derivative_states_timed.argtypes = [ip, ip, vp, ip, vp, vp, vp]
+        L.oracle_qgt_loop_time.argtypes = [ctypes.c_size_t, ip, vp, vp, ip, ip, ip, ip, vp, vp]
         L.oracle_dense_unitary.argtypes = [ip, ip, vp, vp]
         L.oracle_pauli_expectation.argtypes = [ip, vp, ctypes.c_char_p]
         L.oracle_pauli_expectation.restype = dp
@@ -134,6 +136,31 @@ def qgt_from_states(psi, D, variant="elementwise"):
     fn = lib().oracle_qgt_pairwise if variant == "pairwise" else lib().oracle_qgt_elementwise
     fn(dim, P, _p(np.ascontiguousarray(psi)), _p(np.ascontiguousarray(D)), _p(Q))
     return Q
+
+
+def derivative_states_timed(n, ops, threads, psi=None, D=None):
+    """Derivative states on `threads` threads (<= 0: default), into psi / D when given.
+    Returns (psi, D, seconds)."""
+    P = num_params(ops)
+    if psi is None:
+        psi = np.empty(1 << n, np.complex128)
+    if D is None:
+        D = np.empty((P, 1 << n), np.complex128)
+    sec = ctypes.c_double(0.0)
+    assert lib().oracle_derivative_states_timed(n, len(ops), _p(ops), threads, _p(psi), _p(D), ctypes.byref(sec)) == 0
+    return psi, D, sec.value
+
+
+def qgt_loop_time(psi, D, variant, threads, rows, reps):
+    """Rows [0, rows) of the per-element loop ("elementwise": mu <= nu, three dot
+    products per element) or of the P x P loop ("pairwise") over the given states,
+    `reps` times.  Returns ([seconds per repetition], elements per repetition)."""
+    P, dim = D.shape
+    el = ctypes.c_long(0)
+    t = np.zeros(reps, np.float64)
+    v = 1 if variant == "pairwise" else 0
+    assert lib().oracle_qgt_loop_time(dim, P, _p(psi), _p(D), v, threads, rows, reps, _p(t), ctypes.byref(el)) == 0
+    return list(t), el.value
 
 
 def qgt_block(n, ops, rows, cols):
