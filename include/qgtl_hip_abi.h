@@ -221,6 +221,7 @@ int qgtl_hip_free(void *ptr);
 int qgtl_hip_mem_free(size_t *free_bytes);            /* free device memory on the current device */
 int qgtl_hip_host_alloc(void **ptr, size_t bytes);     /* pinned host memory (hipHostMalloc) */
 int qgtl_hip_host_free(void *ptr);
+unsigned long long qgtl_hip_alloc_calls(void);        /* device/pinned allocations and frees made so far */
 int qgtl_hip_h2d(void *dst, const void *src, size_t bytes);   /* async on the library stream */
 int qgtl_hip_d2h(void *dst, const void *src, size_t bytes);   /* async on the library stream */
 int qgtl_hip_d2d(void *dst, const void *src, size_t bytes);
@@ -344,6 +345,16 @@ int qgtl_hip_herm_eig_c128(void *A, void *V, double *ev, int P, int batch, int m
 int qgtl_hip_cholesky_solve_f64(double *A, const double *b, double *x, int P, int batch, double lambda, int *status);
 int qgtl_hip_pcg_f64(const double *A, const double *b, double *x, int P, int batch, double lambda, const double *d,
                      const double *U, int r, double tol, int maxit, int *iters, double *relres);
+
+/* ---- fixed working memory of the no-allocation solve ----
+ * One file-scope device array of QGTL_NOALLOC_SCRATCH_BYTES per GPU (held
+ * until the process ends).  _init looks its device address up on the current
+ * device (called by qgtl_hip_init); _scratch returns that address and size;
+ * _jacobi_diag_f64 writes d = 1 / (diag(A_s) + lambda) for every system. */
+#define QGTL_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int qgtl_hip_noalloc_init(void);
+int qgtl_hip_noalloc_scratch(void **ptr, size_t *bytes);
+int qgtl_hip_jacobi_diag_f64(const double *A, int P, int batch, double lambda, double *d);
 int qgtl_hip_gram_small_c128(const void *M, void *B, int P);
 /* device-resident natural-gradient step: A = Re Q (slot -> parameter order) + lambda I, D = diag A,
  * g = gradient (eg = [E, grad in slot order]); learned-preconditioner features and output transform */

@@ -225,6 +225,21 @@ typedef struct {
 } qgtl_solve_info_t;
 int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double lambda, int method, double tol,
                    int maxit, const double *d, const double *U, int rank, double *x, qgtl_solve_info_t *info);
+/* No counterpart in QGTL: qgtl_solve_f64 for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results; every device
+ * buffer is carved from one fixed per-GPU array of QGTL_NOALLOC_SCRATCH_BYTES
+ * (32 MiB, held from library init until the process ends), whose address is
+ * looked up at init.  Needs 8 (P^2 + 3 P + rank P) batch bytes plus 16 batch
+ * (each rounded up to 256); larger calls, and batch > QGTL_NOALLOC_MAX_BATCH,
+ * fail with QGT_ERROR_INVALID_ARGUMENT.  info[].device_ms is host wall time.
+ * Calls share the array and keep the last call's contents; the library lock
+ * orders calls from one process, but work captured into graphs and replayed
+ * concurrently on one GPU would race on it. */
+#define QGTL_NOALLOC_MAX_BATCH 4096
+int qgtl_solve_f64_noalloc(int P, int batch, const double *A, const double *b, double lambda, int method,
+                           double tol, int maxit, const double *d, const double *U, int rank, double *x,
+                           qgtl_solve_info_t *info);
 /* Eigenvalues (ascending) and eigenvectors (columns of V, row-major) of a
  * Hermitian complex128 matrix, parallel Jacobi on the GPU.  V may be NULL. */
 int qgtl_hermitian_eig(int P, const double *A, double *evals, double *V);
@@ -321,6 +336,10 @@ int qgtl_debug_mfma_layout(float *host_out);
  * fresh device buffers; *ms = its HIP-event time.  Reference of known size
  * for the HBM traffic counters (FETCH_SIZE / WRITE_SIZE). */
 int qgtl_debug_copy_kernel(size_t bytes, double *ms);
+/* Device and pinned-host allocations and frees the library has made so far
+ * (every one goes through one runtime function): lets a test show that a call
+ * allocated nothing. */
+unsigned long long qgtl_debug_alloc_calls(void);
 
 /* Diagnostics: largest |diag(post) R diag(pre) - U| over the sweep passes'
  * gates (the rotation form of the pass descriptors) and how many passes

@@ -14,6 +14,7 @@ hipStream_t g_side = nullptr;
 int g_lane = 0;
 int g_device = -1;
 thread_local char g_msg[256] = "";
+unsigned long long g_allocs = 0;   // hipMalloc / hipHostMalloc / hipFree / hipHostFree calls made
 
 int code_for(hipError_t e) {
     switch (e) {
@@ -66,7 +67,7 @@ int qgtl_hip_init(int device) {
     QGTL_HIP_CHECK(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     QGTL_HIP_CHECK(hipStreamCreateWithFlags(&g_side, hipStreamNonBlocking));
     g_device = device;
-    return 0;
+    return qgtl_hip_noalloc_init();   // the no-allocation solve's scratch address on this device
 }
 
 void *qgtl_hip_stream(void) { return (void *)g_stream; }
@@ -74,14 +75,19 @@ void *qgtl_hip_stream(void) { return (void *)g_stream; }
 int qgtl_hip_malloc(void **ptr, size_t bytes) {
     *ptr = nullptr;
     if (bytes == 0) return 0;
+    g_allocs++;
     QGTL_HIP_CHECK(hipMalloc(ptr, bytes));
     return 0;
 }
 
 int qgtl_hip_free(void *ptr) {
-    if (ptr) QGTL_HIP_CHECK(hipFree(ptr));
+    if (!ptr) return 0;
+    g_allocs++;
+    QGTL_HIP_CHECK(hipFree(ptr));
     return 0;
 }
+
+unsigned long long qgtl_hip_alloc_calls(void) { return g_allocs; }
 
 int qgtl_hip_mem_free(size_t *free_bytes) {
     size_t total = 0;
@@ -93,12 +99,15 @@ int qgtl_hip_mem_free(size_t *free_bytes) {
 int qgtl_hip_host_alloc(void **ptr, size_t bytes) {
     *ptr = nullptr;
     if (bytes == 0) return 0;
+    g_allocs++;
     QGTL_HIP_CHECK(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
     return 0;
 }
 
 int qgtl_hip_host_free(void *ptr) {
-    if (ptr) QGTL_HIP_CHECK(hipHostFree(ptr));
+    if (!ptr) return 0;
+    g_allocs++;
+    QGTL_HIP_CHECK(hipHostFree(ptr));
     return 0;
 }
 

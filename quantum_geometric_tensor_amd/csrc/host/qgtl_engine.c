@@ -1177,6 +1177,8 @@ int qgtl_debug_copy_kernel(size_t bytes, double *ms)
     return 0;
 }
 
+unsigned long long qgtl_debug_alloc_calls(void) { return qgtl_hip_alloc_calls(); }
+
 int qgtl_init_if_needed(void)
 {
     QGTL_LOCKED();

@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "qgtl_api.h"
 #include "qgtl_internal.h"
@@ -119,6 +120,108 @@ int qgtl_solve_f64(int P, int batch, const double *A, const double *b, double la
         if (info) info[s] = inf;
     }
     free(hst); free(hit); free(hres);
+    if (fail) return qgtl_set_error(method == QGTL_SOLVE_CHOLESKY ? QGT_ERROR_MATRIX_SINGULAR : QGT_ERROR_CONVERGENCE_FAILED,
+                                    method == QGTL_SOLVE_CHOLESKY ? "matrix not positive definite" : "CG did not reach the tolerance");
+    return 0;
+}
+
+/* ---------------------------------------------- no-allocation variant */
+/* Per-system results come back through these (the call holds the library
+ * lock, so one call uses them at a time). */
+static int g_na_status[QGTL_NOALLOC_MAX_BATCH];
+static int g_na_iters[QGTL_NOALLOC_MAX_BATCH];
+static double g_na_res[QGTL_NOALLOC_MAX_BATCH];
+
+static size_t carve(size_t *off, size_t bytes)
+{
+    const size_t at = *off;
+    *off += (bytes + 255) / 256 * 256;
+    return at;
+}
+
+static double wall_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* qgtl_solve_f64 with every device buffer carved from the library's fixed
+ * per-GPU scratch array (QGTL_NOALLOC_SCRATCH_BYTES) and no allocation of any
+ * kind: the Jacobi preconditioner is formed on the device, per-system results
+ * return through static host arrays, timing is host wall time. */
+int qgtl_solve_f64_noalloc(int P, int batch, const double *A, const double *b, double lambda, int method, double tol,
+                           int maxit, const double *d, const double *U, int rank, double *x, qgtl_solve_info_t *info)
+{
+    QGTL_LOCKED();
+    if (P < 1 || P > 1024 || batch < 1 || !A || !b || !x)
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64_noalloc: bad arguments");
+    if (batch > QGTL_NOALLOC_MAX_BATCH)
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64_noalloc: batch %d exceeds the maximum %d",
+                              batch, QGTL_NOALLOC_MAX_BATCH);
+    if (method != QGTL_SOLVE_CHOLESKY && method != QGTL_SOLVE_CG && method != QGTL_SOLVE_JACOBI_PCG &&
+        method != QGTL_SOLVE_PCG)
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64_noalloc: unknown method %d", method);
+    if (method == QGTL_SOLVE_PCG && (rank < 0 || rank > 64 || (rank > 0 && !U)))
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT,
+                              "qgtl_solve_f64_noalloc: low-rank factor needs 0 <= rank <= 64");
+    const size_t PP = (size_t)P * P * batch, Pb = (size_t)P * batch;
+    const int pre = method == QGTL_SOLVE_JACOBI_PCG || (method == QGTL_SOLVE_PCG && d);
+    const int lr = method == QGTL_SOLVE_PCG && rank > 0;
+    size_t off = 0;
+    const size_t oA = carve(&off, PP * sizeof(double)), ob = carve(&off, Pb * sizeof(double));
+    const size_t ox = carve(&off, Pb * sizeof(double));
+    const size_t od = pre ? carve(&off, Pb * sizeof(double)) : 0;
+    const size_t oU = lr ? carve(&off, Pb * (size_t)rank * sizeof(double)) : 0;
+    const size_t ost = carve(&off, (size_t)batch * sizeof(int)), oit = carve(&off, (size_t)batch * sizeof(int));
+    const size_t ores = carve(&off, (size_t)batch * sizeof(double));
+    if (off > QGTL_NOALLOC_SCRATCH_BYTES)
+        return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT,
+                              "qgtl_solve_f64_noalloc: P = %d, batch = %d needs %zu bytes of device scratch, the "
+                              "maximum is %zu", P, batch, off, (size_t)QGTL_NOALLOC_SCRATCH_BYTES);
+    TRY(qgtl_init_if_needed());
+    void *base = NULL;
+    size_t cap = 0;
+    TRY(qgtl_hip_noalloc_scratch(&base, &cap));
+    if (off > cap) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qgtl_solve_f64_noalloc: scratch too small");
+    char *s = (char *)base;
+    double *dA = (double *)(s + oA), *db = (double *)(s + ob), *dx = (double *)(s + ox);
+    double *dd = pre ? (double *)(s + od) : NULL, *dU = lr ? (double *)(s + oU) : NULL;
+    int *dst = (int *)(s + ost), *dit = (int *)(s + oit);
+    double *dres = (double *)(s + ores);
+    const double t0 = wall_ms();
+    TRY(qgtl_hip_h2d(dA, A, PP * sizeof(double)));
+    TRY(qgtl_hip_h2d(db, b, Pb * sizeof(double)));
+    if (method == QGTL_SOLVE_CHOLESKY) {
+        TRY(qgtl_hip_cholesky_solve_f64(dA, db, dx, P, batch, lambda, dst));
+    } else {
+        if (method == QGTL_SOLVE_JACOBI_PCG) TRY(qgtl_hip_jacobi_diag_f64(dA, P, batch, lambda, dd));
+        if (method == QGTL_SOLVE_PCG && d) TRY(qgtl_hip_h2d(dd, d, Pb * sizeof(double)));
+        if (lr) TRY(qgtl_hip_h2d(dU, U, Pb * (size_t)rank * sizeof(double)));
+        TRY(qgtl_hip_pcg_f64(dA, db, dx, P, batch, lambda, dd, dU, lr ? rank : 0, tol, maxit, dit, dres));
+    }
+    TRY(qgtl_hip_d2h(x, dx, Pb * sizeof(double)));
+    if (method == QGTL_SOLVE_CHOLESKY) TRY(qgtl_hip_d2h(g_na_status, dst, (size_t)batch * sizeof(int)));
+    else {
+        TRY(qgtl_hip_d2h(g_na_iters, dit, (size_t)batch * sizeof(int)));
+        TRY(qgtl_hip_d2h(g_na_res, dres, (size_t)batch * sizeof(double)));
+    }
+    TRY(qgtl_hip_sync());
+    const float ms = (float)(wall_ms() - t0);
+    int fail = 0;
+    for (int k = 0; k < batch; k++) {
+        qgtl_solve_info_t inf = {0, 0, 0.0, ms};
+        if (method == QGTL_SOLVE_CHOLESKY) {
+            inf.status = g_na_status[k];
+            inf.iterations = 1;
+        } else {
+            inf.iterations = g_na_iters[k];
+            inf.rel_residual = g_na_res[k];
+            inf.status = (g_na_res[k] <= tol) ? 0 : -1;
+        }
+        if (inf.status) fail = 1;
+        if (info) info[k] = inf;
+    }
     if (fail) return qgtl_set_error(method == QGTL_SOLVE_CHOLESKY ? QGT_ERROR_MATRIX_SINGULAR : QGT_ERROR_CONVERGENCE_FAILED,
                                     method == QGTL_SOLVE_CHOLESKY ? "matrix not positive definite" : "CG did not reach the tolerance");
     return 0;

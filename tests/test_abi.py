@@ -26,6 +26,7 @@ def declared_functions(path):
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"^\s*#[^\n]*", "", text, flags=re.M)     # preprocessor lines (macros with arguments)
     text = re.sub(r"typedef[^;]*;", "", text, flags=re.S)   # struct/enum typedefs
     names = set()
     for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_\s\*]*?)\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
