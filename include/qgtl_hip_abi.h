@@ -208,6 +208,8 @@ int qgtl_hip_finalize_grouped(const qgtl_finalize_item_t *items, int nitems, uin
 typedef struct qgtl_hip_graph_s *qgtl_hip_graph_t;
 int qgtl_hip_capture_begin(void);
 int qgtl_hip_capture_end(qgtl_hip_graph_t *graph);       /* instantiates */
+int qgtl_hip_capture_end_update(qgtl_hip_graph_t *graph);   /* update *graph in place, or (re)instantiate */
+void qgtl_hip_capture_abort(void);
 int qgtl_hip_graph_launch(qgtl_hip_graph_t graph);
 void qgtl_hip_graph_destroy(qgtl_hip_graph_t graph);
 int qgtl_hip_cu_count(void);
@@ -249,10 +251,11 @@ int qgtl_hip_lane_wait(qgtl_hip_event_t ev);
 int qgtl_hip_fill_zero_state_c64(void *psi, int n);                    /* |0...0> */
 /* Split-plane form of a final derivative state (the Gram's operand form): the
  * 8 N bytes of state s hold, for each block of 64 amplitudes (512 bytes, the
- * footprint of the same block in complex64), four fp16 rows of 64 values: Re hi,
- * Re lo, Im hi, Im lo, where for each component x: hi = 2^15 x rounded to 11
- * significant bits (nearest, ties away) and lo = fp16(2^15 x - hi).  Amplitude
- * y, row r: fp16 index 256 (y / 64) + 64 r + y % 64. */
+ * footprint of the same block in complex64), 64 hi words then 64 lo words, one
+ * per amplitude, each an fp16 pair (Re piece, Im piece), where for each
+ * component x: hi = 2^15 x rounded to 11 significant bits (nearest, ties away)
+ * and lo = fp16(2^15 x - hi).  Amplitude y: hi word 128 (y / 64) + y % 64, lo
+ * word 64 further (32-bit word indices). */
 /* nstates states at src + s*stride -> dst + s*stride (elements of complex64);
  * split_out = 1: dst receives the split-plane form (register-phase passes only,
  * see qgtl_hip_pass_split_ok); tabs: NULL, or the pass's rotation-form phase
@@ -298,8 +301,9 @@ int qgtl_hip_gram_tiles(int P);                                          /* uppe
 #define QGTL_GRAM_KSTEP 64   /* amplitudes per Gram LDS stage; chunk lengths are multiples of it */
 #define QGTL_GRAM_F16X3 33
 #define QGTL_GRAM_F16X4 34
-/* presplit = 1: D holds split-plane states (QGTL_GRAM_F16X3 / X4 only) */
-int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms, int presplit);
+/* Real-form Gram of complex64 states (split in the kernel): C = Z Z^T with Z
+ * rows (Re d_s, Im d_s) */
+int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms);
 int qgtl_hip_gram_reduce(const float *partials, int P, int nchunks, double *C);   /* C: R x R tiles, FP64 */
 /* Q = normalised G - a a^H from C and injection sums (P x 4 doubles). */
 int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q);
@@ -308,6 +312,22 @@ int qgtl_hip_qgt_finalize(const double *C, const double *inj, int P, double *Q);
 int qgtl_hip_qgt_finalize_psi(const double *C, int P, double *Q);
 /* one complex64 state of 2^n amplitudes (n >= 6) in split-plane form */
 int qgtl_hip_c64_to_split(const void *src, void *dst, int n);
+
+/* ---- complex-form Gram of split-plane states ----
+ * G = D^H D of P states in split-plane form (N % 64 == 0), tiles of 128 states
+ * (nt = ceil(P / 128), T = nt (nt + 1) / 2 upper tile pairs), split-K over
+ * nchunks (a multiple of 8) chunks of amplitudes: fp32 partials of Re G and Im G
+ * per chunk and tile pair (qgtl_hip_gram_cplx_partial_floats(P) floats per
+ * chunk), FP64 sum over the chunks into G (complex128, R x R, R = 128 nt; only
+ * entries (k, l) with k <= l are written), fp16 pieces with
+ * products = 3 (hh + hl + lh) or 4 (+ ll); G is scaled by 2^30. */
+size_t qgtl_hip_gram_cplx_partial_floats(int P);
+/* stage_amps: amplitudes per LDS stage, 16 (four buffers, three stages in flight) or 32 (two) */
+int qgtl_hip_gram_cplx(const void *D, int P, size_t N, float *partials, int nchunks, int products, int stage_amps);
+int qgtl_hip_gram_cplx_reduce(const float *partials, int P, int nchunks, double *G);
+/* Q = normalised G - a a^H from the complex Gram of P + 1 states (state P the
+ * primal state at the derivative states' meeting point, so a_k = G_kP) */
+int qgtl_hip_qgt_finalize_cplx(const double *G, int P, double *Q);
 /* MFMA lane-layout probe (diagnostics): 32 x 32 floats to out_dev */
 int qgtl_hip_debug_mfma_layout(float *out_dev);
 

@@ -60,6 +60,8 @@ typedef struct {
     int32_t n;
     int32_t presplit;               /* 1: final derivative states were written as the Gram's fp16 pieces */
     int32_t meet_block;             /* derivative states met after this block (the last one: all carried forward) */
+    int32_t graph;                  /* 1: the launches ran as a replayed HIP graph (counts are the captured ones) */
+    int32_t reserved;
 } qgtl_stats_t;
 
 /* Tunables chosen by the autotuner (or defaults when NULL / zero fields). */
@@ -80,7 +82,9 @@ typedef struct {
     int32_t no_fused_inject;  /* 0: a derivative-state chunk that is carried through passes is formed by its
                                  first pass while loading (from the primal state, read through the caches),
                                  never written or read as injected; 1: a separate injection kernel writes it */
-    int32_t reserved[1];
+    int32_t no_graph;         /* 0: from the second execute of an unchanged plan (same angles and tune) on,
+                                 the launch sequence runs as one captured HIP graph (per-launch profiling
+                                 keeps it eager); 1: every execute launches its kernels one by one */
 } qgtl_tune_t;
 
 int qgtl_init(int device);
@@ -144,7 +148,9 @@ typedef struct {
     int32_t pass_variant;     /* 0: register phases where they apply; 1: LDS gate groups only */
     int32_t no_graph;         /* 1: issue the launches directly instead of replaying the graph */
     int32_t sweep_form;       /* 0: rotation form of the pass matrices; 1: general complex matrices */
-    int32_t reserved[1];
+    int32_t sweep_schedule;   /* 0: two-sided per instance where its blocks allow it (derivative states of
+                                 late blocks carried backwards through inverse blocks to a meeting block);
+                                 1: every state carried forward to the end (fixed at creation) */
 } qgtl_batch_tune_t;
 
 typedef struct {

@@ -40,7 +40,8 @@ class Stats(ctypes.Structure):
                 ("sweep_bytes", ctypes.c_double), ("gram_launches", ctypes.c_int64),
                 ("gram_flops", ctypes.c_double), ("gram_kernel_ms", ctypes.c_double),
                 ("kernel_launches", ctypes.c_int64), ("P", ctypes.c_int32), ("n", ctypes.c_int32),
-                ("presplit", ctypes.c_int32), ("meet_block", ctypes.c_int32)]
+                ("presplit", ctypes.c_int32), ("meet_block", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -50,7 +51,7 @@ class Tune(ctypes.Structure):
     _fields_ = [("chunk_states", ctypes.c_int32), ("gram_chunks", ctypes.c_int32),
                 ("gram_variant", ctypes.c_int32), ("sweep_form", ctypes.c_int32), ("no_presplit", ctypes.c_int32),
                 ("sweep_schedule", ctypes.c_int32), ("no_fused_inject", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("no_graph", ctypes.c_int32)]
 
 
 _lib = None
@@ -143,11 +144,11 @@ class Plan:
     """A compiled circuit with device buffers, re-run with new angles."""
 
     def __init__(self, n, ops, mode=MODE_MIXED, chunk_states=0, gram_chunks=0, gram_terms=0, sweep_form=0,
-                 presplit=True, two_sided=True, fused_inject=True):
+                 presplit=True, two_sided=True, fused_inject=True, graph=True):
         self.ops = np.ascontiguousarray(ops, OP_DTYPE).copy()
         self.n = n
         tune = Tune(chunk_states, gram_chunks, gram_terms, sweep_form, 0 if presplit else 1, 0 if two_sided else 1,
-                    0 if fused_inject else 1)
+                    0 if fused_inject else 1, 0 if graph else 1)
         h = lib().qgtl_plan_create(n, len(self.ops), _p(self.ops), mode, ctypes.byref(tune))
         if not h:
             raise QGTLError(-1, lib().qgtl_last_error().decode(errors="replace"))
@@ -286,10 +287,10 @@ def rel_err(Q, Qref):
 class BatchTune(ctypes.Structure):
     _fields_ = [("pass_grid", ctypes.c_int32), ("inject_grid", ctypes.c_int32), ("gram_grid", ctypes.c_int32),
                 ("gram_chunk_len", ctypes.c_int32), ("pass_variant", ctypes.c_int32), ("no_graph", ctypes.c_int32),
-                ("sweep_form", ctypes.c_int32), ("reserved", ctypes.c_int32 * 1)]
+                ("sweep_form", ctypes.c_int32), ("sweep_schedule", ctypes.c_int32)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class BatchStats(ctypes.Structure):

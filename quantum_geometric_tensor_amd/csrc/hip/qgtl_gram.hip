@@ -158,53 +158,6 @@ __device__ __forceinline__ void store_rows(const float4 (&vin)[kStageLoads], uin
     }
 }
 
-constexpr int kPlaneLoads = 64 * 4 * (kKS / 8) / kThreads;   // 16-byte loads per thread for 64 split-plane states
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-// Split-plane states (per block of 64 amplitudes, four fp16 rows: Re hi, Re lo,
-// Im hi, Im lo, already split as store_rows would): load i of thread t reads 8
-// values of row (t / 8) % 4 of state s0 + 8 i + t / 32, amplitude block k0 / 64,
-// from amplitude k0 + 8 (t % 8) -- 32 lanes read one state's 512 contiguous
-// bytes, as for complex64.  States past P read state P - 1 instead: their LDS
-// rows then only feed C rows and columns >= 2P, which nothing reads, so no
-// masking is needed and every load is unconditional.  No conversion work: the
-// pieces go to LDS as read.
-__device__ __forceinline__ void load_planes(const uint16_t *__restrict__ D16, int P, size_t N, int s0, size_t k0,
-                                            u32x4 (&v)[kPlaneLoads]) {
-    static_assert(kKS == 64, "one 64-amplitude block per stage");
-    const int c = threadIdx.x & 7, pl = (threadIdx.x >> 3) & 3, sl = threadIdx.x >> 5;
-    if (s0 + 64 <= P && N <= ((size_t)1 << 26)) {
-        // a full block of 64 states: wave-uniform base per load plus one 32-bit lane offset
-        // (below 7 * 8 N + 512 bytes), so no per-load 64-bit address registers
-        const uint32_t lane_off = (uint32_t)(((size_t)sl * 4 * N + 64 * pl + 8 * c) * 2);
-        const char *b0 = reinterpret_cast<const char *>(D16) + ((size_t)s0 * 4 * N + (k0 >> 6) * 256) * 2;
-#pragma unroll
-        for (int i = 0; i < kPlaneLoads; i++)
-            v[i] = *reinterpret_cast<const u32x4 *>(b0 + (size_t)i * 8 * 4 * N * 2 + lane_off);
-        return;
-    }
-    const uint16_t *blk = D16 + (k0 >> 6) * 256 + 64 * pl + 8 * c;
-#pragma unroll
-    for (int i = 0; i < kPlaneLoads; i++) {
-        int s = (s0 + 8 * i + sl < P) ? s0 + 8 * i + sl : P - 1;
-        // an opaque copy: the clamped address is formed at every stage instead of
-        // being hoisted out of the loop as eight 64-bit registers (which spilled)
-        __asm__ volatile("" : "+v"(s));
-        v[i] = *reinterpret_cast<const u32x4 *>(blk + (size_t)s * 4 * N);
-    }
-}
-
-// Plane pl of a state goes to piece array pl % 2 (hi, lo), row 2 s + pl / 2 (Re, Im).
-__device__ __forceinline__ void store_planes(const u32x4 (&v)[kPlaneLoads], uint32_t *hi_rows, uint32_t *lo_rows) {
-    const int c = threadIdx.x & 7, pl = (threadIdx.x >> 3) & 3, sl = threadIdx.x >> 5;
-    uint32_t *piece = (pl & 1) ? lo_rows : hi_rows;
-#pragma unroll
-    for (int i = 0; i < kPlaneLoads; i++) {
-        const int row = 2 * (8 * i + sl) + (pl >> 1);
-        *reinterpret_cast<u32x4 *>(piece + row * (kRow / 2) + 4 * c) = v[i];
-    }
-}
-
 template <int TERMS>
 struct __attribute__((aligned(16))) GramLds {
     static constexpr int S = (TERMS == 6) ? 3 : 2;     // bf16 pieces per value
@@ -214,7 +167,7 @@ struct __attribute__((aligned(16))) GramLds {
 // Unit `bid` of one Gram: chunk c's tiles are the units with bid % 8 == c % 8.
 // ONE_ROUND: a stage issues the A and B loads together (one memory latency per
 // stage); otherwise A, then B (two latencies, 32 fewer registers).
-template <int TERMS, bool ONE_ROUND, bool PRESPLIT = false>
+template <int TERMS, bool ONE_ROUND>
 __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, size_t N, int nt, int nchunks,
                                           size_t chunk_len, float *__restrict__ partials, int bid,
                                           GramLds<TERMS> &sm) {
@@ -258,24 +211,10 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
         brows[p] = diag ? lds[p] : lds[S + p];
     }
 
-    // split planes: the next stage's loads are issued before this stage's MFMAs (no
-    // conversion work is left to do between load and LDS, so the registers allow it)
-    [[maybe_unused]] const uint16_t *D16 = reinterpret_cast<const uint16_t *>(D);
-    [[maybe_unused]] u32x4 pva[kPlaneLoads], pvb[kPlaneLoads];
-    if constexpr (PRESPLIT) {
-        static_assert(S == 2 && F16, "split planes hold two fp16 pieces");
-        if (kbeg < kend) {
-            load_planes(D16, P, N, ti * (kTile / 2), kbeg, pva);
-            if (!diag) load_planes(D16, P, N, tj * (kTile / 2), kbeg, pvb);
-        }
-    }
     for (size_t kb = kbeg; kb < kend; kb += kInner) {
         const size_t ke = (kb + kInner < kend) ? kb + kInner : kend;
         for (size_t k0 = kb; k0 < ke; k0 += kKS) {
-            if constexpr (PRESPLIT) {
-                store_planes(pva, arows[0], arows[1]);
-                if (!diag) store_planes(pvb, brows[0], brows[1]);
-            } else if (ONE_ROUND) {
+            if (ONE_ROUND) {
                 float4 va[kStageLoads], vb[kStageLoads];
                 const uint32_t ma = load_rows(D, P, N, ti * (kTile / 2), k0, va);
                 const uint32_t mb = diag ? 0u : load_rows(D, P, N, tj * (kTile / 2), k0, vb);
@@ -294,12 +233,6 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
                 }
             }
             __syncthreads();
-            if constexpr (PRESPLIT) {
-                if (k0 + kKS < kend) {
-                    load_planes(D16, P, N, ti * (kTile / 2), k0 + kKS, pva);
-                    if (!diag) load_planes(D16, P, N, tj * (kTile / 2), k0 + kKS, pvb);
-                }
-            }
             // unrolled: the next step's fragment reads issue under this step's MFMAs
 #pragma unroll
             for (int kk = 0; kk < kKS / 16; kk++) {
@@ -359,12 +292,12 @@ __device__ __forceinline__ void gram_unit(const float *__restrict__ D, int P, si
         }
 }
 
-template <int TERMS, bool PRESPLIT = false>
+template <int TERMS>
 __global__ __launch_bounds__(kThreads, (TERMS == 6) ? 1 : 2) void gram_split_kernel(const float *__restrict__ D, int P, size_t N,
                                                               int nt, int nchunks, size_t chunk_len,
                                                               float *__restrict__ partials) {
     __shared__ GramLds<TERMS> sm;
-    gram_unit<TERMS, TERMS != 6, PRESPLIT>(D, P, N, nt, nchunks, chunk_len, partials, blockIdx.x, sm);
+    gram_unit<TERMS, TERMS != 6>(D, P, N, nt, nchunks, chunk_len, partials, blockIdx.x, sm);
 }
 
 // Grouped: grid and every unit_begin are multiples of 8, so u % 8 == local % 8
@@ -543,21 +476,15 @@ int qgtl_hip_gram_tiles(int P) {
     return nt * (nt + 1) / 2;
 }
 
-int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms, int presplit) {
+int qgtl_hip_gram_bf16x3(const void *D, int P, size_t N, float *partials, int nchunks, int terms) {
     if (P < 1 || nchunks < 1 || nchunks % kXcd != 0 || (N & 1))
         return qgtl::set_error_msg(-118, "qgtl_hip_gram_bf16x3: nchunks must be a multiple of 8 and N even");
-    if (presplit && ((terms != QGTL_GRAM_F16X3 && terms != QGTL_GRAM_F16X4) || N % kKS != 0))
-        return qgtl::set_error_msg(-118, "qgtl_hip_gram_bf16x3: split planes need an fp16 variant and N % 64 == 0");
     const int nt = (2 * P + kTile - 1) / kTile;
     const int T = nt * (nt + 1) / 2;
     const size_t chunk_len = ((N + nchunks - 1) / nchunks + kKS - 1) / kKS * kKS;
     const dim3 grid((unsigned)(T * nchunks)), block(kThreads);
     if (terms == 6)
         hipLaunchKernelGGL(gram_split_kernel<6>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
-    else if (terms == QGTL_GRAM_F16X3 && presplit)
-        hipLaunchKernelGGL((gram_split_kernel<QGTL_GRAM_F16X3, true>), grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
-    else if (terms == QGTL_GRAM_F16X4 && presplit)
-        hipLaunchKernelGGL((gram_split_kernel<QGTL_GRAM_F16X4, true>), grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     else if (terms == QGTL_GRAM_F16X3)
         hipLaunchKernelGGL(gram_split_kernel<QGTL_GRAM_F16X3>, grid, block, 0, qgtl::stream(), (const float *)D, P, N, nt, nchunks, chunk_len, partials);
     else if (terms == QGTL_GRAM_F16X4)

@@ -203,6 +203,41 @@ int qgtl_hip_capture_end(qgtl_hip_graph_t *out) {
     return 0;
 }
 
+// End a capture into *graph: an existing graph's executable is updated in
+// place when the new sequence has the same shape (kernel arguments changed),
+// else it is replaced.
+int qgtl_hip_capture_end_update(qgtl_hip_graph_t *graph) {
+    hipGraph_t g = nullptr;
+    QGTL_HIP_CHECK(hipStreamEndCapture(g_stream, &g));
+    if (*graph) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate((*graph)->exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+            (void)hipGraphDestroy((*graph)->graph);
+            (*graph)->graph = g;
+            return 0;
+        }
+        (void)hipGetLastError();
+        qgtl_hip_graph_destroy(*graph);
+        *graph = nullptr;
+    }
+    hipGraphExec_t x = nullptr;
+    hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return qgtl::set_error(e, "hipGraphInstantiate");
+    }
+    *graph = new qgtl_hip_graph_s{g, x};
+    return 0;
+}
+
+// Abandon a capture that failed part way (the stream leaves capture mode).
+void qgtl_hip_capture_abort(void) {
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(g_stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+}
+
 int qgtl_hip_graph_launch(qgtl_hip_graph_t graph) {
     QGTL_HIP_CHECK(hipGraphLaunch(graph->exec, g_stream));
     return 0;

@@ -514,18 +514,17 @@ __device__ __forceinline__ void apply_phase_mat(float2 (&v)[kRegs], const qgtl_p
 }
 
 // Amplitudes g (even) and g + 1 of a state in split-plane form: per block of 64
-// amplitudes four fp16 rows (Re hi, Re lo, Im hi, Im lo), here as uint32 pairs
-// (128 per block, 32 per row), the same pieces the Gram would cut from the
-// complex64 values.
+// amplitudes 128 words, the hi words (Re hi, Im hi as an fp16 pair) of the 64
+// amplitudes, then their lo words (Re lo, Im lo) -- the pieces the Gram would
+// cut from the complex64 values, interleaved per amplitude as the Gram's
+// complex products read them.
 __device__ __forceinline__ void write_split_pair(uint32_t *planes, size_t g, float2 a, float2 b) {
-    uint32_t rh, rl, ih, il;
-    qgtl::split_f16_pair(a.x * qgtl::kSplitScale, b.x * qgtl::kSplitScale, rh, rl);
-    qgtl::split_f16_pair(a.y * qgtl::kSplitScale, b.y * qgtl::kSplitScale, ih, il);
-    uint32_t *blk = planes + (g >> 6) * 128 + ((g & 63) >> 1);
-    blk[0] = rh;
-    blk[32] = rl;
-    blk[64] = ih;
-    blk[96] = il;
+    uint32_t h0, l0, h1, l1;
+    qgtl::split_f16_pair(a.x * qgtl::kSplitScale, a.y * qgtl::kSplitScale, h0, l0);
+    qgtl::split_f16_pair(b.x * qgtl::kSplitScale, b.y * qgtl::kSplitScale, h1, l1);
+    uint32_t *blk = planes + (g >> 6) * 128 + (g & 63);
+    *reinterpret_cast<uint2 *>(blk) = make_uint2(h0, h1);
+    *reinterpret_cast<uint2 *>(blk + 64) = make_uint2(l0, l1);
 }
 
 template <int R>
@@ -995,9 +994,10 @@ __global__ __launch_bounds__(kThreads) void dot_states_c64_kernel(const float2 *
         if (y < N) {
             float2 a;
             if constexpr (PLANES) {
-                const _Float16 *pl = reinterpret_cast<const _Float16 *>(d) + (size_t)(y >> 6) * 256 + (y & 63);
+                // hi word (Re hi, Im hi) and lo word of amplitude y
+                const _Float16 *pl = reinterpret_cast<const _Float16 *>(d) + (size_t)(y >> 6) * 256 + 2 * (y & 63);
                 const float inv = 1.0f / qgtl::kSplitScale;
-                a = make_float2(((float)pl[0] + (float)pl[64]) * inv, ((float)pl[128] + (float)pl[192]) * inv);
+                a = make_float2(((float)pl[0] + (float)pl[128]) * inv, ((float)pl[1] + (float)pl[129]) * inv);
             } else {
                 a = d[y];
             }

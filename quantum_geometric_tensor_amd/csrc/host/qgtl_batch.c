@@ -3,15 +3,24 @@
  * launch sequence.
  *
  * Every instance is compiled by the single-instance plan builder (blocks,
- * passes, injection operators).  The batch then lays all instances out in
- * a few device arenas and advances them together, breadth-first: for block
- * step b, pass j, ONE grouped launch sweeps the primal and every existing
- * derivative state of every instance that has such a pass; one grouped
- * launch injects the new derivative states of all instances; after the last
- * block one grouped launch per Gram precision class, one split-K reduction
- * and one FP64 correction finish every Q.  The primal is slot 0 of an
- * instance's state array and derivative state s is slot 1 + s, so one pass
- * item covers all of an instance's live states.  Launch grids are
+ * passes and inverse passes, injection operators).  The batch then lays all
+ * instances out in a few device arenas and advances them together,
+ * breadth-first, on the two-sided schedule of the single plans: each instance
+ * has a meeting block c (the one that minimises its state-passes).  Forward
+ * phase, block step b: ONE grouped launch injects, for every instance with
+ * b > c, the states of block b at the block's input (from the primal before
+ * the block, with the block's inverse folded into the 2 x 2); for pass j one
+ * grouped launch sweeps the primal and, for instances with b <= c, every
+ * forward derivative state; one grouped launch injects the new forward states
+ * of instances with b <= c.  Backward phase, block step b' descending: one
+ * grouped launch per inverse pass carries every instance's states of blocks
+ * > b' (b' > c) back through block b'.  All states then sit after their
+ * instance's block c, in one arena, and one grouped launch per Gram precision
+ * class, one split-K reduction and one FP64 correction finish every Q (the
+ * inner products are the same after any common unitary).  The primal is slot
+ * 0 of an instance's state array and derivative state s is slot 1 + s (forward
+ * states first), so one pass item covers a contiguous range of an instance's
+ * live states.  Launch grids are
  * persistent (a few workgroups per CU looping over units) and the whole
  * sequence is captured once in a HIP graph; set_angles only rewrites the
  * descriptors the graph reads.
@@ -66,6 +75,9 @@ typedef struct {
     int nt, T, nchunks, terms;
     size_t chunk_len;
     int flips;                 /* any pass of the instance stores to the other buffer */
+    int meet;                  /* states meet after this block (the last one: all carried forward) */
+    int nfwd;                  /* derivative states of blocks <= meet (slots 1 .. nfwd) */
+    int final_arena;           /* 0: x, 1: y -- where all derivative states end */
 } inst_t;
 
 struct qgtl_batch {
@@ -116,7 +128,7 @@ static int regs_variant(const qgtl_pass_desc_t *d, int pass_variant)
 static size_t count_pass_descs(const qgtl_plan_t *p)
 {
     size_t c = 0;
-    for (int b = 0; b < p->nblocks; b++) c += (size_t)p->blocks[b].npasses;
+    for (int b = 0; b < p->nblocks; b++) c += (size_t)p->blocks[b].npasses + (size_t)p->blocks[b].ninv;
     return c;
 }
 
@@ -130,16 +142,24 @@ static size_t count_inj_descs(const qgtl_plan_t *p)
     return c;
 }
 
-/* Descriptor order: instance, block, pass (or injection chunk); the
- * schedule walks the same order. */
+/* Descriptor order per instance: the passes of every block, then the inverse
+ * passes of every block (block order); the injection chunks in block order,
+ * forward (A after the block) for blocks <= meet, backward (B at the block's
+ * input, on the parameter's qubit) for later blocks.  The schedule walks the
+ * same order. */
 static void fill_descs(struct qgtl_batch *B)
 {
     size_t ip = 0, ii = 0;
     for (int i = 0; i < B->count; i++) {
         const qgtl_plan_t *p = B->inst[i].plan;
+        const int meet = B->inst[i].meet;
+        for (int b = 0; b < p->nblocks; b++)
+            for (int j = 0; j < p->blocks[b].npasses; j++) B->pass_h[ip++] = p->blocks[b].pass[j];
+        for (int b = 0; b < p->nblocks; b++)
+            for (int j = 0; j < p->blocks[b].ninv; j++) B->pass_h[ip++] = p->blocks[b].ipass[j];
         for (int b = 0; b < p->nblocks; b++) {
             const qgtl_block_t *blk = &p->blocks[b];
-            for (int j = 0; j < blk->npasses; j++) B->pass_h[ip++] = blk->pass[j];
+            const int backward = b > meet;
             const int K = blk->param_end - blk->param_begin;
             for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                 const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
@@ -149,11 +169,12 @@ static void fill_descs(struct qgtl_batch *B)
                 d->K = cnt;
                 for (int j = 0; j < cnt; j++) {
                     const qgtl_param_t *pp = &p->params[p->slot_param[blk->param_begin + j0 + j]];
-                    d->v[j] = pp->v;
-                    d->w[j] = pp->w;
+                    const qgtl_mat2 *M = backward ? &pp->B : &pp->A;
+                    d->v[j] = backward ? 1u << pp->qubit : pp->v;
+                    d->w[j] = backward ? 1u << pp->qubit : pp->w;
                     for (int e = 0; e < 4; e++) {
-                        d->A[j][e].re = (float)pp->A.m[e].re;
-                        d->A[j][e].im = (float)pp->A.m[e].im;
+                        d->A[j][e].re = (float)M->m[e].re;
+                        d->A[j][e].im = (float)M->m[e].im;
                     }
                 }
             }
@@ -213,6 +234,60 @@ static void gram_shape(const struct qgtl_batch *B, inst_t *in)
     in->T = (int)upper_tiles(in->nt);
 }
 
+static int block_flips(const qgtl_block_t *b, int inv)
+{
+    int f = 0;
+    const int np = inv ? b->ninv : b->npasses;
+    for (int j = 0; j < np; j++) f += inv ? b->ipass_flips[j] : b->pass_flips[j];
+    return f;
+}
+
+/* Arena (0: x, 1: y) of instance i's backward group before it is carried
+ * through block b (meet <= b <= nblocks - 2), i.e. where the states of block
+ * b + 1 are injected; final_arena for b = meet. */
+static int bwd_arena(const inst_t *in, int b)
+{
+    const qgtl_plan_t *p = in->plan;
+    int a = in->final_arena;
+    for (int b2 = in->meet + 1; b2 <= b; b2++) a ^= block_flips(&p->blocks[b2], 1) & 1;
+    return a;
+}
+
+static size_t arena_off(const inst_t *in, int a) { return a ? in->y_off : in->x_off; }
+
+typedef struct {
+    int psi;                   /* arena of the primal */
+    int bwd;                   /* arena of the backward group during the backward phase */
+    int fwd_slots;             /* forward derivative states created so far */
+    size_t pass_desc, inv_desc, inj_desc;
+} sched_t;
+
+/* Pass items of one launch: (instance, pass descriptor, first state slot,
+ * state count, source / destination arena); the arena flips when the pass
+ * stores to the other buffer. */
+static int add_pass_item(struct qgtl_batch *B, qgtl_pass_item_t *it, int *k, uint64_t *u, int i, size_t desc,
+                         int flip, int slot0, int nstates, int *arena)
+{
+    const inst_t *in = &B->inst[i];
+    if (flip && in->y_off == SIZE_MAX)
+        return qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: flip without a second buffer");
+    const qgtl_pass_desc_t *d = &B->pass_h[desc];
+    const size_t c64 = 8;
+    it[*k].desc = dev_addr(B->pass_d, desc * sizeof(qgtl_pass_desc_t));
+    it[*k].src = dev_addr(B->states, (arena_off(in, *arena) + (size_t)slot0 * in->N) * c64);
+    it[*k].dst = dev_addr(B->states, (arena_off(in, *arena ^ flip) + (size_t)slot0 * in->N) * c64);
+    it[*k].stride = in->N;
+    it[*k].ntiles = 1u << d->nrest;
+    it[*k].nstates = (uint32_t)nstates;
+    it[*k].unit_begin = (uint32_t)*u;
+    it[*k].variant = (uint32_t)regs_variant(d, B->tune.pass_variant);
+    *u += (uint64_t)it[*k].ntiles * (uint64_t)nstates;
+    B->sweep_bytes += (double)nstates * (double)in->N * 16.0;
+    *arena ^= flip;
+    (*k)++;
+    return 0;
+}
+
 static int build_schedule(struct qgtl_batch *B)
 {
     const int cnt = B->count;
@@ -221,27 +296,32 @@ static int build_schedule(struct qgtl_batch *B)
     B->units = 0;
     B->sweep_bytes = 0.0;
     B->gram_flops = 0.0;
-    size_t *cur = (size_t *)malloc(sizeof(size_t) * 2 * (size_t)cnt);
-    int *slot = (int *)calloc((size_t)cnt, sizeof(int));
-    size_t *pdesc = (size_t *)malloc(sizeof(size_t) * 2 * (size_t)cnt);   /* next pass / inject desc per instance */
+    sched_t *sc = (sched_t *)calloc((size_t)cnt, sizeof(sched_t));
     size_t bytes = sizeof(qgtl_pass_item_t) * (size_t)cnt * 2;
     if (bytes < sizeof(qgtl_inject_item_t) * (size_t)cnt * 8) bytes = sizeof(qgtl_inject_item_t) * (size_t)cnt * 8;
     void *tmp = malloc(bytes + sizeof(qgtl_gram_item_t) * (size_t)cnt * 2);
-    if (!cur || !slot || !pdesc || !tmp) {
-        free(cur); free(slot); free(pdesc); free(tmp);
+    if (!sc || !tmp) {
+        free(sc); free(tmp);
         return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
     }
     int rc = 0;
     size_t ip = 0, ii = 0;
-    int maxb = 0;
+    int maxb = 0, maxinv = 0;
     for (int i = 0; i < cnt; i++) {
-        cur[2 * i] = B->inst[i].x_off;
-        cur[2 * i + 1] = B->inst[i].y_off;
-        pdesc[2 * i] = ip;
-        pdesc[2 * i + 1] = ii;
-        ip += count_pass_descs(B->inst[i].plan);
-        ii += count_inj_descs(B->inst[i].plan);
-        if (B->inst[i].plan->nblocks > maxb) maxb = B->inst[i].plan->nblocks;
+        const qgtl_plan_t *p = B->inst[i].plan;
+        sc[i].psi = 0;
+        sc[i].pass_desc = ip;
+        size_t nf = 0;
+        for (int b = 0; b < p->nblocks; b++) {
+            nf += (size_t)p->blocks[b].npasses;
+            if (p->blocks[b].ninv > maxinv) maxinv = p->blocks[b].ninv;
+        }
+        sc[i].inv_desc = ip + nf;
+        sc[i].inj_desc = ii;
+        sc[i].bwd = B->inst[i].meet < p->nblocks - 1 ? bwd_arena(&B->inst[i], p->nblocks - 2) : B->inst[i].final_arena;
+        ip += count_pass_descs(p);
+        ii += count_inj_descs(p);
+        if (p->nblocks > maxb) maxb = p->nblocks;
     }
     const size_t c64 = 8;
 
@@ -250,7 +330,7 @@ static int build_schedule(struct qgtl_batch *B)
         qgtl_fill_item_t *it = (qgtl_fill_item_t *)tmp;
         uint64_t u = 0;
         for (int i = 0; i < cnt; i++) {
-            it[i].psi = dev_addr(B->states, cur[2 * i] * c64);
+            it[i].psi = dev_addr(B->states, B->inst[i].x_off * c64);
             it[i].N = (uint32_t)B->inst[i].N;
             it[i].unit_begin = (uint32_t)u;
             u += (B->inst[i].N + INJECT_TILE - 1) / INJECT_TILE;
@@ -258,97 +338,115 @@ static int build_schedule(struct qgtl_batch *B)
         rc = push_items(B, it, sizeof(*it) * (size_t)cnt, BL_FILL, 0, cnt, u);
         if (rc) goto out;
     }
+    /* forward phase */
     for (int b = 0; b < maxb; b++) {
-        int maxp = 0;
-        for (int i = 0; i < cnt; i++)
-            if (b < B->inst[i].plan->nblocks && B->inst[i].plan->blocks[b].npasses > maxp)
-                maxp = B->inst[i].plan->blocks[b].npasses;
-        for (int j = 0; j < maxp; j++) {
-          /* one launch per kernel variant (register-phase / LDS body) */
-          for (int var = 1; var >= 0; var--) {
-            qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
+        for (int dir = 1; dir >= 0; dir--) {
+            /* dir 1: backward injections at block b's input (before its passes);
+             * dir 0: the passes, then forward injections after block b */
+            if (dir == 0) {
+                int maxp = 0;
+                for (int i = 0; i < cnt; i++)
+                    if (b < B->inst[i].plan->nblocks && B->inst[i].plan->blocks[b].npasses > maxp)
+                        maxp = B->inst[i].plan->blocks[b].npasses;
+                for (int j = 0; j < maxp; j++) {
+                    /* one launch per kernel variant (register-phase / LDS body) */
+                    for (int var = 1; var >= 0; var--) {
+                        qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
+                        int k = 0;
+                        uint64_t u = 0;
+                        for (int i = 0; i < cnt; i++) {
+                            const inst_t *in = &B->inst[i];
+                            const qgtl_plan_t *p = in->plan;
+                            if (b >= p->nblocks || j >= p->blocks[b].npasses) continue;
+                            size_t o = 0;
+                            for (int b2 = 0; b2 < b; b2++) o += (size_t)p->blocks[b2].npasses;
+                            const size_t desc = sc[i].pass_desc + o + (size_t)j;
+                            if (regs_variant(&B->pass_h[desc], B->tune.pass_variant) != var) continue;
+                            const int nstates = b <= in->meet ? 1 + sc[i].fwd_slots : 1;
+                            rc = add_pass_item(B, it, &k, &u, i, desc, p->blocks[b].pass_flips[j], 0, nstates,
+                                               &sc[i].psi);
+                            if (rc) goto out;
+                        }
+                        rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, var, k, u);
+                        if (rc) goto out;
+                    }
+                }
+            }
+            qgtl_inject_item_t *it = (qgtl_inject_item_t *)tmp;
             int k = 0;
             uint64_t u = 0;
             for (int i = 0; i < cnt; i++) {
-                const qgtl_plan_t *p = B->inst[i].plan;
-                if (b >= p->nblocks || j >= p->blocks[b].npasses) continue;
-                {
-                    size_t o2 = 0;
-                    for (int b2 = 0; b2 < b; b2++) o2 += (size_t)p->blocks[b2].npasses;
-                    if (regs_variant(&B->pass_h[pdesc[2 * i] + o2 + (size_t)j], B->tune.pass_variant) != var) continue;
-                }
+                const inst_t *in = &B->inst[i];
+                const qgtl_plan_t *p = in->plan;
+                if (b >= p->nblocks || (dir == 1) != (b > in->meet)) continue;
                 const qgtl_block_t *blk = &p->blocks[b];
-                const size_t di = pdesc[2 * i];
-                size_t o = 0;
-                for (int b2 = 0; b2 < b; b2++) o += (size_t)p->blocks[b2].npasses;
-                const qgtl_pass_desc_t *d = &B->pass_h[di + o + (size_t)j];
-                const int flip = blk->pass_flips[j];
-                if (flip && B->inst[i].y_off == SIZE_MAX) {
-                    rc = qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: flip without a second buffer");
-                    goto out;
+                const int K = blk->param_end - blk->param_begin;
+                const size_t N = in->N;
+                /* forward states go where the primal is; the states of a later block where the
+                 * backward group will be when they join it */
+                const int dst_arena = dir ? bwd_arena(in, b - 1) : sc[i].psi;
+                for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
+                    const int c = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
+                    const size_t s0 = (size_t)(blk->param_begin + j0);
+                    if (k >= cnt * 8) {
+                        rc = qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "more than %d injection chunks in one block",
+                                            8 * cnt);
+                        goto out;
+                    }
+                    it[k].desc = dev_addr(B->inj_d, sc[i].inj_desc++ * sizeof(qgtl_inject_desc_t));
+                    it[k].psi = dev_addr(B->states, arena_off(in, sc[i].psi) * c64);
+                    it[k].dst = dev_addr(B->states, (arena_off(in, dst_arena) + (1 + s0) * N) * c64);
+                    it[k].partials = dev_addr(B->inj_partials, (in->part_off + s0 * (size_t)in->tiles * 4) *
+                                                                   sizeof(double));
+                    it[k].stride = N;
+                    it[k].tiles = (uint32_t)in->tiles;
+                    it[k].K = (uint32_t)c;
+                    it[k].unit_begin = (uint32_t)u;
+                    it[k].pad = 0;
+                    u += (uint64_t)it[k].tiles * (uint64_t)c;
+                    k++;
                 }
-                const uint32_t nstates = (uint32_t)(1 + slot[i]);
-                it[k].desc = dev_addr(B->pass_d, (di + o + (size_t)j) * sizeof(qgtl_pass_desc_t));
-                it[k].src = dev_addr(B->states, cur[2 * i] * c64);
-                it[k].dst = dev_addr(B->states, cur[2 * i + flip] * c64);
-                it[k].stride = B->inst[i].N;
-                it[k].ntiles = 1u << d->nrest;
-                it[k].nstates = nstates;
-                it[k].unit_begin = (uint32_t)u;
-                it[k].variant = (uint32_t)regs_variant(d, B->tune.pass_variant);
-                u += (uint64_t)it[k].ntiles * nstates;
-                B->sweep_bytes += (double)nstates * (double)B->inst[i].N * 16.0;
-                if (flip) {
-                    const size_t t = cur[2 * i];
-                    cur[2 * i] = cur[2 * i + 1];
-                    cur[2 * i + 1] = t;
-                }
-                k++;
+                if (!dir) sc[i].fwd_slots += K;
             }
-            rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, var, k, u);
+            rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_INJECT, 0, k, u);
             if (rc) goto out;
-          }
         }
-        /* injections of block b */
-        qgtl_inject_item_t *it = (qgtl_inject_item_t *)tmp;
-        int k = 0;
-        uint64_t u = 0;
-        for (int i = 0; i < cnt; i++) {
-            const qgtl_plan_t *p = B->inst[i].plan;
-            if (b >= p->nblocks) continue;
-            const qgtl_block_t *blk = &p->blocks[b];
-            const int K = blk->param_end - blk->param_begin;
-            const size_t N = B->inst[i].N;
-            for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
-                const int c = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
-                const size_t s0 = (size_t)(slot[i] + j0);
-                if (k >= cnt * 8) {
-                    rc = qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "more than %d injection chunks in one block",
-                                        8 * cnt);
-                    goto out;
-                }
-                it[k].desc = dev_addr(B->inj_d, pdesc[2 * i + 1]++ * sizeof(qgtl_inject_desc_t));
-                it[k].psi = dev_addr(B->states, cur[2 * i] * c64);
-                it[k].dst = dev_addr(B->states, (cur[2 * i] + (1 + s0) * N) * c64);
-                it[k].partials = dev_addr(B->inj_partials, (B->inst[i].part_off + s0 * (size_t)B->inst[i].tiles * 4) *
-                                                               sizeof(double));
-                it[k].stride = N;
-                it[k].tiles = (uint32_t)B->inst[i].tiles;
-                it[k].K = (uint32_t)c;
-                it[k].unit_begin = (uint32_t)u;
-                it[k].pad = 0;
-                u += (uint64_t)it[k].tiles * (uint64_t)c;
-                k++;
-            }
-            slot[i] += K;
-        }
-        rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_INJECT, 0, k, u);
-        if (rc) goto out;
     }
     for (int i = 0; i < cnt; i++)
-        if (slot[i] != B->inst[i].P) {
-            rc = qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: instance %d created %d of %d states", i,
-                                slot[i], B->inst[i].P);
+        if (sc[i].fwd_slots != B->inst[i].nfwd) {
+            rc = qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: instance %d created %d of %d forward states",
+                                i, sc[i].fwd_slots, B->inst[i].nfwd);
+            goto out;
+        }
+    /* backward phase: block b' descending carries the states of blocks > b' through its inverse */
+    for (int bb = maxb - 2; bb >= 0; bb--) {
+        for (int j = 0; j < maxinv; j++) {
+            for (int var = 1; var >= 0; var--) {
+                qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
+                int k = 0;
+                uint64_t u = 0;
+                for (int i = 0; i < cnt; i++) {
+                    const inst_t *in = &B->inst[i];
+                    const qgtl_plan_t *p = in->plan;
+                    if (bb <= in->meet || bb > p->nblocks - 2 || j >= p->blocks[bb].ninv) continue;
+                    size_t o = 0;
+                    for (int b2 = 0; b2 < bb; b2++) o += (size_t)p->blocks[b2].ninv;
+                    const size_t desc = sc[i].inv_desc + o + (size_t)j;
+                    if (regs_variant(&B->pass_h[desc], B->tune.pass_variant) != var) continue;
+                    const int first = p->blocks[bb + 1].param_begin;
+                    rc = add_pass_item(B, it, &k, &u, i, desc, p->blocks[bb].ipass_flips[j], 1 + first, in->P - first,
+                                       &sc[i].bwd);
+                    if (rc) goto out;
+                }
+                rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, var, k, u);
+                if (rc) goto out;
+            }
+        }
+    }
+    for (int i = 0; i < cnt; i++)
+        if (sc[i].bwd != B->inst[i].final_arena) {
+            rc = qgtl_set_error(QGT_ERROR_COMPUTATION_FAILED, "internal: instance %d backward states ended off their arena",
+                                i);
             goto out;
         }
     {   /* FP64 injection sums */
@@ -374,7 +472,7 @@ static int build_schedule(struct qgtl_batch *B)
         for (int i = 0; i < cnt; i++) {
             const inst_t *in = &B->inst[i];
             if (in->terms != term_classes[tc]) continue;
-            it[k].D = dev_addr(B->states, (cur[2 * i] + in->N) * c64);
+            it[k].D = dev_addr(B->states, (arena_off(in, in->final_arena) + in->N) * c64);
             it[k].partials = dev_addr(B->gram_partials, in->gp_off * sizeof(float));
             it[k].N = in->N;
             it[k].chunk_len = in->chunk_len;
@@ -423,7 +521,7 @@ static int build_schedule(struct qgtl_batch *B)
         if (rc) goto out;
     }
 out:
-    free(cur); free(slot); free(pdesc); free(tmp);
+    free(sc); free(tmp);
     return rc;
 }
 
@@ -561,7 +659,7 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
         qgtl_tune_t pt;
         memset(&pt, 0, sizeof pt);
         pt.sweep_form = B->tune.sweep_form == 1 ? 1 : 0;
-        pt.sweep_schedule = 1;      /* the batch sweeps breadth-first: no inverse blocks */
+        pt.sweep_schedule = B->tune.sweep_schedule == 1 ? 1 : 0;   /* 0: with inverse blocks (two-sided) */
         in->plan = qgtl_plan_create(n[i], nops[i], ops + op_off, QGTL_MODE_MIXED, &pt);
         if (!in->plan) {
             char msg[512];
@@ -589,7 +687,15 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
                 return fail_create(B);
             }
             for (int j = 0; j < p->blocks[b].npasses; j++) in->flips |= p->blocks[b].pass_flips[j];
+            for (int j = 0; j < p->blocks[b].ninv; j++) in->flips |= p->blocks[b].ipass_flips[j];
         }
+        /* the meeting block of the two-sided schedule; the forward group ends where the
+         * primal is after that block, and so must every backward-carried state */
+        in->meet = qgtl_plan_meet_block(in->plan);
+        in->nfwd = p->blocks[in->meet].param_end;
+        int fa = 0;
+        for (int b = 0; b <= in->meet; b++) fa ^= block_flips(&p->blocks[b], 0) & 1;
+        in->final_arena = fa;
         const size_t states = (size_t)(in->P + 1) * in->N;
         in->x_off = xo;
         xo += states;

@@ -330,6 +330,7 @@ void qgtl_plan_destroy(qgtl_plan_t *p)
 {
     QGTL_LOCKED();
     if (!p) return;
+    qgtl_hip_graph_destroy(p->graph);
     plan_free_device(p);
     free(p->inj_slots_host);
     qgtl_hip_event_destroy(p->ev_psi);
@@ -390,6 +391,29 @@ static int gram_chunks(const struct qgtl_plan *p, size_t free_bytes)
     return (int)want;
 }
 
+/* The same for the complex-form Gram of split-plane states (tile pairs of 128
+ * states, both parts of G per pair): chunks of ~2048 amplitudes, at least ~512
+ * workgroups (one per CU at a time), partials within the same budget. */
+static int gram_chunks_cplx(const struct qgtl_plan *p, size_t free_bytes)
+{
+    if (p->tune.gram_chunks > 0) return (p->tune.gram_chunks + 7) / 8 * 8;
+    const size_t per_chunk = qgtl_hip_gram_cplx_partial_floats(p->P + 1) * sizeof(float);
+    const size_t T = per_chunk / (2 * 128 * 128 * sizeof(float));
+    size_t want = p->N / 2048;
+    const size_t minw = (512 + T - 1) / T;
+    if (want < minw) want = minw;
+    size_t budget = ((size_t)p->P + 1) * p->N * 8 / 16;
+    if (budget > free_bytes / 2) budget = free_bytes / 2;
+    if (budget < ((size_t)1 << 30)) budget = (size_t)1 << 30;
+    const size_t cap = budget / per_chunk;
+    if (want > cap) want = cap;
+    const size_t maxc = p->N / 256;
+    if (want > maxc) want = maxc;
+    want = (want + 7) / 8 * 8;
+    if (want < 8) want = 8;
+    return (int)want;
+}
+
 /* Split-BF16x3 leaves a ~2.7e-6 bias (the dropped lo*lo product) and x4
  * a 2^-17 representation error that averages out only over unstructured
  * states; the default is the two-piece fp16 split (see qgtl_internal.h). */
@@ -398,6 +422,8 @@ static int gram_terms(const struct qgtl_plan *p)
     if (qgtl_valid_gram_terms(p->tune.gram_variant)) return p->tune.gram_variant;
     return qgtl_default_gram_terms(p->n);
 }
+
+static int gram_products(const struct qgtl_plan *p) { return gram_terms(p) == QGTL_GRAM_F16X4 ? 4 : 3; }
 
 static int alloc_mixed(struct qgtl_plan *p)
 {
@@ -419,11 +445,17 @@ static int alloc_mixed(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_malloc((void **)&p->inj_sums, P * 4 * sizeof(double)));
         size_t free_bytes = 0;
         HIPCALL(qgtl_hip_mem_free(&free_bytes));
+        /* partials and C sized for both Gram forms (the plan picks one at execute) */
         p->gram_chunks = gram_chunks(p, free_bytes);
+        p->gram_chunks_cplx = gram_chunks_cplx(p, free_bytes);
         const size_t T = (size_t)qgtl_hip_gram_tiles(p->P + 1);
-        HIPCALL(qgtl_hip_malloc((void **)&p->gram_partials, (size_t)p->gram_chunks * T * 128 * 128 * sizeof(float)));
+        size_t pbytes = (size_t)p->gram_chunks * T * 128 * 128 * sizeof(float);
+        const size_t pc = (size_t)p->gram_chunks_cplx * qgtl_hip_gram_cplx_partial_floats(p->P + 1) * sizeof(float);
+        if (pc > pbytes) pbytes = pc;
+        HIPCALL(qgtl_hip_malloc((void **)&p->gram_partials, pbytes));
         const size_t nt = (2 * (P + 1) + 127) / 128, R = nt * 128;
-        HIPCALL(qgtl_hip_malloc((void **)&p->C, R * R * sizeof(double)));
+        const size_t ntc = (P + 1 + 127) / 128, Rc = ntc * 128;
+        HIPCALL(qgtl_hip_malloc((void **)&p->C, (R * R > 2 * Rc * Rc ? R * R : 2 * Rc * Rc) * sizeof(double)));
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
         HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
     }
@@ -620,6 +652,8 @@ static int choose_meet(const struct qgtl_plan *p, int depth_first)
     return best;
 }
 
+int qgtl_plan_meet_block(const struct qgtl_plan *p) { return choose_meet(p, any_flip(p)); }
+
 /* Split-plane final states: depth-first plans in which every chunk that is
  * carried anywhere ends with a register-phase pass (the last pass of block
  * meet forward, the last inverse pass of block meet + 1 backward) and no
@@ -812,7 +846,10 @@ static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
     return 0;
 }
 
-static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
+/* Everything before the launches of a mixed execute: device buffers, phase
+ * tables and injection slots uploaded, the schedule chosen, the events of the
+ * two lanes made (none of it may happen under graph capture). */
+static int prepare_mixed(struct qgtl_plan *p)
 {
     int rc = alloc_mixed(p);
     if (rc) return rc;
@@ -822,21 +859,37 @@ static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
     p->meet = choose_meet(p, depth_first);
     p->presplit = presplit_ok(p, depth_first, gram_terms(p));
     p->psi_in_gram = depth_first && p->P > 0 && p->N >= 64;
-    const int lanes = depth_first && !g_profiling;
-    if (lanes && !p->ev_psi) {
+    if (depth_first && !p->ev_psi) {
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_psi));
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_inj));
     }
-    rc = run_mixed_lanes(p, st, lanes);
+    return 0;
+}
+
+/* The launches of a mixed execute (prepare_mixed done). */
+static int run_mixed(struct qgtl_plan *p, qgtl_stats_t *st)
+{
+    const int lanes = any_flip(p) && !g_profiling;
+    int rc = run_mixed_lanes(p, st, lanes);
     if (lanes) qgtl_hip_set_lane(0);
     if (rc) return rc;
     if (p->P == 0) return 0;
     const int Pg = p->P + (p->psi_in_gram ? 1 : 0);
-    LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, Pg, p->N, p->gram_partials, p->gram_chunks, gram_terms(p),
-                                             p->presplit));
-    LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, Pg, p->gram_chunks, p->C));
-    if (p->psi_in_gram) LAUNCH(K_FINAL, qgtl_hip_qgt_finalize_psi(p->C, p->P, p->Qdev));
-    else LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
+    /* split-plane states (the primal state among them): the complex-form Gram */
+    p->gram_cplx = p->presplit;
+    if (p->gram_cplx) {
+        /* stages of 32 amplitudes (two LDS buffers): at n = 20, P = 240 the Gram took
+         * 0.95-0.97 ms, with stages of 16 (four buffers) 0.98-1.02 ms */
+        LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_cplx(p->D, Pg, p->N, p->gram_partials, p->gram_chunks_cplx,
+                                               gram_products(p), 32));
+        LAUNCH(K_GRAM_AUX, qgtl_hip_gram_cplx_reduce(p->gram_partials, Pg, p->gram_chunks_cplx, p->C));
+        LAUNCH(K_FINAL, qgtl_hip_qgt_finalize_cplx(p->C, p->P, p->Qdev));
+    } else {
+        LAUNCH(K_GRAM_MAIN, qgtl_hip_gram_bf16x3(p->D, Pg, p->N, p->gram_partials, p->gram_chunks, gram_terms(p)));
+        LAUNCH(K_GRAM_AUX, qgtl_hip_gram_reduce(p->gram_partials, Pg, p->gram_chunks, p->C));
+        if (p->psi_in_gram) LAUNCH(K_FINAL, qgtl_hip_qgt_finalize_psi(p->C, p->P, p->Qdev));
+        else LAUNCH(K_FINAL, qgtl_hip_qgt_finalize(p->C, p->inj_sums, p->P, p->Qdev));
+    }
     if (st) {
         st->gram_launches += 2;
         st->gram_flops += 4.0 * (double)p->N * p->P * (p->P + 1);
@@ -938,7 +991,35 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
     st->n = p->n;
     const double t0 = now_ms();
     prof_reset();
-    rc = (p->mode == QGTL_MODE_MIXED) ? run_mixed(p, st) : run_plain(p, st);
+    if (p->mode == QGTL_MODE_MIXED) {
+        rc = prepare_mixed(p);
+        if (rc) return rc;
+        const int graphable = !g_profiling && !p->tune.no_graph && p->P > 0;
+        if (graphable && p->graph && p->graph_gen == p->build_gen) {
+            /* unchanged plan: replay the captured launch sequence */
+            rc = qgtl_hip_graph_launch(p->graph);
+            if (rc) return hip_fail(rc, "graph launch");
+            *st = p->graph_st;
+        } else if (graphable && p->exec_gen == p->build_gen) {
+            /* second execute of this build: capture the launches, then replay them */
+            HIPCALL(qgtl_hip_capture_begin());
+            rc = run_mixed(p, st);
+            if (rc) { qgtl_hip_capture_abort(); qgtl_hip_set_lane(0); return rc; }
+            rc = qgtl_hip_capture_end_update(&p->graph);
+            if (rc) return hip_fail(rc, "graph capture");
+            p->graph_gen = p->build_gen;
+            p->graph_st = *st;
+            p->graph_st.graph = 1;
+            rc = qgtl_hip_graph_launch(p->graph);
+            if (rc) return hip_fail(rc, "graph launch");
+            *st = p->graph_st;
+        } else {
+            rc = run_mixed(p, st);
+            p->exec_gen = p->build_gen;
+        }
+    } else {
+        rc = run_plain(p, st);
+    }
     if (rc) { qgtl_hip_sync(); prof_reset(); return rc; }
     st->presplit = p->mode == QGTL_MODE_MIXED ? p->presplit : 0;
     st->meet_block = p->mode == QGTL_MODE_MIXED ? p->meet : p->nblocks - 1;
@@ -1089,11 +1170,12 @@ int qgtl_plan_debug_buffers(qgtl_plan_t *p, double *C_out, double *inj_out, int 
     QGTL_LOCKED();
     if (!p || p->mode != QGTL_MODE_MIXED || !p->C) return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "no mixed-mode result");
     /* depth-first plans: C is the Gram of P + 1 states (the last one the primal
-     * state) and the injection sums are not computed */
+     * state) and the injection sums are not computed; after the complex-form
+     * Gram C holds G (R x R complex128, R_out = -R), else the real form (R x R) */
     const int Pg = p->P + (p->psi_in_gram ? 1 : 0);
-    const size_t R = (size_t)((2 * Pg + 127) / 128) * 128;
-    if (R_out) *R_out = (int)R;
-    if (C_out) HIPCALL(qgtl_hip_d2h(C_out, p->C, R * R * sizeof(double)));
+    const size_t R = p->gram_cplx ? (size_t)((Pg + 127) / 128) * 128 : (size_t)((2 * Pg + 127) / 128) * 128;
+    if (R_out) *R_out = p->gram_cplx ? -(int)R : (int)R;
+    if (C_out) HIPCALL(qgtl_hip_d2h(C_out, p->C, R * R * (p->gram_cplx ? 2 : 1) * sizeof(double)));
     if (inj_out) HIPCALL(qgtl_hip_d2h(inj_out, p->inj_sums, (size_t)p->P * 4 * sizeof(double)));
     HIPCALL(qgtl_hip_sync());
     return 0;
@@ -1111,6 +1193,25 @@ int qgtl_plan_overlaps(qgtl_plan_t *p, double *a)
     if (p->mode != QGTL_MODE_MIXED || !p->C || p->P < 1)
         return qgtl_set_error(QGT_ERROR_INVALID_STATE, "qgtl_plan_overlaps: no mixed-mode result");
     const size_t P = (size_t)p->P;
+    if (p->gram_cplx) {
+        /* complex Gram of P + 1 states: a_k = G_kP / (2 sqrt(G_kk G_PP)) */
+        const size_t R = ((P + 1 + 127) / 128) * 128;
+        double *G = (double *)malloc(R * R * 2 * sizeof(double));
+        if (!G) return qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
+        int rc = qgtl_hip_d2h(G, p->C, R * R * 2 * sizeof(double));
+        if (!rc) rc = qgtl_hip_sync();
+        if (rc) { free(G); return hip_fail(rc, "qgtl_plan_overlaps"); }
+        const double gmm = G[2 * (P * R + P)];
+        for (size_t s = 0; s < P; s++) {
+            const double gkk = G[2 * (s * R + s)];
+            const double sc = 0.5 / sqrt(gkk * gmm);
+            const size_t k = (size_t)p->slot_param[s];
+            a[2 * k] = G[2 * (s * R + P)] * sc;
+            a[2 * k + 1] = G[2 * (s * R + P) + 1] * sc;
+        }
+        free(G);
+        return 0;
+    }
     if (p->psi_in_gram) {
         const size_t R = ((2 * (P + 1) + 127) / 128) * 128;
         double *C = (double *)malloc(R * R * sizeof(double));

@@ -88,7 +88,9 @@ struct qgtl_plan {
     size_t inj_partials_len;
     double *inj_sums;                    /* P x 4 */
     float *gram_partials;
-    int gram_chunks;
+    int gram_chunks;           /* split-K chunks of the real-form Gram (complex64 states) */
+    int gram_chunks_cplx;      /* ... of the complex-form Gram (split-plane states) */
+    int gram_cplx;             /* the last execute ran the complex-form Gram: C holds G (R x R complex128) */
     double *C;                           /* mixed: R x R;  plain: G (P x P complex) */
     double *Qdev;                        /* P x P complex */
     double *Qhost;                       /* pinned host copy of Qdev (download staging) */
@@ -123,6 +125,12 @@ struct qgtl_plan {
     double *ng_A, *ng_D, *ng_g, *ng_x, *ng_d, *ng_U, *ng_theta, *ng_res;
     int *ng_it;
     int ng_rank;
+    /* the launch sequence as a HIP graph: build_gen counts plan builds (angles or
+     * tune changed); the graph is captured on the second execute of one build and
+     * replayed while the build is unchanged */
+    uint64_t build_gen, exec_gen, graph_gen;
+    qgtl_hip_graph_t graph;
+    qgtl_stats_t graph_st;               /* counts of the captured sequence */
 };
 
 /* plan construction (qgtl_plan.c) */
@@ -152,6 +160,10 @@ static inline void qgtl_unlock_scope(int *unused) { (void)unused; qgtl_unlock();
  * thread per plan -- a worker taking the lock the caller holds would wait for
  * the caller, which waits for the workers. */
 int qgtl_plan_set_angles_nolock(qgtl_plan_t *p, int nops, const double *angles);
+
+/* The block after which the derivative states of the plan meet on the two-sided
+ * schedule (the last block when every state is carried forward). */
+int qgtl_plan_meet_block(const struct qgtl_plan *p);
 
 
 

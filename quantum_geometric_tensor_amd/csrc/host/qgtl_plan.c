@@ -607,6 +607,7 @@ static qgtl_block_t *open_block(qgtl_block_t *blocks, int i, int n)
 int qgtl_plan_build(struct qgtl_plan *plan)
 {
     const int n = plan->n;
+    plan->build_gen++;                   /* any captured launch sequence is stale */
     if (n < 1 || n > QGTL_MAX_QUBITS)
         return qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "qubit count %d outside 1..%d", n, QGTL_MAX_QUBITS);
     /* validate ops and count parameters */

@@ -53,6 +53,34 @@ def test_batch_uneven_sizes_vs_oracle():
     b.close()
 
 
+def test_batch_two_sided_schedule():
+    """Instances above 13 qubits meet at a middle block: the states of later blocks are
+    injected at their block's input and carried backwards through inverse blocks, in grouped
+    launches over all instances, while smaller instances run forward only.  Q matches the
+    all-forward batch and the oracle with fewer sweep bytes, also after new angles."""
+    cir = _circuits([(12, 3), (14, 4), (15, 5), (16, 3), (14, 2), (13, 4), (15, 1)], 1234)
+    circuits = [(n, Q.hea_ops(n, L, th)) for n, L, th in cir]
+    two = Q.Batch(circuits)
+    fwd = Q.Batch(circuits, Q.BatchTune(sweep_schedule=1))
+    for rnd in range(2):
+        Qt, st_t = two.execute()
+        Qf, st_f = fwd.execute()
+        print(f"round {rnd}: sweep GB two-sided {st_t['sweep_bytes'] / 1e9:.2f}, forward {st_f['sweep_bytes'] / 1e9:.2f}; "
+              f"device ms {st_t['device_ms']:.2f} vs {st_f['device_ms']:.2f}", flush=True)
+        assert st_t["sweep_bytes"] < 0.7 * st_f["sweep_bytes"]
+        for (n, L, th), a, b in zip(cir, Qt, Qf):
+            ref = O.qgt(n, O.hea_ops(n, L, th))
+            ea, eb = Q.rel_err(a, ref), Q.rel_err(b, ref)
+            print(f"  n={n} L={L}: two-sided {ea:.2e}, forward {eb:.2e}", flush=True)
+            assert ea <= TOL and eb <= TOL
+        cir = [(n, L, O.uniform_angles(2000 + i, 2 * n * L)) for i, (n, L, _) in enumerate(cir)]
+        ang = [Q.hea_ops(n, L, th)["angle"] for n, L, th in cir]
+        two.set_angles(ang)
+        fwd.set_angles(ang)
+    two.close()
+    fwd.close()
+
+
 def test_batch_graph_replays_match_eager_and_new_angles():
     cir = _circuits([(4, 2), (10, 3), (13, 2), (14, 2), (15, 2)], 950)
     circuits = [(n, Q.hea_ops(n, L, th)) for n, L, th in cir]
