@@ -518,14 +518,47 @@ __device__ __forceinline__ void apply_phase_mat(float2 (&v)[kRegs], const qgtl_p
 // amplitudes, then their lo words (Re lo, Im lo) -- the pieces the Gram would
 // cut from the complex64 values, interleaved per amplitude as the Gram's
 // complex products read them.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+
+// 16-byte global load / store of two amplitudes; NT: non-temporal (streamed
+// through the caches: each state is read and written once per pass)
+template <bool NT>
+__device__ __forceinline__ float4 gload4(const float2 *p) {
+    if constexpr (NT) {
+        const f32x4_t q = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t *>(p));
+        return make_float4(q.x, q.y, q.z, q.w);
+    } else {
+        return *reinterpret_cast<const float4 *>(p);
+    }
+}
+
+template <bool NT>
+__device__ __forceinline__ void gstore4(float2 *p, float2 a, float2 b) {
+    if constexpr (NT) {
+        const f32x4_t q = {a.x, a.y, b.x, b.y};
+        __builtin_nontemporal_store(q, reinterpret_cast<f32x4_t *>(p));
+    } else {
+        *reinterpret_cast<float4 *>(p) = make_float4(a.x, a.y, b.x, b.y);
+    }
+}
+
+template <bool NT = false>
 __device__ __forceinline__ void write_split_pair(uint32_t *planes, size_t g, float2 a, float2 b) {
     uint32_t h0, l0, h1, l1;
     qgtl::split_f16_pair(a.x * qgtl::kSplitScale, a.y * qgtl::kSplitScale, h0, l0);
     qgtl::split_f16_pair(b.x * qgtl::kSplitScale, b.y * qgtl::kSplitScale, h1, l1);
     uint32_t *blk = planes + (g >> 6) * 128 + (g & 63);
-    *reinterpret_cast<uint2 *>(blk) = make_uint2(h0, h1);
-    *reinterpret_cast<uint2 *>(blk + 64) = make_uint2(l0, l1);
+    if constexpr (NT) {
+        const u32x2_t hi = {h0, h1}, lo = {l0, l1};
+        __builtin_nontemporal_store(hi, reinterpret_cast<u32x2_t *>(blk));
+        __builtin_nontemporal_store(lo, reinterpret_cast<u32x2_t *>(blk + 64));
+    } else {
+        *reinterpret_cast<uint2 *>(blk) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2 *>(blk + 64) = make_uint2(l0, l1);
+    }
 }
+
 
 template <int R>
 struct RegLoop {
@@ -554,16 +587,17 @@ struct RegLoop {
         v[R - 1] = cmul(v[R - 1], cmul(pt, make_float2(q.re, q.im)));
     }
     // global access in pairs (register bit 0 = tile bit 0)
+    template <bool NT = false>
     __device__ __forceinline__ static void load_global(const float2 *src, const GlobalMap &g, float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::load_global(src, g, v);
-        const float4 q = *reinterpret_cast<const float4 *>(src + global_index<R - 2>(g));
+        RegLoop<R - 2>::template load_global<NT>(src, g, v);
+        const float4 q = gload4<NT>(src + global_index<R - 2>(g));
         v[R - 2] = make_float2(q.x, q.y);
         v[R - 1] = make_float2(q.z, q.w);
     }
+    template <bool NT = false>
     __device__ __forceinline__ static void store_global(float2 *dst, const GlobalMap &g, const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global(dst, g, v);
-        *reinterpret_cast<float4 *>(dst + global_index<R - 2>(g)) =
-            make_float4(v[R - 2].x, v[R - 2].y, v[R - 1].x, v[R - 1].y);
+        RegLoop<R - 2>::template store_global<NT>(dst, g, v);
+        gstore4<NT>(dst + global_index<R - 2>(g), v[R - 2], v[R - 1]);
     }
     // derivative state formed while loading: element y of the state is
     // M[s][s] psi[y] + M[s][1-s] psi[y ^ v], s = parity(y & w), the same
@@ -597,10 +631,11 @@ struct RegLoop {
         v[R - 1] = cmadd(make_float2(0.f, 0.f), s1 ? m11 : m00, make_float2(qa.z, qa.w));
     }
     // the same stores writing the split-plane form
+    template <bool NT = false>
     __device__ __forceinline__ static void store_global_split(uint32_t *planes, const GlobalMap &g,
                                                               const float2 (&v)[kRegs]) {
-        RegLoop<R - 2>::store_global_split(planes, g, v);
-        write_split_pair(planes, global_index<R - 2>(g), v[R - 2], v[R - 1]);
+        RegLoop<R - 2>::template store_global_split<NT>(planes, g, v);
+        write_split_pair<NT>(planes, global_index<R - 2>(g), v[R - 2], v[R - 1]);
     }
 };
 
@@ -612,12 +647,15 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void load_lds_ring(const float2 *, const RingLds &, float2 (&)[kRegs]) {}
     template <bool POST>
     __device__ __forceinline__ static void phase_split(float2, const qgtl_pass_desc_t &, float2 (&)[kRegs]) {}
+    template <bool NT = false>
     __device__ __forceinline__ static void load_global(const float2 *, const GlobalMap &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global_inject(const float2 *, const GlobalMap &,
                                                               const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global_inject_diag(const float2 *, const GlobalMap &,
                                                                    const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
+    template <bool NT = false>
     __device__ __forceinline__ static void store_global(float2 *, const GlobalMap &, const float2 (&)[kRegs]) {}
+    template <bool NT = false>
     __device__ __forceinline__ static void store_global_split(uint32_t *, const GlobalMap &, const float2 (&)[kRegs]) {}
 };
 
@@ -694,7 +732,7 @@ __device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_
 // through m.gin): the inverse ring's in-tile part, the phase layers and gate
 // phases, the ring or natural store.  SPLIT: dst receives the split-plane
 // form of the result (a final pass).
-template <bool SPLIT>
+template <bool SPLIT, bool NT = false>
 __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], const TileMaps &m,
                                                    const RegLayout &L0, float2 *dst, float2 *tile,
                                                    const float2 *post_tab, const float2 *pre_tab) {
@@ -754,14 +792,14 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         RegLoop<kRegs>::load_lds(tile, L0, v);
     }
     if constexpr (SPLIT)
-        RegLoop<kRegs>::store_global_split(reinterpret_cast<uint32_t *>(dst), m.gout, v);
+        RegLoop<kRegs>::template store_global_split<NT>(reinterpret_cast<uint32_t *>(dst), m.gout, v);
     else
-        RegLoop<kRegs>::store_global(dst, m.gout, v);
+        RegLoop<kRegs>::template store_global<NT>(dst, m.gout, v);
 }
 
 // One tile r of one state.  inj != nullptr: src is the primal state the
 // derivative state *inj is injected from while loading.
-template <bool SPLIT = false>
+template <bool SPLIT = false, bool NT = false>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
                                                float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr,
@@ -772,11 +810,11 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     float2 v[kRegs];
     if (inj && inj->v == 0) RegLoop<kRegs>::load_global_inject_diag(src, m.gin, *inj, v);
     else if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
-    else RegLoop<kRegs>::load_global(src, m.gin, v);
-    pass_compute_store<SPLIT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
+    else RegLoop<kRegs>::template load_global<NT>(src, m.gin, v);
+    pass_compute_store<SPLIT, NT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
 }
 
-template <bool SPLIT>
+template <bool SPLIT, bool NT = false>
 __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
                                                                  const float2 *__restrict__ src_all,
                                                                  float2 *dst_all, size_t stride,
@@ -784,8 +822,8 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
     __shared__ float2 tile[kRegTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
-    pass_regs_body<SPLIT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
-                          blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
+    pass_regs_body<SPLIT, NT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
+                              blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
 }
 
 // The first pass of a derivative-state chunk with the injection folded into
@@ -819,7 +857,8 @@ __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pa
         t = b / nstates;
     }
     const qgtl_inject_slot_t q = inj[s];
-    pass_regs_body<SPLIT>(d, psi, dst_all + (size_t)s * stride, t, tile, hi_off, tabs, tabs + kPhaseTab, gtabs, &q);
+    pass_regs_body<SPLIT, true>(d, psi, dst_all + (size_t)s * stride, t, tile, hi_off, tabs, tabs + kPhaseTab, gtabs,
+                                &q);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -868,9 +907,9 @@ __device__ __forceinline__ void inject_body(const qgtl_inject_desc_t &d, const f
             if constexpr (!STORE) {
                 // sums only: the state itself is formed by the first pass that reads it
             } else if constexpr (SPLIT) {
-                write_split_pair(reinterpret_cast<uint32_t *>(dst), y, o0, o1);
+                write_split_pair<true>(reinterpret_cast<uint32_t *>(dst), y, o0, o1);
             } else {
-                *reinterpret_cast<float4 *>(dst + y) = make_float4(o0.x, o0.y, o1.x, o1.y);
+                gstore4<true>(dst + y, o0, o1);
             }
             if constexpr (SUMS) {
                 const double dr0 = o0.x, di0 = o0.y, pr0 = a0.x, pi0 = a0.y;
@@ -1106,7 +1145,7 @@ __global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pa
         __syncthreads();
         const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
         float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
-        if (VARIANT) pass_regs_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        if (VARIANT) pass_regs_body<false, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
         else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
     }
 }
@@ -1253,11 +1292,12 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
     if (split_out && (!regs || src == dst || stride < ((size_t)1 << desc->n)))
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: split-plane output needs a register pass, not in place");
     const float2 *gt = (const float2 *)tabs;
+    // non-temporal loads and stores: every tile is read once and written once per pass
     if (regs && split_out)
-        hipLaunchKernelGGL(pass_c64_regs_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+        hipLaunchKernelGGL((pass_c64_regs_kernel<true, true>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride, gt);
     else if (regs)
-        hipLaunchKernelGGL(pass_c64_regs_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
+        hipLaunchKernelGGL((pass_c64_regs_kernel<false, true>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride, gt);
     else
         hipLaunchKernelGGL(pass_c64_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
