@@ -356,9 +356,17 @@ class Batch:
             raise ValueError("one angle per op of every instance")
         _check(_batch_lib().qgtl_batch_set_angles(self._h, _p(a)))
 
-    def execute(self, want_q=True):
+    def execute(self, want_q=True, out=None):
+        """Run the batch; returns ([Q_i], stats).  out: a complex128 array of q_size
+        elements to write the matrices into (reused across calls instead of a new
+        array per call); the returned Q_i are views of it."""
         st = BatchStats()
-        buf = np.zeros(self.q_size, np.complex128) if want_q else None
+        if want_q and out is not None:
+            if out.dtype != np.complex128 or out.size != self.q_size or not out.flags.c_contiguous:
+                raise ValueError("out must be a contiguous complex128 array of q_size elements")
+            buf = out.reshape(-1)
+        else:
+            buf = np.zeros(self.q_size, np.complex128) if want_q else None
         _check(_batch_lib().qgtl_batch_execute(self._h, _p(buf) if want_q else None, ctypes.byref(st)))
         Qs = None
         if want_q:

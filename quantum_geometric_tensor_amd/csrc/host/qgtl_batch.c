@@ -78,6 +78,7 @@ typedef struct {
     int meet;                  /* states meet after this block (the last one: all carried forward) */
     int nfwd;                  /* derivative states of blocks <= meet (slots 1 .. nfwd) */
     int final_arena;           /* 0: x, 1: y -- where all derivative states end */
+    size_t pass_off, inj_off;  /* the instance's first pass / injection descriptor */
 } inst_t;
 
 struct qgtl_batch {
@@ -147,39 +148,46 @@ static size_t count_inj_descs(const qgtl_plan_t *p)
  * forward (A after the block) for blocks <= meet, backward (B at the block's
  * input, on the parameter's qubit) for later blocks.  The schedule walks the
  * same order. */
-static void fill_descs(struct qgtl_batch *B)
+/* the descriptors of instance i from its plan, at its fixed offsets (so that
+ * instances can be filled in parallel) */
+static void fill_inst_descs(struct qgtl_batch *B, int i)
 {
-    size_t ip = 0, ii = 0;
-    for (int i = 0; i < B->count; i++) {
-        const qgtl_plan_t *p = B->inst[i].plan;
-        const int meet = B->inst[i].meet;
-        for (int b = 0; b < p->nblocks; b++)
-            for (int j = 0; j < p->blocks[b].npasses; j++) B->pass_h[ip++] = p->blocks[b].pass[j];
-        for (int b = 0; b < p->nblocks; b++)
-            for (int j = 0; j < p->blocks[b].ninv; j++) B->pass_h[ip++] = p->blocks[b].ipass[j];
-        for (int b = 0; b < p->nblocks; b++) {
-            const qgtl_block_t *blk = &p->blocks[b];
-            const int backward = b > meet;
-            const int K = blk->param_end - blk->param_begin;
-            for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
-                const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
-                qgtl_inject_desc_t *d = &B->inj_h[ii++];
-                memset(d, 0, sizeof *d);
-                d->n = p->n;
-                d->K = cnt;
-                for (int j = 0; j < cnt; j++) {
-                    const qgtl_param_t *pp = &p->params[p->slot_param[blk->param_begin + j0 + j]];
-                    const qgtl_mat2 *M = backward ? &pp->B : &pp->A;
-                    d->v[j] = backward ? 1u << pp->qubit : pp->v;
-                    d->w[j] = backward ? 1u << pp->qubit : pp->w;
-                    for (int e = 0; e < 4; e++) {
-                        d->A[j][e].re = (float)M->m[e].re;
-                        d->A[j][e].im = (float)M->m[e].im;
-                    }
+    const inst_t *in = &B->inst[i];
+    const qgtl_plan_t *p = in->plan;
+    const int meet = in->meet;
+    size_t ip = in->pass_off, ii = in->inj_off;
+    for (int b = 0; b < p->nblocks; b++)
+        for (int j = 0; j < p->blocks[b].npasses; j++) B->pass_h[ip++] = p->blocks[b].pass[j];
+    for (int b = 0; b < p->nblocks; b++)
+        for (int j = 0; j < p->blocks[b].ninv; j++) B->pass_h[ip++] = p->blocks[b].ipass[j];
+    for (int b = 0; b < p->nblocks; b++) {
+        const qgtl_block_t *blk = &p->blocks[b];
+        const int backward = b > meet;
+        const int K = blk->param_end - blk->param_begin;
+        for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
+            const int cnt = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
+            qgtl_inject_desc_t *d = &B->inj_h[ii++];
+            memset(d, 0, sizeof *d);
+            d->n = p->n;
+            d->K = cnt;
+            for (int j = 0; j < cnt; j++) {
+                const qgtl_param_t *pp = &p->params[p->slot_param[blk->param_begin + j0 + j]];
+                const qgtl_mat2 *M = backward ? &pp->B : &pp->A;
+                d->v[j] = backward ? 1u << pp->qubit : pp->v;
+                d->w[j] = backward ? 1u << pp->qubit : pp->w;
+                for (int e = 0; e < 4; e++) {
+                    d->A[j][e].re = (float)M->m[e].re;
+                    d->A[j][e].im = (float)M->m[e].im;
                 }
             }
         }
     }
+}
+
+static void fill_descs(struct qgtl_batch *B)
+{
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < B->count; i++) fill_inst_descs(B, i);
     B->descs_dirty = 1;
 }
 
@@ -620,8 +628,8 @@ void qgtl_batch_destroy(qgtl_batch_t *B)
     qgtl_hip_free(B->inj_d);
     qgtl_hip_free(B->items_d);
     free(B->inst);
-    free(B->pass_h);
-    free(B->inj_h);
+    qgtl_hip_host_free(B->pass_h);
+    qgtl_hip_host_free(B->inj_h);
     free(B->items_h);
     free(B->launches);
     qgtl_hip_host_free(B->qhost);
@@ -713,18 +721,17 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
         in->q_off = qo;
         qo += (size_t)in->P * (size_t)in->P;
         in->terms = qgtl_default_gram_terms(n[i]);
+        in->pass_off = B->npass;
+        in->inj_off = B->ninj;
         B->npass += count_pass_descs(p);
         B->ninj += count_inj_descs(p);
     }
     B->q_len = qo;
     if (qgtl_init_if_needed()) return fail_create(B);
     B->cus = qgtl_hip_cu_count();
-    B->pass_h = (qgtl_pass_desc_t *)calloc(B->npass ? B->npass : 1, sizeof(qgtl_pass_desc_t));
-    B->inj_h = (qgtl_inject_desc_t *)calloc(B->ninj ? B->ninj : 1, sizeof(qgtl_inject_desc_t));
-    if (!B->pass_h || !B->inj_h) {
-        qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory");
-        return fail_create(B);
-    }
+    /* pinned host mirrors: the per-execution uploads of the angle-dependent descriptors are direct DMA */
+    CTRY(qgtl_hip_host_alloc((void **)&B->pass_h, (B->npass ? B->npass : 1) * sizeof(qgtl_pass_desc_t)));
+    CTRY(qgtl_hip_host_alloc((void **)&B->inj_h, (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t)));
     CTRY(qgtl_hip_host_alloc((void **)&B->qhost, qo * 2 * sizeof(double)));
     CTRY(qgtl_hip_malloc(&B->states, xo * 8));
     CTRY(qgtl_hip_malloc((void **)&B->inj_partials, po * sizeof(double)));
@@ -769,14 +776,18 @@ int qgtl_batch_set_angles(qgtl_batch_t *B, const double *angles)
 #pragma omp parallel for schedule(dynamic, 4) reduction(min : first_bad)
     for (int i = 0; i < B->count; i++) {
         const inst_t *in = &B->inst[i];
-        if (qgtl_plan_set_angles_nolock(in->plan, in->nops, angles + in->op_off) != 0 && i < first_bad) first_bad = i;
+        if (qgtl_plan_set_angles_nolock(in->plan, in->nops, angles + in->op_off) != 0) {
+            if (i < first_bad) first_bad = i;
+        } else {
+            fill_inst_descs(B, i);
+        }
     }
+    B->descs_dirty = 1;
     if (first_bad < B->count) {
         /* again on this thread, for the error message */
         const inst_t *in = &B->inst[first_bad];
         return qgtl_plan_set_angles_nolock(in->plan, in->nops, angles + in->op_off);
     }
-    fill_descs(B);
     return 0;
 }
 
@@ -852,12 +863,21 @@ int qgtl_batch_execute(qgtl_batch_t *B, double *Q, qgtl_batch_stats_t *st)
     if (Q) {
         BTRY(qgtl_hip_d2h(B->qhost, B->Qdev, B->q_len * 2 * sizeof(double)));
         BTRY(qgtl_hip_sync());
+        /* slot order (creation order) -> parameter numbers, per instance on the host cores
+         * (a plain copy where the two orders agree) */
+#pragma omp parallel for schedule(dynamic, 8)
         for (int i = 0; i < B->count; i++) {
             const inst_t *in = &B->inst[i];
             const int *sp = in->plan->slot_param;
             const size_t P = (size_t)in->P;
             const double *src = B->qhost + 2 * in->q_off;
             double *dst = Q + 2 * in->q_off;
+            int identity = 1;
+            for (size_t a = 0; a < P && identity; a++) identity = sp[a] == (int)a;
+            if (identity) {
+                memcpy(dst, src, P * P * 2 * sizeof(double));
+                continue;
+            }
             for (size_t a = 0; a < P; a++)
                 for (size_t c = 0; c < P; c++) {
                     const size_t o = 2 * ((size_t)sp[a] * P + (size_t)sp[c]);

@@ -738,6 +738,10 @@ int qgtl_plan_build(struct qgtl_plan *plan)
                 finish_rotation(&b->ipass[i], uadj);
             }
         }
+        /* images of the basis vectors under the inverse M-stage (the Z-frame masks below) */
+        uint32_t inv_img[QGTL_MAX_QUBITS];
+        if (b->param_end > b->param_begin)
+            for (int j2 = 0; j2 < n; j2++) inv_img[j2] = perm_apply_inverse(b, 1u << j2);
         for (int j = b->param_begin; j < b->param_end; j++) {
             qgtl_param_t *pp = &plan->params[order[j]];
             const int q = pp->qubit;
@@ -752,7 +756,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
             pp->v = perm_apply(b, 1u << q);
             uint32_t w = 0;
             for (int j2 = 0; j2 < n; j2++)
-                if ((perm_apply_inverse(b, 1u << j2) >> q) & 1u) w |= 1u << j2;
+                if ((inv_img[j2] >> q) & 1u) w |= 1u << j2;
             pp->w = w;
         }
     }
