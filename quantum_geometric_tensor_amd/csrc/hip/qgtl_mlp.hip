@@ -142,8 +142,9 @@ __global__ void to_bf16_kernel(const float *__restrict__ w, uint16_t *out, size_
         out[i] = bf16_rn(w[i]);
 }
 
-constexpr int kMaxRank = 16;
+constexpr int kMaxRank = 32;
 constexpr int kMaxP = 1024;
+constexpr int kMaxPR = 8192;    // P * r: the low-rank factor held in LDS as the network's fp32 outputs (32 KB)
 
 __device__ double block_sum(double v, double *red) {
 #pragma unroll
@@ -156,9 +157,41 @@ __device__ double block_sum(double v, double *red) {
     return s;
 }
 
+// Columns jb .. jb + 7 of the low-rank factor are handled together, in registers (compile-time indices).
+constexpr int kJB = 8;
+
+// (A_s u)_k,j for row k and the column block j = jb + i, i < kJB (zero past r), FP64; u_lj = yu[l r + j] / sqrt(P)
+__device__ __forceinline__ void row_times_u(const float *__restrict__ As, const float *yu, int P, int r, int k, int jb,
+                                            double us, double (&acc)[kJB]) {
+#pragma unroll
+    for (int i = 0; i < kJB; i++) acc[i] = 0.0;
+    for (int l = 0; l < P; l++) {
+        const double a = As[(size_t)k * P + l];
+#pragma unroll
+        for (int i = 0; i < kJB; i++)
+            if (jb + i < r) acc[i] += a * ((double)yu[l * r + jb + i] * us);
+    }
+}
+
+// (u W^-1)_k,j for the same column block
+__device__ __forceinline__ void row_times_winv(const float *yu, const double *Wi, int r, int k, int jb, double us,
+                                               double (&uw)[kJB]) {
+#pragma unroll
+    for (int i = 0; i < kJB; i++) uw[i] = 0.0;
+    for (int a = 0; a < r; a++) {
+        const double ua = (double)yu[k * r + a] * us;
+#pragma unroll
+        for (int i = 0; i < kJB; i++)
+            if (jb + i < r) uw[i] += ua * Wi[a * r + jb + i];
+    }
+}
+
 // Kaporin loss of sample b and dL/dy (scaled by `scale`), FP64.
 //   s_k = y[k(1+r)], u_kj = y[k(1+r)+1+j] / sqrt(P), Mt = diag(e^s) + u u^T,
 //   L = P log(tr(Mt As)/P) - sum s - log det(I + u^T e^-s u).
+// The low-rank outputs stay fp32 in LDS (each use scales them by 1/sqrt(P) in FP64, the value an FP64 copy
+// of u would hold).  A_s u is formed twice, for the trace and again for the gradient, eight columns at a
+// time in registers, rather than held (P r doubles do not fit beside u at rank 32).
 __global__ __launch_bounds__(kThreads) void kaporin_kernel(int P, int r, const float *__restrict__ Y,
                                                            const float *__restrict__ As_all, const int *__restrict__ idx,
                                                            double *loss, float *dY, float scale) {
@@ -166,29 +199,25 @@ __global__ __launch_bounds__(kThreads) void kaporin_kernel(int P, int r, const f
     const float *y = Y + (size_t)b * P * (1 + r);
     const float *As = As_all + (size_t)idx[b] * P * P;
     float *dy = dY + (size_t)b * P * (1 + r);
-    __shared__ double u[kMaxP * kMaxRank / 4];    // P * r <= 4096
-    __shared__ double Au[kMaxP * kMaxRank / 4];
+    __shared__ float yu[kMaxPR];
     __shared__ double W[kMaxRank * kMaxRank], Wi[kMaxRank * kMaxRank];
     __shared__ double red[kThreads / 64];
     __shared__ double sh_logdet;
     const double us = 1.0 / sqrt((double)P);
     const int st = 1 + r;
-    for (int e = threadIdx.x; e < P * r; e += kThreads) u[e] = (double)y[(size_t)(e / r) * st + 1 + e % r] * us;
+    for (int e = threadIdx.x; e < P * r; e += kThreads) yu[e] = y[(size_t)(e / r) * st + 1 + e % r];
     __syncthreads();
     double trp = 0.0, ssum = 0.0;
     for (int k = threadIdx.x; k < P; k += kThreads) {
         const double s = y[(size_t)k * st];
         ssum += s;
         trp += exp(s) * (double)As[(size_t)k * P + k];
-        double acc[kMaxRank];
-        for (int j = 0; j < r; j++) acc[j] = 0.0;
-        for (int l = 0; l < P; l++) {
-            const double a = As[(size_t)k * P + l];
-            for (int j = 0; j < r; j++) acc[j] += a * u[l * r + j];
-        }
-        for (int j = 0; j < r; j++) {
-            Au[k * r + j] = acc[j];
-            trp += u[k * r + j] * acc[j];
+        for (int jb = 0; jb < r; jb += kJB) {
+            double acc[kJB];
+            row_times_u(As, yu, P, r, k, jb, us, acc);
+#pragma unroll
+            for (int i = 0; i < kJB; i++)
+                if (jb + i < r) trp += ((double)yu[k * r + jb + i] * us) * acc[i];
         }
     }
     const double tr = block_sum(trp, red);
@@ -196,7 +225,8 @@ __global__ __launch_bounds__(kThreads) void kaporin_kernel(int P, int r, const f
     for (int e = threadIdx.x; e < r * r; e += kThreads) {
         const int a = e / r, c = e % r;
         double v = (a == c) ? 1.0 : 0.0;
-        for (int k = 0; k < P; k++) v += exp(-(double)y[(size_t)k * st]) * u[k * r + a] * u[k * r + c];
+        for (int k = 0; k < P; k++)
+            v += exp(-(double)y[(size_t)k * st]) * ((double)yu[k * r + a] * us) * ((double)yu[k * r + c] * us);
         W[e] = v;
     }
     __syncthreads();
@@ -235,16 +265,23 @@ __global__ __launch_bounds__(kThreads) void kaporin_kernel(int P, int r, const f
     for (int k = threadIdx.x; k < P; k += kThreads) {
         const double s = y[(size_t)k * st], ems = exp(-s);
         double q = 0.0;
-        double uw[kMaxRank];
-        for (int j = 0; j < r; j++) {
-            double t = 0.0;
-            for (int a = 0; a < r; a++) t += u[k * r + a] * Wi[a * r + j];
-            uw[j] = t;
-            q += t * u[k * r + j];
+        for (int jb = 0; jb < r; jb += kJB) {
+            double uw[kJB];
+            row_times_winv(yu, Wi, r, k, jb, us, uw);
+#pragma unroll
+            for (int i = 0; i < kJB; i++)
+                if (jb + i < r) q += uw[i] * ((double)yu[k * r + jb + i] * us);
         }
         dy[(size_t)k * st] = (float)(scale * (g * exp(s) * (double)As[(size_t)k * P + k] - 1.0 + ems * q));
-        for (int j = 0; j < r; j++)
-            dy[(size_t)k * st + 1 + j] = (float)(scale * (2.0 * g * Au[k * r + j] - 2.0 * ems * uw[j]) * us);
+        for (int jb = 0; jb < r; jb += kJB) {
+            double acc[kJB], uw[kJB];
+            row_times_u(As, yu, P, r, k, jb, us, acc);
+            row_times_winv(yu, Wi, r, k, jb, us, uw);
+#pragma unroll
+            for (int i = 0; i < kJB; i++)
+                if (jb + i < r)
+                    dy[(size_t)k * st + 1 + jb + i] = (float)(scale * (2.0 * g * acc[i] - 2.0 * ems * uw[i]) * us);
+        }
     }
 }
 
@@ -312,8 +349,8 @@ int qgtl_hip_to_bf16(const float *w, uint16_t *out, size_t n) {
 
 int qgtl_hip_kaporin(int P, int r, int B, const float *Y, const float *As_all, const int *idx, double *loss,
                      float *dY, float scale) {
-    if (P < 1 || P > kMaxP || r < 0 || r > kMaxRank || P * r > kMaxP * kMaxRank / 4 || B < 1)
-        return qgtl::set_error_msg(-118, "kaporin: needs P <= 1024, r <= 16, P r <= 4096");
+    if (P < 1 || P > kMaxP || r < 0 || r > kMaxRank || P * r > kMaxPR || B < 1)
+        return qgtl::set_error_msg(-118, "kaporin: needs P <= 1024, r <= 32, P r <= 8192");
     hipLaunchKernelGGL(kaporin_kernel, dim3((unsigned)B), dim3(kThreads), 0, qgtl::stream(), P, r, Y, As_all, idx,
                        loss, dY, scale);
     QGTL_LAUNCH_CHECK("kaporin");

@@ -116,8 +116,9 @@ qgtl_mlp_t *qgtl_mlp_create(int in_dim, int hidden, int hidden_layers, int P, in
 {
     QGTL_LOCKED();
     if (in_dim < 1 || hidden < 1 || hidden_layers < 1 || hidden_layers >= MLP_MAX_LAYERS || P < 1 || P > 1024 ||
-        rank < 0 || rank > 16 || P * rank > 4096) {
-        qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT, "mlp: bad shape (hidden layers 1..7, P <= 1024, rank <= 16)");
+        rank < 0 || rank > 32 || P * rank > 8192) {
+        qgtl_set_error(QGT_ERROR_INVALID_ARGUMENT,
+                       "mlp: bad shape (hidden layers 1..7, P <= 1024, rank <= 32, P x rank <= 8192)");
         return NULL;
     }
     if (qgtl_init_if_needed()) return NULL;

@@ -66,8 +66,11 @@ def test_mlp_forward_matches_oracle():
     m.close()
 
 
-def test_mlp_loss_and_gradient_match_oracle():
-    m = make_net(seed=5)
+# gradient bars about 3x the measured normwise errors (3.3e-3 at rank 2, 1.1e-2 at rank 32: the network's
+# activations enter each GEMM as bf16, the oracle keeps them in FP64)
+@pytest.mark.parametrize("P,r,gbar", [(6, 2, 1e-2), (40, 32, 3.5e-2)])
+def test_mlp_loss_and_gradient_match_oracle(P, r, gbar):
+    m = make_net(seed=5, P=P, r=r)
     S = 20
     rng = np.random.default_rng(1)
     X = rng.normal(size=(S, m.in_dim)).astype(np.float32)
@@ -81,7 +84,7 @@ def test_mlp_loss_and_gradient_match_oracle():
     cos = float(np.dot(g, go) / (np.linalg.norm(g) * np.linalg.norm(go)))
     print(f"loss {loss:.6f} oracle {lo:.6f}; grad normwise err {gerr:.2e} cos {cos:.6f}")
     assert abs(loss - lo) <= 1e-2 * max(1.0, abs(lo))
-    assert gerr <= 1e-2 and cos >= 0.9999
+    assert gerr <= gbar and cos >= 0.9999
     m.close()
 
 
