@@ -25,7 +25,9 @@
 // load's latency), one barrier per stage and no fence on it.  The MFMA
 // accumulators are added into an fp32 outer sum every 256 amplitudes, so
 // rounding does not grow with the chunk length; the chunks' fp32 partials are
-// summed in FP64 in a fixed order by the reduce kernel.
+// summed in FP64 in a fixed order by the reduce kernel.  For 129 .. 256 states
+// (two tiles) one workgroup per chunk computes all three tile pairs instead
+// (gram_cplx256_kernel below), reading each row once.
 //
 // LDS row of a state and stage: 8 slots of 16 bytes, slots 0..3 the hi words of
 // the 16 amplitudes (4 per slot), 4..7 the lo words, stored swizzled (swz).
@@ -339,38 +341,232 @@ __global__ __launch_bounds__(kThreads, 1) void gram_cplx_kernel(const uint8_t *_
         gram_cplx_unit<SA, PT, true, 0>(D, P, N, T, tile, chunk, ti, tj, chunk_len, w, partials, lds);
 }
 
-// G[ti 128 + r][tj 128 + c] = sum over chunks (fixed order, FP64) of the
+// ---------------------------------------------------------------------------
+// Whole-chunk form for 129 .. 256 states (two 128-state tiles): one workgroup
+// of 12 waves per split-K chunk computes all three tile pairs, i.e. the 36
+// block pairs (a, b) of 32-state blocks a <= b of the 256-state upper triangle,
+// from one stage buffer holding both tiles -- every row of the chunk is read
+// from HBM once (the per-tile-pair form reads each row tile twice, once per
+// tile pair that holds it).  Wave w multiplies three A blocks against one B
+// block (the B fragment and its Y form are shared); a pair with a > b yields
+// G_ab = conj(G_ba) and is stored transposed with Im negated.  The assignment
+// orients the 28 block pairs a != b so that blocks 0..3 serve as B for 2 of
+// them and blocks 4..7 for 5 (with its diagonal pair: 3 and 6 = one and two
+// waves), so every wave has three pairs and every SIMD nine.  The 96
+// accumulator registers of three pairs leave no room for the outer fp32 sums
+// at three waves per SIMD: the MFMA accumulators hold the whole chunk (at most
+// a few thousand amplitudes; rounding ~1e-8 of |d_k| |d_l| per chunk partial).
+constexpr int kThreads256 = 768;
+constexpr int kWaves256 = kThreads256 / 64;
+
+struct Role3 {
+    int a0, a1, a2, b;
+};
+
+__device__ __forceinline__ Role3 wave_role3(int w) {
+    switch (w) {
+    case 0: return Role3{0, 1, 2, 0};
+    case 1: return Role3{1, 2, 3, 1};
+    case 2: return Role3{2, 3, 4, 2};
+    case 3: return Role3{3, 0, 4, 3};
+    case 4: return Role3{4, 0, 1, 4};
+    case 5: return Role3{5, 6, 7, 4};
+    case 6: return Role3{5, 0, 1, 5};
+    case 7: return Role3{2, 3, 6, 5};
+    case 8: return Role3{6, 0, 1, 6};
+    case 9: return Role3{2, 3, 7, 6};
+    case 10: return Role3{7, 0, 1, 7};
+    default: return Role3{2, 3, 5, 7};
+    }
+}
+
+// The four K-steps of one 32-amplitude stage for a wave's three pairs; buf
+// holds rows 0..255 (block k = rows 32 k .. 32 k + 31).
+template <int PT>
+__device__ __forceinline__ void stage_mfma3(const uint8_t *buf, const Role3 &ro, f32x16 (&re)[3], f32x16 (&im)[3],
+                                            uint32_t lane_row, uint32_t lane_x, uint32_t half) {
+#pragma unroll 1
+    for (int t = 0; t < Geo<32>::kSteps; t++) {
+        u32x4 ah[3], al[3], bh, bl, yh, yl;
+        ah[0] = frag<32>(buf, ro.a0, 0, t, lane_row, lane_x, half);
+        al[0] = frag<32>(buf, ro.a0, 1, t, lane_row, lane_x, half);
+        ah[1] = frag<32>(buf, ro.a1, 0, t, lane_row, lane_x, half);
+        al[1] = frag<32>(buf, ro.a1, 1, t, lane_row, lane_x, half);
+        ah[2] = frag<32>(buf, ro.a2, 0, t, lane_row, lane_x, half);
+        al[2] = frag<32>(buf, ro.a2, 1, t, lane_row, lane_x, half);
+        bh = frag<32>(buf, ro.b, 0, t, lane_row, lane_x, half);
+        bl = frag<32>(buf, ro.b, 1, t, lane_row, lane_x, half);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            yh[e] = swap_neg(bh[e]);
+            yl[e] = swap_neg(bl[e]);
+        }
+        // smallest terms first; the six accumulators interleaved
+        if constexpr (PT == 4) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                re[i] = mma(al[i], bl, re[i]);
+                im[i] = mma(al[i], yl, im[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            re[i] = mma(al[i], bh, re[i]);
+            im[i] = mma(al[i], yh, im[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            re[i] = mma(ah[i], bl, re[i]);
+            im[i] = mma(ah[i], yl, im[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            re[i] = mma(ah[i], bh, re[i]);
+            im[i] = mma(ah[i], yh, im[i]);
+        }
+    }
+}
+
+// Direct-to-LDS loads of one stage (amplitudes k0 .. k0 + 31) of states 0..255:
+// 64 wave-loads of 1 KB (four 256-byte rows each), wave-load g = 12 i + w.
+// States past P read state P - 1 (their rows feed only G entries >= P).
+__device__ __forceinline__ void issue_stage256(const uint8_t *__restrict__ D, size_t SB, int P, size_t k0,
+                                               uint8_t *buf, int w, uint32_t lane) {
+    typedef Geo<32> G;
+    const size_t ko = (k0 >> 6) * 512 + (k0 & 63) * 4;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int g = i * kWaves256 + w;
+        if (g >= 64) break;                                  // uniform: waves 4..11 issue five
+        int row = 4 * g + (int)(lane / G::kSlots);
+        __asm__ volatile("" : "+v"(row));
+        const uint32_t lgc = (lane % G::kSlots) ^ G::swz((uint32_t)row);
+        const uint32_t in_blk = (lgc / (G::kSlots / 2)) * 256 + (lgc % (G::kSlots / 2)) * 16;
+        const int st = row < P ? row : P - 1;
+        const uint8_t *src = D + (size_t)st * SB + ko + in_blk;
+        const uint32_t dst =
+            __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void_t *)(buf + g * 1024));  // uniform
+        asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                     :
+                     : "s"(dst), "v"(src)
+                     : "memory");
+    }
+}
+
+// Block pair (a, b) of the wave: G_ab into the tile-pair partials of the chunk,
+// transposed and conjugated when a > b.
+__device__ __forceinline__ void store_pair256(float *__restrict__ out, int a, int b, const f32x16 &re,
+                                              const f32x16 &im, uint32_t lane) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const int r = a * 32 + (e >> 2) * 8 + (int)(lane >> 5) * 4 + (e & 3);
+        const int c = b * 32 + (int)(lane & 31);
+        const bool tr = a > b;
+        const int R = tr ? c : r, C = tr ? r : c;
+        const int tile = (R >> 7) + (C >> 7);                // (0,0) 0, (0,1) 1, (1,1) 2
+        float *t = out + (size_t)tile * 2 * kTileElems;
+        t[(R & 127) * kTileStates + (C & 127)] = re[e];
+        t[kTileElems + (R & 127) * kTileStates + (C & 127)] = tr ? -im[e] : im[e];
+    }
+}
+
+template <int PT>
+__global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint8_t *__restrict__ D, int P, size_t N,
+                                                                     size_t chunk_len, float *__restrict__ partials) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes];
+    typedef Geo<32> G;
+    constexpr int kBuf = 2 * G::kTileBytes;                  // 256 rows of one stage: 64 KB
+    static_assert(2 * kBuf <= kLdsBytes, "two stage buffers");
+    const int chunk = blockIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63;
+    const Role3 ro = wave_role3(w);
+    const uint32_t lane_row = (lane & 31) * G::kRowBytes, lane_x = G::swz(lane & 31), half = lane >> 5;
+    const size_t SB = N * 8;
+    const size_t kbeg = (size_t)chunk * chunk_len;
+    const size_t kend = (kbeg + chunk_len < N) ? kbeg + chunk_len : N;
+    const int nst = kend > kbeg ? (int)((kend - kbeg) / 32) : 0;
+
+    f32x16 re[3], im[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            re[i][e] = 0.f;
+            im[i][e] = 0.f;
+        }
+    if (nst > 0) issue_stage256(D, SB, P, kbeg, lds, w, lane);
+    for (int st = 0; st < nst; st++) {
+        // this wave's loads of stage st have landed; after the barrier every wave's
+        // have, and every wave is done with stage st - 1's buffer, which stage st + 1 refills
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        if (st + 1 < nst) issue_stage256(D, SB, P, kbeg + (size_t)(st + 1) * 32, lds + ((st + 1) & 1) * kBuf, w, lane);
+        stage_mfma3<PT>(lds + (st & 1) * kBuf, ro, re, im, lane_row, lane_x, half);
+    }
+    float *out = partials + (size_t)chunk * 3 * 2 * kTileElems;
+    store_pair256(out, ro.a0, ro.b, re[0], im[0], lane);
+    store_pair256(out, ro.a1, ro.b, re[1], im[1], lane);
+    store_pair256(out, ro.a2, ro.b, re[2], im[2], lane);
+}
+
+// G[ti 128 + r][tj 128 + c] = sum over chunks (FP64, fixed order) of the
 // partials, for the blocks the Gram computed (on diagonal tile pairs the
-// 32-state blocks on or above the diagonal).
+// 32-state blocks on or above the diagonal).  A workgroup takes 64 adjacent
+// entries; its eight waves sum the chunks c = w (mod 8), each in increasing
+// order with eight loads in flight, and wave 0 adds the eight sums in wave
+// order.
 __global__ __launch_bounds__(kThreads) void gram_cplx_reduce_kernel(const float *__restrict__ partials, int nt,
                                                                     int nchunks, double2 *__restrict__ G) {
+    constexpr int kGroups = kThreads / 64;                   // waves per entry, chunk stride
+    __shared__ double2 part[kGroups][64];
     const int T = nt * (nt + 1) / 2;
     const int R = nt * kTileStates;
-    const size_t total = (size_t)T * kTileElems;
+    const size_t ngrp = (size_t)T * kTileElems / 64;
     const size_t cs = (size_t)T * 2 * kTileElems;
-    for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+    const int lane = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
+    for (size_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
+        const size_t idx = grp * 64 + lane;
         const int tile = (int)(idx / kTileElems), e = (int)(idx % kTileElems);
         const int r = e / kTileStates, c = e % kTileStates;
         int ti, tj;
         tile_coords(tile, nt, ti, tj);
-        if (ti == tj && (r >> 5) > (c >> 5)) continue;
-        const float *q = partials + (size_t)tile * 2 * kTileElems + e;
+        const bool skip = ti == tj && (r >> 5) > (c >> 5);   // never written
         double sr = 0.0, si = 0.0;
-        // nchunks is a multiple of 8: eight chunks' loads in flight, then the adds in chunk order
-        for (int k = 0; k < nchunks; k += 8) {
-            float vr[8], vi[8];
+        if (!skip) {
+            const float *q = partials + (size_t)tile * 2 * kTileElems + e;
+            int k = g;
+            for (; k + 7 * kGroups < nchunks; k += 8 * kGroups) {
+                float vr[8], vi[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                vr[u] = q[(size_t)(k + u) * cs];
-                vi[u] = q[(size_t)(k + u) * cs + kTileElems];
+                for (int u = 0; u < 8; u++) {
+                    vr[u] = q[(size_t)(k + u * kGroups) * cs];
+                    vi[u] = q[(size_t)(k + u * kGroups) * cs + kTileElems];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    sr += (double)vr[u];
+                    si += (double)vi[u];
+                }
             }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                sr += (double)vr[u];
-                si += (double)vi[u];
+            for (; k < nchunks; k += kGroups) {
+                sr += (double)q[(size_t)k * cs];
+                si += (double)q[(size_t)k * cs + kTileElems];
             }
         }
-        G[(size_t)(ti * kTileStates + r) * R + tj * kTileStates + c] = make_double2(sr, si);
+        part[g][lane] = make_double2(sr, si);
+        __syncthreads();
+        if (g == 0 && !skip) {
+            double2 t = part[0][lane];
+#pragma unroll
+            for (int w = 1; w < kGroups; w++) {
+                t.x += part[w][lane].x;
+                t.y += part[w][lane].y;
+            }
+            G[(size_t)(ti * kTileStates + r) * R + tj * kTileStates + c] = t;
+        }
+        __syncthreads();
     }
 }
 
@@ -420,6 +616,17 @@ int qgtl_hip_gram_cplx(const void *D, int P, size_t N, float *partials, int nchu
     const size_t chunk_len = ((N + nchunks - 1) / nchunks + 31) / 32 * 32;
     const dim3 grid((unsigned)(T * nchunks)), block(kThreads);
     const uint8_t *d = (const uint8_t *)D;
+    if (nt == 2) {
+        // every chunk's three tile pairs in one workgroup
+        if (products == 4)
+            hipLaunchKernelGGL((gram_cplx256_kernel<4>), dim3((unsigned)nchunks), dim3(kThreads256), 0, qgtl::stream(),
+                               d, P, N, chunk_len, partials);
+        else
+            hipLaunchKernelGGL((gram_cplx256_kernel<3>), dim3((unsigned)nchunks), dim3(kThreads256), 0, qgtl::stream(),
+                               d, P, N, chunk_len, partials);
+        QGTL_LAUNCH_CHECK("gram_cplx256");
+        return 0;
+    }
     if (stage_amps == 16 && products == 4)
         hipLaunchKernelGGL((gram_cplx_kernel<16, 4>), grid, block, 0, qgtl::stream(), d, P, N, nt, nchunks, chunk_len, partials);
     else if (stage_amps == 16)
@@ -437,7 +644,7 @@ int qgtl_hip_gram_cplx_reduce(const float *partials, int P, int nchunks, double 
         return qgtl::set_error_msg(-118, "qgtl_hip_gram_cplx_reduce: nchunks must be a positive multiple of 8");
     const int nt = (P + kTileStates - 1) / kTileStates;
     const size_t total = (size_t)(nt * (nt + 1) / 2) * kTileElems;
-    const unsigned grid = (unsigned)((total + kThreads - 1) / kThreads);
+    const unsigned grid = (unsigned)(total / 64);            // one 64-entry group per workgroup
     hipLaunchKernelGGL(gram_cplx_reduce_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), partials, nt, nchunks,
                        reinterpret_cast<double2 *>(G));
     QGTL_LAUNCH_CHECK("gram_cplx_reduce");

@@ -399,7 +399,9 @@ static int gram_chunks_cplx(const struct qgtl_plan *p, size_t free_bytes)
     if (p->tune.gram_chunks > 0) return (p->tune.gram_chunks + 7) / 8 * 8;
     const size_t per_chunk = qgtl_hip_gram_cplx_partial_floats(p->P + 1) * sizeof(float);
     const size_t T = per_chunk / (2 * 128 * 128 * sizeof(float));
-    size_t want = p->N / 2048;
+    /* 129..256 states run one workgroup per chunk whose MFMA accumulators hold the whole
+     * chunk (no outer fp32 sums): chunks of 1024 amplitudes bound the accumulation's rounding */
+    size_t want = T == 3 ? p->N / 1024 : p->N / 2048;
     const size_t minw = (512 + T - 1) / T;
     if (want < minw) want = minw;
     size_t budget = ((size_t)p->P + 1) * p->N * 8 / 16;

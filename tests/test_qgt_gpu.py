@@ -59,6 +59,18 @@ def test_qgt_mixed(n, L):
     assert err <= MIXED_TOL, err
 
 
+# 129 .. 256 Gram states (P + 1 with the primal): one workgroup per split-K chunk computes the
+# three tile pairs; P = 128 puts one state past the first tile, P = 252 fills the second
+@pytest.mark.parametrize("n,L", [(16, 4), (14, 9), (12, 10)])
+def test_qgt_mixed_two_tile_gram(n, L):
+    th, ops, oops = hea(n, L, seed=300 + n)
+    ref = O.qgt(n, oops)
+    got = Q.qgt(n, ops, Q.MODE_MIXED)
+    err = Q.rel_err(got, ref)
+    print(f"n={n} L={L} P={2 * n * L}: rel err {err:.2e}")
+    assert err <= MIXED_TOL, err
+
+
 def test_qgt_mixed_deterministic():
     th, ops, _ = hea(14, 3, seed=7)
     a = Q.qgt(14, ops, Q.MODE_MIXED)
