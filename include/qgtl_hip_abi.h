@@ -330,6 +330,7 @@ int qgtl_hip_gram_cplx_reduce(const float *partials, int P, int nchunks, double 
 int qgtl_hip_qgt_finalize_cplx(const double *G, int P, double *Q);
 /* MFMA lane-layout probe (diagnostics): 32 x 32 floats to out_dev */
 int qgtl_hip_debug_mfma_layout(float *out_dev);
+int qgtl_hip_debug_mfma16_layout(float *out_dev);   /* v_mfma_f32_16x16x32_f16 probe, 16 x 16 floats */
 
 /* ---- complex128 plain path ---- */
 int qgtl_hip_fill_zero_state_c128(void *psi, int n);

@@ -337,6 +337,9 @@ int qgtl_ng_step(qgtl_plan_t *plan, double *theta, const qgtl_ng_config_t *cfg, 
 /* Diagnostics: v_mfma_f32_32x32x16_bf16 lane-layout probe; writes the 32 x 32
  * product D[i][j] = (1 + i/16)(i%16 + 16 (j%8)) as the Gram kernel reads it. */
 int qgtl_debug_mfma_layout(float *host_out);
+/* Diagnostics: v_mfma_f32_16x16x32_f16 lane-layout probe (the 16 x 16 form of the two-tile
+ * Gram); writes the 16 x 16 product D[i][j] = i + 16 (i & 1) + 32 j as that kernel reads it. */
+int qgtl_debug_mfma_layout16(float *host_out);
 
 /* Diagnostics: one plain copy kernel of `bytes` (multiple of 16) between two
  * fresh device buffers; *ms = its HIP-event time.  Reference of known size

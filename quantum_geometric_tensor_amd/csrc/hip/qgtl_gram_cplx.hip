@@ -43,6 +43,7 @@ namespace {
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) const void gbl_void_t;
@@ -354,8 +355,11 @@ __global__ __launch_bounds__(kThreads, 1) void gram_cplx_kernel(const uint8_t *_
 // them and blocks 4..7 for 5 (with its diagonal pair: 3 and 6 = one and two
 // waves), so every wave has three pairs and every SIMD nine.  The 96
 // accumulator registers of three pairs leave no room for the outer fp32 sums
-// at three waves per SIMD: the MFMA accumulators hold the whole chunk (at most
-// a few thousand amplitudes; rounding ~1e-8 of |d_k| |d_l| per chunk partial).
+// at three waves per SIMD: the MFMA accumulators hold the whole chunk.  Their
+// rounding grows with the chunk length (each MFMA adds its products into an
+// accumulator that grows over the chunk, whatever the instruction's K), so the
+// chunks are kept at 1024 amplitudes: normwise error 3.4e-7 at n = 20, L = 6
+// against 1.4e-7 with the 256-amplitude outer sums of the per-tile-pair form.
 constexpr int kThreads256 = 768;
 constexpr int kWaves256 = kThreads256 / 64;
 
@@ -377,53 +381,6 @@ __device__ __forceinline__ Role3 wave_role3(int w) {
     case 9: return Role3{2, 3, 7, 6};
     case 10: return Role3{7, 0, 1, 7};
     default: return Role3{2, 3, 5, 7};
-    }
-}
-
-// The four K-steps of one 32-amplitude stage for a wave's three pairs; buf
-// holds rows 0..255 (block k = rows 32 k .. 32 k + 31).
-template <int PT>
-__device__ __forceinline__ void stage_mfma3(const uint8_t *buf, const Role3 &ro, f32x16 (&re)[3], f32x16 (&im)[3],
-                                            uint32_t lane_row, uint32_t lane_x, uint32_t half) {
-#pragma unroll 1
-    for (int t = 0; t < Geo<32>::kSteps; t++) {
-        u32x4 ah[3], al[3], bh, bl, yh, yl;
-        ah[0] = frag<32>(buf, ro.a0, 0, t, lane_row, lane_x, half);
-        al[0] = frag<32>(buf, ro.a0, 1, t, lane_row, lane_x, half);
-        ah[1] = frag<32>(buf, ro.a1, 0, t, lane_row, lane_x, half);
-        al[1] = frag<32>(buf, ro.a1, 1, t, lane_row, lane_x, half);
-        ah[2] = frag<32>(buf, ro.a2, 0, t, lane_row, lane_x, half);
-        al[2] = frag<32>(buf, ro.a2, 1, t, lane_row, lane_x, half);
-        bh = frag<32>(buf, ro.b, 0, t, lane_row, lane_x, half);
-        bl = frag<32>(buf, ro.b, 1, t, lane_row, lane_x, half);
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            yh[e] = swap_neg(bh[e]);
-            yl[e] = swap_neg(bl[e]);
-        }
-        // smallest terms first; the six accumulators interleaved
-        if constexpr (PT == 4) {
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                re[i] = mma(al[i], bl, re[i]);
-                im[i] = mma(al[i], yl, im[i]);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            re[i] = mma(al[i], bh, re[i]);
-            im[i] = mma(al[i], yh, im[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            re[i] = mma(ah[i], bl, re[i]);
-            im[i] = mma(ah[i], yl, im[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            re[i] = mma(ah[i], bh, re[i]);
-            im[i] = mma(ah[i], yh, im[i]);
-        }
     }
 }
 
@@ -453,21 +410,106 @@ __device__ __forceinline__ void issue_stage256(const uint8_t *__restrict__ D, si
     }
 }
 
-// Block pair (a, b) of the wave: G_ab into the tile-pair partials of the chunk,
-// transposed and conjugated when a > b.
-__device__ __forceinline__ void store_pair256(float *__restrict__ out, int a, int b, const f32x16 &re,
-                                              const f32x16 &im, uint32_t lane) {
+// 16 x 16 x 32 form of the same stage (v_mfma_f32_16x16x32_f16: each
+// instruction adds 32 products, half as many accumulator roundings per entry
+// as 32 x 32 x 16 for the same K): a block pair is 2 x 2 sub-blocks of 16
+// states.  Lane l holds A[row l % 16][k = 8 (l / 16) .. + 7] (one 16-byte
+// slot: four amplitudes), B likewise by column, and accumulator register e
+// is D[4 (l / 16) + e][l % 16] (checked on the device by
+// qgtl_debug_mfma_layout16).  K-step t of a 32-amplitude stage covers
+// amplitudes 16 t .. 16 t + 15: logical slot 8 p + 4 t + l / 16.
+__device__ __forceinline__ f32x4 mma16(u32x4 a, u32x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(as_f16(a), as_f16(b), c, 0, 0, 0);
+}
+
+__device__ __forceinline__ u32x4 frag16(const uint8_t *buf, int blk, int sub, int p, int t, uint32_t lane) {
+    const uint32_t row = (uint32_t)(32 * blk + 16 * sub) + (lane & 15);
+    const uint32_t slot = ((uint32_t)(8 * p + 4 * t) + (lane >> 4)) ^ (lane & 15);   // swz(row) = row & 15
+    return *reinterpret_cast<const u32x4 *>(buf + row * Geo<32>::kRowBytes + slot * 16);
+}
+
+// acc[i][2 sa + sb]: pair i, sub-block (sa, sb); Re and Im
+template <int PT>
+__device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &ro, f32x4 (&re)[3][4],
+                                               f32x4 (&im)[3][4], uint32_t lane) {
+    const int ab[3] = {ro.a0, ro.a1, ro.a2};
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        u32x4 bh[2], bl[2], yh[2], yl[2];
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        const int r = a * 32 + (e >> 2) * 8 + (int)(lane >> 5) * 4 + (e & 3);
-        const int c = b * 32 + (int)(lane & 31);
-        const bool tr = a > b;
-        const int R = tr ? c : r, C = tr ? r : c;
-        const int tile = (R >> 7) + (C >> 7);                // (0,0) 0, (0,1) 1, (1,1) 2
-        float *t = out + (size_t)tile * 2 * kTileElems;
-        t[(R & 127) * kTileStates + (C & 127)] = re[e];
-        t[kTileElems + (R & 127) * kTileStates + (C & 127)] = tr ? -im[e] : im[e];
+        for (int sb = 0; sb < 2; sb++) {
+            bh[sb] = frag16(buf, ro.b, sb, 0, t, lane);
+            bl[sb] = frag16(buf, ro.b, sb, 1, t, lane);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                yh[sb][e] = swap_neg(bh[sb][e]);
+                yl[sb][e] = swap_neg(bl[sb][e]);
+            }
+        }
+#pragma unroll
+        for (int sa = 0; sa < 2; sa++) {
+            u32x4 ah[3], al[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                ah[i] = frag16(buf, ab[i], sa, 0, t, lane);
+                al[i] = frag16(buf, ab[i], sa, 1, t, lane);
+            }
+            // smallest terms first; the accumulators interleaved
+            if constexpr (PT == 4) {
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int sb = 0; sb < 2; sb++) {
+                        re[i][2 * sa + sb] = mma16(al[i], bl[sb], re[i][2 * sa + sb]);
+                        im[i][2 * sa + sb] = mma16(al[i], yl[sb], im[i][2 * sa + sb]);
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int sb = 0; sb < 2; sb++) {
+                    re[i][2 * sa + sb] = mma16(al[i], bh[sb], re[i][2 * sa + sb]);
+                    im[i][2 * sa + sb] = mma16(al[i], yh[sb], im[i][2 * sa + sb]);
+                }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int sb = 0; sb < 2; sb++) {
+                    re[i][2 * sa + sb] = mma16(ah[i], bl[sb], re[i][2 * sa + sb]);
+                    im[i][2 * sa + sb] = mma16(ah[i], yl[sb], im[i][2 * sa + sb]);
+                }
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int sb = 0; sb < 2; sb++) {
+                    re[i][2 * sa + sb] = mma16(ah[i], bh[sb], re[i][2 * sa + sb]);
+                    im[i][2 * sa + sb] = mma16(ah[i], yh[sb], im[i][2 * sa + sb]);
+                }
+        }
     }
+}
+
+// one accumulator entry of pair (a, b) into the tile-pair partials (transposed and
+// conjugated when a > b)
+__device__ __forceinline__ void store_entry256(float *__restrict__ out, int a, int b, int r, int c, float vr, float vi) {
+    const bool tr = a > b;
+    const int R = tr ? c : r, C = tr ? r : c;
+    const int tile = (R >> 7) + (C >> 7);                    // (0,0) 0, (0,1) 1, (1,1) 2
+    float *t = out + (size_t)tile * 2 * kTileElems;
+    t[(R & 127) * kTileStates + (C & 127)] = vr;
+    t[kTileElems + (R & 127) * kTileStates + (C & 127)] = tr ? -vi : vi;
+}
+
+__device__ __forceinline__ void store_pair256_16(float *__restrict__ out, int a, int b, const f32x4 (&re)[4],
+                                                 const f32x4 (&im)[4], uint32_t lane) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = a * 32 + 16 * (q >> 1) + 4 * (int)(lane >> 4) + e;
+            const int c = b * 32 + 16 * (q & 1) + (int)(lane & 15);
+            store_entry256(out, a, b, r, c, re[q][e], im[q][e]);
+        }
 }
 
 template <int PT>
@@ -481,20 +523,22 @@ __global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
     const Role3 ro = wave_role3(w);
-    const uint32_t lane_row = (lane & 31) * G::kRowBytes, lane_x = G::swz(lane & 31), half = lane >> 5;
     const size_t SB = N * 8;
     const size_t kbeg = (size_t)chunk * chunk_len;
     const size_t kend = (kbeg + chunk_len < N) ? kbeg + chunk_len : N;
     const int nst = kend > kbeg ? (int)((kend - kbeg) / 32) : 0;
 
-    f32x16 re[3], im[3];
+    f32x4 re4[3][4], im4[3][4];
 #pragma unroll
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 3; i++) {
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            re[i][e] = 0.f;
-            im[i][e] = 0.f;
-        }
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                re4[i][q][e] = 0.f;
+                im4[i][q][e] = 0.f;
+            }
+    }
     if (nst > 0) issue_stage256(D, SB, P, kbeg, lds, w, lane);
     for (int st = 0; st < nst; st++) {
         // this wave's loads of stage st have landed; after the barrier every wave's
@@ -503,12 +547,30 @@ __global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         if (st + 1 < nst) issue_stage256(D, SB, P, kbeg + (size_t)(st + 1) * 32, lds + ((st + 1) & 1) * kBuf, w, lane);
-        stage_mfma3<PT>(lds + (st & 1) * kBuf, ro, re, im, lane_row, lane_x, half);
+        stage_mfma3_16<PT>(lds + (st & 1) * kBuf, ro, re4, im4, lane);
     }
     float *out = partials + (size_t)chunk * 3 * 2 * kTileElems;
-    store_pair256(out, ro.a0, ro.b, re[0], im[0], lane);
-    store_pair256(out, ro.a1, ro.b, re[1], im[1], lane);
-    store_pair256(out, ro.a2, ro.b, re[2], im[2], lane);
+    store_pair256_16(out, ro.a0, ro.b, re4[0], im4[0], lane);
+    store_pair256_16(out, ro.a1, ro.b, re4[1], im4[1], lane);
+    store_pair256_16(out, ro.a2, ro.b, re4[2], im4[2], lane);
+}
+
+// v_mfma_f32_16x16x32_f16 lane-layout probe: A[i][k] = 1 for k = i + 16 (i & 1) (else 0),
+// B[k][j] = k + 32 j, so D[i][j] = i + 16 (i & 1) + 32 j, written out[16 i + j] with the layout
+// the 16 x 16 Gram form assumes.
+__global__ void mfma16_layout_probe_kernel(float *out) {
+    const uint32_t lane = threadIdx.x;
+    f16x8 a, b;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int i = (int)(lane & 15), k = 8 * (int)(lane >> 4) + e, j = (int)(lane & 15);
+        a[e] = (_Float16)((k == i + 16 * (i & 1)) ? 1.f : 0.f);
+        b[e] = (_Float16)(float)(k + 32 * j);
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < 4; e++) out[(4 * (lane >> 4) + e) * 16 + (lane & 15)] = acc[e];
 }
 
 // G[ti 128 + r][tj 128 + c] = sum over chunks (FP64, fixed order) of the
@@ -618,12 +680,13 @@ int qgtl_hip_gram_cplx(const void *D, int P, size_t N, float *partials, int nchu
     const uint8_t *d = (const uint8_t *)D;
     if (nt == 2) {
         // every chunk's three tile pairs in one workgroup
+        // (16 x 16 x 32 MFMA form: 0.75 ms at n = 20, P = 240, against 0.83 ms for a 32 x 32 x 16 form of the
+        // same kernel)
+        const dim3 g((unsigned)nchunks), b(kThreads256);
         if (products == 4)
-            hipLaunchKernelGGL((gram_cplx256_kernel<4>), dim3((unsigned)nchunks), dim3(kThreads256), 0, qgtl::stream(),
-                               d, P, N, chunk_len, partials);
+            hipLaunchKernelGGL((gram_cplx256_kernel<4>), g, b, 0, qgtl::stream(), d, P, N, chunk_len, partials);
         else
-            hipLaunchKernelGGL((gram_cplx256_kernel<3>), dim3((unsigned)nchunks), dim3(kThreads256), 0, qgtl::stream(),
-                               d, P, N, chunk_len, partials);
+            hipLaunchKernelGGL((gram_cplx256_kernel<3>), g, b, 0, qgtl::stream(), d, P, N, chunk_len, partials);
         QGTL_LAUNCH_CHECK("gram_cplx256");
         return 0;
     }
@@ -648,6 +711,12 @@ int qgtl_hip_gram_cplx_reduce(const float *partials, int P, int nchunks, double 
     hipLaunchKernelGGL(gram_cplx_reduce_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), partials, nt, nchunks,
                        reinterpret_cast<double2 *>(G));
     QGTL_LAUNCH_CHECK("gram_cplx_reduce");
+    return 0;
+}
+
+int qgtl_hip_debug_mfma16_layout(float *out_dev) {
+    hipLaunchKernelGGL(mfma16_layout_probe_kernel, dim3(1), dim3(64), 0, qgtl::stream(), out_dev);
+    QGTL_LAUNCH_CHECK("mfma16_layout_probe");
     return 0;
 }
 

@@ -1164,6 +1164,20 @@ int qgtl_debug_mfma_layout(float *host_out)
     return rc ? hip_fail(rc, "qgtl_debug_mfma_layout") : 0;
 }
 
+int qgtl_debug_mfma_layout16(float *host_out)
+{
+    QGTL_LOCKED();
+    int rc = ensure_init();
+    if (rc) return rc;
+    void *d = NULL;
+    HIPCALL(qgtl_hip_malloc(&d, 16 * 16 * sizeof(float)));
+    rc = qgtl_hip_debug_mfma16_layout((float *)d);
+    if (!rc) rc = qgtl_hip_d2h(host_out, d, 16 * 16 * sizeof(float));
+    if (!rc) rc = qgtl_hip_sync();
+    qgtl_hip_free(d);
+    return rc ? hip_fail(rc, "qgtl_debug_mfma_layout16") : 0;
+}
+
 /* Diagnostics: after qgtl_plan_execute in mixed mode, copy the FP64 Gram
  * C = Z Z^T (R x R, R = 128 ceil(2P/128), slot order) and the injection sums
  * (P x 4: Re/Im <d|psi>, |d|^2, |psi|^2, slot order). */

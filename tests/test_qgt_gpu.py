@@ -32,6 +32,16 @@ def test_mfma_lane_layout():
     np.testing.assert_array_equal(out.reshape(32, 32), expect.astype(np.float32))
 
 
+def test_mfma16_lane_layout():
+    out = np.zeros(16 * 16, np.float32)
+    rc = Q.lib().qgtl_debug_mfma_layout16(out.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0, Q.lib().qgtl_last_error()
+    i = np.arange(16)[:, None]
+    j = np.arange(16)[None, :]
+    expect = i + 16 * (i & 1) + 32 * j
+    np.testing.assert_array_equal(out.reshape(16, 16), expect.astype(np.float32))
+
+
 @pytest.mark.parametrize("n,L", [(1, 2), (2, 2), (4, 3), (10, 4), (13, 2), (14, 2), (16, 2)])
 def test_statevector_hea(n, L):
     th, ops, oops = hea(n, L, seed=n)
