@@ -147,7 +147,7 @@ typedef struct {
 typedef struct {
     uint64_t desc;          /* qgtl_inject_desc_t */
     uint64_t psi, dst;      /* state k of the item written at dst + k * stride */
-    uint64_t partials;      /* [K][tiles][4] FP64 sums */
+    uint64_t partials;      /* [K][tiles][4] FP64 sums; 0: no sums */
     uint64_t stride;
     uint32_t tiles;         /* ceil(2^n / 2048) */
     uint32_t K;
@@ -181,7 +181,8 @@ typedef struct {
 } qgtl_gram_reduce_item_t;
 
 typedef struct {
-    uint64_t C, inj, Q;     /* Q: P x P complex FP64 in D-slot order */
+    uint64_t C, inj, Q;     /* Q: P x P complex FP64 in D-slot order; inj = 0: C holds P + 1 states, the
+                               last the primal, whose row gives the overlaps and norms */
     int32_t R, P;
     uint32_t unit_begin;    /* units: ceil(P * P / 256) */
     uint32_t pad;

@@ -437,9 +437,13 @@ __global__ __launch_bounds__(kThreads) void finalize_grouped_kernel(const qgtl_f
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const qgtl_finalize_item_t it = items[qgtl::find_item(items, nitems, u)];
         const size_t idx = (size_t)(u - it.unit_begin) * kThreads + threadIdx.x;
-        if (idx < (size_t)it.P * it.P)
+        if (idx >= (size_t)it.P * it.P) continue;
+        if (it.inj)             // overlaps and norms from FP64 injection sums
             qgt_finalize_elem(reinterpret_cast<const double *>(it.C), it.R, reinterpret_cast<const double *>(it.inj),
                               it.P, reinterpret_cast<double *>(it.Q), idx);
+        else                    // ... from the Gram row of the primal state (state P)
+            qgt_finalize_psi_elem(reinterpret_cast<const double *>(it.C), it.R, it.P, reinterpret_cast<double *>(it.Q),
+                                  idx);
     }
 }
 

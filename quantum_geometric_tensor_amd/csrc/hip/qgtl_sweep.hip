@@ -1160,8 +1160,13 @@ __global__ __launch_bounds__(kThreads) void inject_grouped_kernel(const qgtl_inj
         const int k = (int)(local / it.tiles);
         __syncthreads();
         const qgtl_inject_desc_t &d = *reinterpret_cast<const qgtl_inject_desc_t *>(it.desc);
-        inject_body(d, reinterpret_cast<const float2 *>(it.psi), reinterpret_cast<float2 *>(it.dst) + (size_t)k * it.stride,
-                    k, t, reinterpret_cast<double *>(it.partials) + ((size_t)k * it.tiles + t) * 4, red);
+        float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)k * it.stride;
+        if (it.partials)        // FP64 sums of <d|psi>, |d|^2, |psi|^2 per tile as well
+            inject_body(d, reinterpret_cast<const float2 *>(it.psi), dst, k, t,
+                        reinterpret_cast<double *>(it.partials) + ((size_t)k * it.tiles + t) * 4, red);
+        else
+            inject_body<false, true, kInjectPerThread, false>(d, reinterpret_cast<const float2 *>(it.psi), dst, k, t,
+                                                              nullptr, red);
     }
 }
 

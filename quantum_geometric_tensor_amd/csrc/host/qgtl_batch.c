@@ -15,8 +15,11 @@
  * of instances with b <= c.  Backward phase, block step b' descending: one
  * grouped launch per inverse pass carries every instance's states of blocks
  * > b' (b' > c) back through block b'.  All states then sit after their
- * instance's block c, in one arena, and one grouped launch per Gram precision
- * class, one split-K reduction and one FP64 correction finish every Q (the
+ * instance's block c, in one arena, beside a copy of the primal after block c
+ * (slot P + 1, made by an identity injection), and one grouped launch per Gram
+ * precision class over the P + 1 states, one split-K reduction and one FP64
+ * correction finish every Q: the overlaps <d_k|psi> and the norms are the
+ * Gram's primal row and diagonal, so the injections write states only (the
  * inner products are the same after any common unitary).  The primal is slot
  * 0 of an instance's state array and derivative state s is slot 1 + s (forward
  * states first), so one pass item covers a contiguous range of an instance's
@@ -50,7 +53,7 @@
 #define GRAM_KS QGTL_GRAM_KSTEP
 #define XCDS 8
 
-enum { BL_FILL, BL_PASS, BL_INJECT, BL_REDUCE, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL };
+enum { BL_FILL, BL_PASS, BL_INJECT, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL };
 
 typedef struct {
     int kind;
@@ -66,8 +69,6 @@ typedef struct {
     size_t N;
     size_t op_off;             /* first op of the instance in the creation arrays */
     size_t x_off, y_off;       /* complex64 elements into the state arena; y_off == SIZE_MAX: no flips */
-    size_t part_off;           /* doubles into the injection partials */
-    size_t sums_off;           /* doubles into the injection sums */
     size_t gp_off;             /* floats into the Gram partials */
     size_t c_off;              /* doubles into C */
     size_t q_off;              /* complex elements of Q */
@@ -94,7 +95,7 @@ struct qgtl_batch {
     int descs_dirty;
     /* arenas */
     void *states;
-    double *inj_partials, *inj_sums, *C, *Qdev;
+    double *C, *Qdev;
     float *gram_partials;
     size_t gp_len;             /* floats allocated */
     size_t q_len;              /* complex elements */
@@ -133,6 +134,8 @@ static size_t count_pass_descs(const qgtl_plan_t *p)
     return c;
 }
 
+/* the injection chunks of every block, then one identity "injection" that
+ * copies the primal state at the meeting block into slot P + 1 */
 static size_t count_inj_descs(const qgtl_plan_t *p)
 {
     size_t c = 0;
@@ -140,7 +143,7 @@ static size_t count_inj_descs(const qgtl_plan_t *p)
         const int K = p->blocks[b].param_end - p->blocks[b].param_begin;
         c += (size_t)((K + QGTL_INJECT_MAX - 1) / QGTL_INJECT_MAX);
     }
-    return c;
+    return c + 1;
 }
 
 /* Descriptor order per instance: the passes of every block, then the inverse
@@ -182,6 +185,12 @@ static void fill_inst_descs(struct qgtl_batch *B, int i)
             }
         }
     }
+    qgtl_inject_desc_t *d = &B->inj_h[ii];          /* the primal copy: d[y] = psi[y] */
+    memset(d, 0, sizeof *d);
+    d->n = p->n;
+    d->K = 1;
+    d->A[0][0].re = 1.f;
+    d->A[0][3].re = 1.f;
 }
 
 static void fill_descs(struct qgtl_batch *B)
@@ -238,7 +247,7 @@ static void gram_shape(const struct qgtl_batch *B, inst_t *in)
     if (nch < XCDS) nch = XCDS;
     in->nchunks = (int)nch;
     in->chunk_len = ((in->N + nch - 1) / nch + GRAM_KS - 1) / GRAM_KS * GRAM_KS;
-    in->nt = (2 * in->P + GRAM_TILE - 1) / GRAM_TILE;
+    in->nt = (2 * (in->P + 1) + GRAM_TILE - 1) / GRAM_TILE;
     in->T = (int)upper_tiles(in->nt);
 }
 
@@ -306,7 +315,7 @@ static int build_schedule(struct qgtl_batch *B)
     B->gram_flops = 0.0;
     sched_t *sc = (sched_t *)calloc((size_t)cnt, sizeof(sched_t));
     size_t bytes = sizeof(qgtl_pass_item_t) * (size_t)cnt * 2;
-    if (bytes < sizeof(qgtl_inject_item_t) * (size_t)cnt * 8) bytes = sizeof(qgtl_inject_item_t) * (size_t)cnt * 8;
+    if (bytes < sizeof(qgtl_inject_item_t) * (size_t)cnt * 9) bytes = sizeof(qgtl_inject_item_t) * (size_t)cnt * 9;
     void *tmp = malloc(bytes + sizeof(qgtl_gram_item_t) * (size_t)cnt * 2);
     if (!sc || !tmp) {
         free(sc); free(tmp);
@@ -404,8 +413,7 @@ static int build_schedule(struct qgtl_batch *B)
                     it[k].desc = dev_addr(B->inj_d, sc[i].inj_desc++ * sizeof(qgtl_inject_desc_t));
                     it[k].psi = dev_addr(B->states, arena_off(in, sc[i].psi) * c64);
                     it[k].dst = dev_addr(B->states, (arena_off(in, dst_arena) + (1 + s0) * N) * c64);
-                    it[k].partials = dev_addr(B->inj_partials, (in->part_off + s0 * (size_t)in->tiles * 4) *
-                                                                   sizeof(double));
+                    it[k].partials = 0;                       /* no FP64 sums: the overlaps come from the Gram */
                     it[k].stride = N;
                     it[k].tiles = (uint32_t)in->tiles;
                     it[k].K = (uint32_t)c;
@@ -415,6 +423,20 @@ static int build_schedule(struct qgtl_batch *B)
                     k++;
                 }
                 if (!dir) sc[i].fwd_slots += K;
+                if (!dir && b == in->meet) {
+                    /* the primal after the meeting block joins the Gram as slot P + 1 */
+                    it[k].desc = dev_addr(B->inj_d, (in->inj_off + count_inj_descs(p) - 1) * sizeof(qgtl_inject_desc_t));
+                    it[k].psi = dev_addr(B->states, arena_off(in, sc[i].psi) * c64);
+                    it[k].dst = dev_addr(B->states, (arena_off(in, sc[i].psi) + (size_t)(in->P + 1) * N) * c64);
+                    it[k].partials = 0;
+                    it[k].stride = N;
+                    it[k].tiles = (uint32_t)in->tiles;
+                    it[k].K = 1;
+                    it[k].unit_begin = (uint32_t)u;
+                    it[k].pad = 0;
+                    u += (uint64_t)it[k].tiles;
+                    k++;
+                }
             }
             rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_INJECT, 0, k, u);
             if (rc) goto out;
@@ -457,21 +479,6 @@ static int build_schedule(struct qgtl_batch *B)
                                 i);
             goto out;
         }
-    {   /* FP64 injection sums */
-        qgtl_reduce_item_t *it = (qgtl_reduce_item_t *)tmp;
-        uint64_t u = 0;
-        for (int i = 0; i < cnt; i++) {
-            it[i].partials = dev_addr(B->inj_partials, B->inst[i].part_off * sizeof(double));
-            it[i].out = dev_addr(B->inj_sums, B->inst[i].sums_off * sizeof(double));
-            it[i].tiles = (uint32_t)B->inst[i].tiles;
-            it[i].K = (uint32_t)B->inst[i].P;
-            it[i].unit_begin = (uint32_t)u;
-            it[i].pad = 0;
-            u += (uint64_t)B->inst[i].P;
-        }
-        rc = push_items(B, it, sizeof(*it) * (size_t)cnt, BL_REDUCE, 0, cnt, u);
-        if (rc) goto out;
-    }
     static const int term_classes[5] = {6, 4, 3, QGTL_GRAM_F16X3, QGTL_GRAM_F16X4};
     for (int tc = 0; tc < 5; tc++) {
         qgtl_gram_item_t *it = (qgtl_gram_item_t *)tmp;
@@ -484,7 +491,7 @@ static int build_schedule(struct qgtl_batch *B)
             it[k].partials = dev_addr(B->gram_partials, in->gp_off * sizeof(float));
             it[k].N = in->N;
             it[k].chunk_len = in->chunk_len;
-            it[k].P = in->P;
+            it[k].P = in->P + 1;                       /* the derivative states and the primal */
             it[k].nt = in->nt;
             it[k].nchunks = in->nchunks;
             it[k].unit_begin = (uint32_t)u;
@@ -517,7 +524,7 @@ static int build_schedule(struct qgtl_batch *B)
         for (int i = 0; i < cnt; i++) {
             const inst_t *in = &B->inst[i];
             it[i].C = dev_addr(B->C, in->c_off * sizeof(double));
-            it[i].inj = dev_addr(B->inj_sums, in->sums_off * sizeof(double));
+            it[i].inj = 0;                             /* a_k and the norms from the Gram's primal row */
             it[i].Q = dev_addr(B->Qdev, in->q_off * 2 * sizeof(double));
             it[i].R = in->nt * GRAM_TILE;
             it[i].P = in->P;
@@ -558,7 +565,6 @@ static int issue(struct qgtl_batch *B)
             BTRY(qgtl_hip_pass_grouped((const qgtl_pass_item_t *)items, L->nitems, L->units, g, L->terms));
             break;
         case BL_INJECT: BTRY(qgtl_hip_inject_grouped((const qgtl_inject_item_t *)items, L->nitems, L->units, g)); break;
-        case BL_REDUCE: BTRY(qgtl_hip_reduce_grouped((const qgtl_reduce_item_t *)items, L->nitems, L->units, g)); break;
         case BL_GRAM:
             BTRY(qgtl_hip_gram_grouped((const qgtl_gram_item_t *)items, L->nitems, L->units, g, L->terms));
             break;
@@ -619,8 +625,6 @@ void qgtl_batch_destroy(qgtl_batch_t *B)
     qgtl_hip_graph_destroy(B->graph);
     for (int i = 0; i < B->count; i++) qgtl_plan_destroy(B->inst[i].plan);
     qgtl_hip_free(B->states);
-    qgtl_hip_free(B->inj_partials);
-    qgtl_hip_free(B->inj_sums);
     qgtl_hip_free(B->gram_partials);
     qgtl_hip_free(B->C);
     qgtl_hip_free(B->Qdev);
@@ -661,7 +665,7 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
     if (tune) B->tune = *tune;
     B->inst = (inst_t *)calloc((size_t)count, sizeof(inst_t));
     if (!B->inst) { qgtl_set_error(QGT_ERROR_NO_MEMORY, "out of host memory"); return fail_create(B); }
-    size_t op_off = 0, xo = 0, po = 0, so = 0, co = 0, qo = 0;
+    size_t op_off = 0, xo = 0, co = 0, qo = 0;
     for (int i = 0; i < count; i++) {
         inst_t *in = &B->inst[i];
         qgtl_tune_t pt;
@@ -704,17 +708,13 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
         int fa = 0;
         for (int b = 0; b <= in->meet; b++) fa ^= block_flips(&p->blocks[b], 0) & 1;
         in->final_arena = fa;
-        const size_t states = (size_t)(in->P + 1) * in->N;
+        const size_t states = (size_t)(in->P + 2) * in->N;    /* primal, P derivative states, primal at the meet */
         in->x_off = xo;
         xo += states;
         in->y_off = SIZE_MAX;
         if (in->flips) { in->y_off = xo; xo += states; }
         in->tiles = (int)((in->N + INJECT_TILE - 1) / INJECT_TILE);
-        in->part_off = po;
-        po += (size_t)in->P * (size_t)in->tiles * 4;
-        in->sums_off = so;
-        so += (size_t)in->P * 4;
-        in->nt = (2 * in->P + GRAM_TILE - 1) / GRAM_TILE;
+        in->nt = (2 * (in->P + 1) + GRAM_TILE - 1) / GRAM_TILE;
         in->T = (int)upper_tiles(in->nt);
         in->c_off = co;
         co += (size_t)(in->nt * GRAM_TILE) * (size_t)(in->nt * GRAM_TILE);
@@ -734,13 +734,11 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
     CTRY(qgtl_hip_host_alloc((void **)&B->inj_h, (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t)));
     CTRY(qgtl_hip_host_alloc((void **)&B->qhost, qo * 2 * sizeof(double)));
     CTRY(qgtl_hip_malloc(&B->states, xo * 8));
-    CTRY(qgtl_hip_malloc((void **)&B->inj_partials, po * sizeof(double)));
-    CTRY(qgtl_hip_malloc((void **)&B->inj_sums, so * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->C, co * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->Qdev, qo * 2 * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->pass_d, (B->npass ? B->npass : 1) * sizeof(qgtl_pass_desc_t)));
     CTRY(qgtl_hip_malloc((void **)&B->inj_d, (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t)));
-    B->device_bytes = xo * 8 + (po + so + co + qo * 2) * sizeof(double) +
+    B->device_bytes = xo * 8 + (co + qo * 2) * sizeof(double) +
                       (B->npass ? B->npass : 1) * sizeof(qgtl_pass_desc_t) +
                       (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t);
     fill_descs(B);
