@@ -141,7 +141,8 @@ typedef struct {
     uint32_t ntiles;        /* tiles per state, 2^(n - m) */
     uint32_t nstates;
     uint32_t unit_begin;    /* units: ntiles * nstates, tile-major within a state */
-    uint32_t variant;       /* 0: LDS gate groups; 1: register phases (m = 13, nphase >= 1, no PERM store) */
+    uint32_t variant;       /* 0: LDS gate groups; 1: register phases (m = 13, nphase >= 1, no PERM store);
+                               2: register phases storing the split-plane form (a final pass) */
 } qgtl_pass_item_t;
 
 typedef struct {
@@ -152,8 +153,9 @@ typedef struct {
     uint32_t tiles;         /* ceil(2^n / 2048) */
     uint32_t K;
     uint32_t unit_begin;    /* units: tiles * K */
-    uint32_t pad;
+    uint32_t pad;           /* QGTL_INJECT_ITEM_SPLIT: write the states in split-plane form (no sums) */
 } qgtl_inject_item_t;
+#define QGTL_INJECT_ITEM_SPLIT 1u
 
 typedef struct {
     uint64_t partials;      /* [K][tiles][4] */
@@ -204,6 +206,14 @@ int qgtl_hip_reduce_grouped(const qgtl_reduce_item_t *items, int nitems, uint32_
 int qgtl_hip_gram_grouped(const qgtl_gram_item_t *items, int nitems, uint32_t units, int grid, int terms);
 int qgtl_hip_gram_reduce_grouped(const qgtl_gram_reduce_item_t *items, int nitems, uint32_t units, int grid);
 int qgtl_hip_finalize_grouped(const qgtl_finalize_item_t *items, int nitems, uint32_t units, int grid);
+/* The complex-form Gram of split-plane states, grouped, for instances of 129..256 Gram
+ * states (P = the item's state count, nt = 2): one unit per split-K chunk, the
+ * three tile pairs' fp32 partials per chunk; its reduction into G (complex128,
+ * nt 128 x nt 128; units per item: T 256, T = 3) and the FP64 correction with
+ * the primal as the last state (units per item: ceil(P^2 / 512), P = parameters). */
+int qgtl_hip_gram_cplx_grouped(const qgtl_gram_item_t *items, int nitems, uint32_t units, int grid, int products);
+int qgtl_hip_gram_cplx_reduce_grouped(const qgtl_gram_reduce_item_t *items, int nitems, uint32_t units, int grid);
+int qgtl_hip_finalize_cplx_grouped(const qgtl_finalize_item_t *items, int nitems, uint32_t units, int grid);
 
 /* ---- graphs: capture the library stream's launches and replay them ---- */
 typedef struct qgtl_hip_graph_s *qgtl_hip_graph_t;

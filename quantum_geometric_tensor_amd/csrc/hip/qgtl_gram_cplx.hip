@@ -512,14 +512,14 @@ __device__ __forceinline__ void store_pair256_16(float *__restrict__ out, int a,
         }
 }
 
+// One split-K chunk of a two-tile Gram: stages of 32 amplitudes through the two
+// LDS buffers, the chunk's fp32 partials of the three tile pairs at the end.
 template <int PT>
-__global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint8_t *__restrict__ D, int P, size_t N,
-                                                                     size_t chunk_len, float *__restrict__ partials) {
-    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes];
+__device__ __forceinline__ void gram_cplx256_unit(const uint8_t *__restrict__ D, int P, size_t N, size_t chunk_len,
+                                                  int chunk, float *__restrict__ partials, uint8_t *lds) {
     typedef Geo<32> G;
     constexpr int kBuf = 2 * G::kTileBytes;                  // 256 rows of one stage: 64 KB
     static_assert(2 * kBuf <= kLdsBytes, "two stage buffers");
-    const int chunk = blockIdx.x;
     const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
     const Role3 ro = wave_role3(w);
@@ -555,6 +555,31 @@ __global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint
     store_pair256_16(out, ro.a2, ro.b, re4[2], im4[2], lane);
 }
 
+template <int PT>
+__global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_kernel(const uint8_t *__restrict__ D, int P, size_t N,
+                                                                     size_t chunk_len, float *__restrict__ partials) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes];
+    gram_cplx256_unit<PT>(D, P, N, chunk_len, (int)blockIdx.x, partials, lds);
+}
+
+// Grouped (many instances, 129..256 Gram states each): unit u = chunk u - unit_begin of
+// its item; the workgroup loops over units with a grid stride.
+template <int PT>
+__global__ __launch_bounds__(kThreads256, 1) void gram_cplx256_grouped_kernel(const qgtl_gram_item_t *__restrict__ items,
+                                                                             int nitems, uint32_t units) {
+    __shared__ __attribute__((aligned(16))) uint8_t lds[kLdsBytes];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_gram_item_t it = items[qgtl::find_item(items, nitems, u)];
+        // every wave is done with the previous unit's stage buffers
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        gram_cplx256_unit<PT>(qgtl::global_ptr<const uint8_t>(qgtl::uniform(it.D)), qgtl::uniform(it.P),
+                              (size_t)qgtl::uniform(it.N), (size_t)qgtl::uniform(it.chunk_len),
+                              (int)(u - qgtl::uniform(it.unit_begin)), qgtl::global_ptr<float>(qgtl::uniform(it.partials)),
+                              lds);
+    }
+}
+
 // v_mfma_f32_16x16x32_f16 lane-layout probe: A[i][k] = 1 for k = i + 16 (i & 1) (else 0),
 // B[k][j] = k + 32 j, so D[i][j] = i + 16 (i & 1) + 32 j, written out[16 i + j] with the layout
 // the 16 x 16 Gram form assumes.
@@ -579,85 +604,118 @@ __global__ void mfma16_layout_probe_kernel(float *out) {
 // entries; its eight waves sum the chunks c = w (mod 8), each in increasing
 // order with eight loads in flight, and wave 0 adds the eight sums in wave
 // order.
-__global__ __launch_bounds__(kThreads) void gram_cplx_reduce_kernel(const float *__restrict__ partials, int nt,
-                                                                    int nchunks, double2 *__restrict__ G) {
+// Group grp (64 adjacent entries of the tile-pair partials) of one reduction.
+__device__ __forceinline__ void gram_cplx_reduce_group(const float *__restrict__ partials, int nt, int nchunks,
+                                                       double2 *__restrict__ G, size_t grp, double2 (*part)[64]) {
     constexpr int kGroups = kThreads / 64;                   // waves per entry, chunk stride
-    __shared__ double2 part[kGroups][64];
     const int T = nt * (nt + 1) / 2;
     const int R = nt * kTileStates;
-    const size_t ngrp = (size_t)T * kTileElems / 64;
     const size_t cs = (size_t)T * 2 * kTileElems;
     const int lane = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
-    for (size_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) {
-        const size_t idx = grp * 64 + lane;
-        const int tile = (int)(idx / kTileElems), e = (int)(idx % kTileElems);
-        const int r = e / kTileStates, c = e % kTileStates;
-        int ti, tj;
-        tile_coords(tile, nt, ti, tj);
-        const bool skip = ti == tj && (r >> 5) > (c >> 5);   // never written
-        double sr = 0.0, si = 0.0;
-        if (!skip) {
-            const float *q = partials + (size_t)tile * 2 * kTileElems + e;
-            int k = g;
-            for (; k + 7 * kGroups < nchunks; k += 8 * kGroups) {
-                float vr[8], vi[8];
+    const size_t idx = grp * 64 + lane;
+    const int tile = (int)(idx / kTileElems), e = (int)(idx % kTileElems);
+    const int r = e / kTileStates, c = e % kTileStates;
+    int ti, tj;
+    tile_coords(tile, nt, ti, tj);
+    const bool skip = ti == tj && (r >> 5) > (c >> 5);       // never written
+    double sr = 0.0, si = 0.0;
+    if (!skip) {
+        const float *q = partials + (size_t)tile * 2 * kTileElems + e;
+        int k = g;
+        for (; k + 7 * kGroups < nchunks; k += 8 * kGroups) {
+            float vr[8], vi[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    vr[u] = q[(size_t)(k + u * kGroups) * cs];
-                    vi[u] = q[(size_t)(k + u * kGroups) * cs + kTileElems];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    sr += (double)vr[u];
-                    si += (double)vi[u];
-                }
+            for (int u = 0; u < 8; u++) {
+                vr[u] = q[(size_t)(k + u * kGroups) * cs];
+                vi[u] = q[(size_t)(k + u * kGroups) * cs + kTileElems];
             }
-            for (; k < nchunks; k += kGroups) {
-                sr += (double)q[(size_t)k * cs];
-                si += (double)q[(size_t)k * cs + kTileElems];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                sr += (double)vr[u];
+                si += (double)vi[u];
             }
         }
-        part[g][lane] = make_double2(sr, si);
-        __syncthreads();
-        if (g == 0 && !skip) {
-            double2 t = part[0][lane];
-#pragma unroll
-            for (int w = 1; w < kGroups; w++) {
-                t.x += part[w][lane].x;
-                t.y += part[w][lane].y;
-            }
-            G[(size_t)(ti * kTileStates + r) * R + tj * kTileStates + c] = t;
+        for (; k < nchunks; k += kGroups) {
+            sr += (double)q[(size_t)k * cs];
+            si += (double)q[(size_t)k * cs + kTileElems];
         }
-        __syncthreads();
+    }
+    part[g][lane] = make_double2(sr, si);
+    __syncthreads();
+    if (g == 0 && !skip) {
+        double2 t = part[0][lane];
+#pragma unroll
+        for (int w = 1; w < kGroups; w++) {
+            t.x += part[w][lane].x;
+            t.y += part[w][lane].y;
+        }
+        G[(size_t)(ti * kTileStates + r) * R + tj * kTileStates + c] = t;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void gram_cplx_reduce_kernel(const float *__restrict__ partials, int nt,
+                                                                    int nchunks, double2 *__restrict__ G) {
+    __shared__ double2 part[kThreads / 64][64];
+    const size_t ngrp = (size_t)(nt * (nt + 1) / 2) * kTileElems / 64;
+    for (size_t grp = blockIdx.x; grp < ngrp; grp += gridDim.x) gram_cplx_reduce_group(partials, nt, nchunks, G, grp, part);
+}
+
+// grouped: unit = one 64-entry group of an item (units per item: T * 256)
+__global__ __launch_bounds__(kThreads) void gram_cplx_reduce_grouped_kernel(
+    const qgtl_gram_reduce_item_t *__restrict__ items, int nitems, uint32_t units) {
+    __shared__ double2 part[kThreads / 64][64];
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_gram_reduce_item_t it = items[qgtl::find_item(items, nitems, u)];
+        gram_cplx_reduce_group(qgtl::global_ptr<const float>(qgtl::uniform(it.partials)), qgtl::uniform(it.nt),
+                               qgtl::uniform(it.nchunks), qgtl::global_ptr<double2>(qgtl::uniform(it.C)),
+                               (size_t)(u - qgtl::uniform(it.unit_begin)), part);
     }
 }
 
 // Q_kl = G_kl / (4 sqrt(G_kk G_ll)) - a_k conj(a_l), a_k = G_kP / (2 sqrt(G_kk G_PP)):
 // state P is the primal state, and the normalisation applies the exact
 // identities ||d_k psi|| = 1/2 and ||psi|| = 1 (the 2^30 scale of G cancels).
+__device__ __forceinline__ void qgt_finalize_cplx_elem(const double2 *__restrict__ G, int R, int P,
+                                                       double *__restrict__ Q, size_t idx) {
+    const int k = (int)(idx / P), l = (int)(idx % P);
+    if (k > l) return;
+    const int m = P;
+    const double2 g = G[(size_t)k * R + l];
+    const double gkk = G[(size_t)k * R + k].x, gll = G[(size_t)l * R + l].x, gmm = G[(size_t)m * R + m].x;
+    const double2 ak0 = G[(size_t)k * R + m], al0 = G[(size_t)l * R + m];
+    const double sg = 0.25 / sqrt(gkk * gll);
+    const double sk = 0.5 / sqrt(gkk * gmm), sl = 0.5 / sqrt(gll * gmm);
+    const double akr = ak0.x * sk, aki = ak0.y * sk, alr = al0.x * sl, ali = al0.y * sl;
+    const double pr = akr * alr + aki * ali, pi = aki * alr - akr * ali;
+    const double qr = g.x * sg - pr, qi = (k == l) ? 0.0 : g.y * sg - pi;
+    Q[2 * ((size_t)k * P + l)] = qr;
+    Q[2 * ((size_t)k * P + l) + 1] = qi;
+    if (k != l) {
+        Q[2 * ((size_t)l * P + k)] = qr;
+        Q[2 * ((size_t)l * P + k) + 1] = -qi;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void qgt_finalize_cplx_kernel(const double2 *__restrict__ G, int R, int P,
                                                                      double *__restrict__ Q) {
     for (size_t idx = blockIdx.x * (size_t)kThreads + threadIdx.x; idx < (size_t)P * P;
-         idx += (size_t)gridDim.x * kThreads) {
-        const int k = (int)(idx / P), l = (int)(idx % P);
-        if (k > l) continue;
-        const int m = P;
-        const double2 g = G[(size_t)k * R + l];
-        const double gkk = G[(size_t)k * R + k].x, gll = G[(size_t)l * R + l].x, gmm = G[(size_t)m * R + m].x;
-        const double2 ak0 = G[(size_t)k * R + m], al0 = G[(size_t)l * R + m];
-        const double sg = 0.25 / sqrt(gkk * gll);
-        const double sk = 0.5 / sqrt(gkk * gmm), sl = 0.5 / sqrt(gll * gmm);
-        const double akr = ak0.x * sk, aki = ak0.y * sk, alr = al0.x * sl, ali = al0.y * sl;
-        const double pr = akr * alr + aki * ali, pi = aki * alr - akr * ali;
-        const double qr = g.x * sg - pr, qi = (k == l) ? 0.0 : g.y * sg - pi;
-        Q[2 * ((size_t)k * P + l)] = qr;
-        Q[2 * ((size_t)k * P + l) + 1] = qi;
-        if (k != l) {
-            Q[2 * ((size_t)l * P + k)] = qr;
-            Q[2 * ((size_t)l * P + k) + 1] = -qi;
-        }
+         idx += (size_t)gridDim.x * kThreads)
+        qgt_finalize_cplx_elem(G, R, P, Q, idx);
+}
+
+// grouped: unit = kThreads entries of an item's P x P (units per item: ceil(P^2 / kThreads))
+__global__ __launch_bounds__(kThreads) void finalize_cplx_grouped_kernel(const qgtl_finalize_item_t *__restrict__ items,
+                                                                         int nitems, uint32_t units) {
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const qgtl_finalize_item_t it = items[qgtl::find_item(items, nitems, u)];
+        const size_t idx = (size_t)(u - it.unit_begin) * kThreads + threadIdx.x;
+        if (idx < (size_t)it.P * it.P)
+            qgt_finalize_cplx_elem(reinterpret_cast<const double2 *>(it.C), it.R, it.P, reinterpret_cast<double *>(it.Q),
+                                   idx);
     }
 }
+
 
 }  // namespace
 
@@ -711,6 +769,39 @@ int qgtl_hip_gram_cplx_reduce(const float *partials, int P, int nchunks, double 
     hipLaunchKernelGGL(gram_cplx_reduce_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), partials, nt, nchunks,
                        reinterpret_cast<double2 *>(G));
     QGTL_LAUNCH_CHECK("gram_cplx_reduce");
+    return 0;
+}
+
+static unsigned grouped_grid(uint32_t units, int grid) {
+    if (grid <= 0 || (uint32_t)grid > units) return units;
+    return (unsigned)grid;
+}
+
+int qgtl_hip_gram_cplx_grouped(const qgtl_gram_item_t *items, int nitems, uint32_t units, int grid, int products) {
+    if (!items || nitems < 1 || units < 1 || (products != 3 && products != 4))
+        return qgtl::set_error_msg(-118, "gram_cplx_grouped: empty item list or bad product count");
+    const dim3 g(grouped_grid(units, grid)), b(kThreads256);
+    if (products == 4)
+        hipLaunchKernelGGL((gram_cplx256_grouped_kernel<4>), g, b, 0, qgtl::stream(), items, nitems, units);
+    else
+        hipLaunchKernelGGL((gram_cplx256_grouped_kernel<3>), g, b, 0, qgtl::stream(), items, nitems, units);
+    QGTL_LAUNCH_CHECK("gram_cplx_grouped");
+    return 0;
+}
+
+int qgtl_hip_gram_cplx_reduce_grouped(const qgtl_gram_reduce_item_t *items, int nitems, uint32_t units, int grid) {
+    if (!items || nitems < 1 || units < 1) return qgtl::set_error_msg(-118, "gram_cplx_reduce_grouped: empty item list");
+    hipLaunchKernelGGL(gram_cplx_reduce_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0,
+                       qgtl::stream(), items, nitems, units);
+    QGTL_LAUNCH_CHECK("gram_cplx_reduce_grouped");
+    return 0;
+}
+
+int qgtl_hip_finalize_cplx_grouped(const qgtl_finalize_item_t *items, int nitems, uint32_t units, int grid) {
+    if (!items || nitems < 1 || units < 1) return qgtl::set_error_msg(-118, "finalize_cplx_grouped: empty item list");
+    hipLaunchKernelGGL(finalize_cplx_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
+                       items, nitems, units);
+    QGTL_LAUNCH_CHECK("finalize_cplx_grouped");
     return 0;
 }
 

@@ -1145,7 +1145,8 @@ __global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pa
         __syncthreads();
         const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
         float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
-        if (VARIANT) pass_regs_body<false, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
         else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
     }
 }
@@ -1164,6 +1165,9 @@ __global__ __launch_bounds__(kThreads) void inject_grouped_kernel(const qgtl_inj
         if (it.partials)        // FP64 sums of <d|psi>, |d|^2, |psi|^2 per tile as well
             inject_body(d, reinterpret_cast<const float2 *>(it.psi), dst, k, t,
                         reinterpret_cast<double *>(it.partials) + ((size_t)k * it.tiles + t) * 4, red);
+        else if (it.pad & QGTL_INJECT_ITEM_SPLIT)   // the state in split-plane form
+            inject_body<true, true, kInjectPerThread, false>(d, reinterpret_cast<const float2 *>(it.psi), dst, k, t,
+                                                             nullptr, red);
         else
             inject_body<false, true, kInjectPerThread, false>(d, reinterpret_cast<const float2 *>(it.psi), dst, k, t,
                                                               nullptr, red);
@@ -1224,7 +1228,10 @@ int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t un
 
 int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid, int variant) {
     QGTL_GROUPED_ARGS_OK(items, nitems, units);
-    if (variant)
+    if (variant == 2)
+        hipLaunchKernelGGL(pass_grouped_kernel<2>, dim3(grouped_grid(units, grid)), dim3(kThreads), 0,
+                           qgtl::stream(), items, nitems, units);
+    else if (variant)
         hipLaunchKernelGGL(pass_grouped_kernel<1>, dim3(grouped_grid(units, grid)), dim3(kThreads), 0,
                            qgtl::stream(), items, nitems, units);
     else

@@ -53,7 +53,9 @@
 #define GRAM_KS QGTL_GRAM_KSTEP
 #define XCDS 8
 
-enum { BL_FILL, BL_PASS, BL_INJECT, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL };
+enum { BL_FILL, BL_PASS, BL_INJECT, BL_GRAM, BL_GRAM_REDUCE, BL_FINAL, BL_GRAM_CPLX, BL_GRAM_REDUCE_CPLX, BL_FINAL_CPLX };
+
+#define CPLX_CHUNK 1024           /* amplitudes per split-K chunk of the complex-form Gram */
 
 typedef struct {
     int kind;
@@ -80,6 +82,8 @@ typedef struct {
     int nfwd;                  /* derivative states of blocks <= meet (slots 1 .. nfwd) */
     int final_arena;           /* 0: x, 1: y -- where all derivative states end */
     size_t pass_off, inj_off;  /* the instance's first pass / injection descriptor */
+    int cplx;                  /* final states in split-plane form and the complex-form Gram (129..256 Gram
+                                  states, register final passes); else complex64 and the real-form Gram */
 } inst_t;
 
 struct qgtl_batch {
@@ -239,8 +243,32 @@ static int push_items(struct qgtl_batch *B, const void *items, size_t bytes, int
 static uint64_t dev_addr(const void *base, size_t byte_off) { return (uint64_t)(uintptr_t)base + byte_off; }
 
 /* Gram chunking of instance i for the current tune. */
+/* The final pass of instance i's forward group (block meet's last pass) and of its
+ * backward group (block meet + 1's last inverse pass) can store the split-plane form. */
+static int split_final_ok(const struct qgtl_batch *B, const inst_t *in)
+{
+    const qgtl_plan_t *p = in->plan;
+    const qgtl_block_t *bm = &p->blocks[in->meet];
+    if (bm->npasses < 1 || !regs_variant(&bm->pass[bm->npasses - 1], B->tune.pass_variant)) return 0;
+    if (in->meet < p->nblocks - 1) {
+        const qgtl_block_t *bb = &p->blocks[in->meet + 1];
+        if (bb->ninv < 1 || !regs_variant(&bb->ipass[bb->ninv - 1], B->tune.pass_variant)) return 0;
+    }
+    return 1;
+}
+
 static void gram_shape(const struct qgtl_batch *B, inst_t *in)
 {
+    const int states = in->P + 1;
+    in->cplx = (in->terms == QGTL_GRAM_F16X3 || in->terms == QGTL_GRAM_F16X4) && states > GRAM_TILE &&
+               states <= 2 * GRAM_TILE && in->n >= 14 && split_final_ok(B, in);
+    if (in->cplx) {
+        in->nchunks = (int)((in->N + CPLX_CHUNK - 1) / CPLX_CHUNK);
+        in->chunk_len = CPLX_CHUNK;
+        in->nt = 2;
+        in->T = 3;
+        return;
+    }
     const size_t want = B->tune.gram_chunk_len > 0 ? (size_t)B->tune.gram_chunk_len : 16384;
     size_t nch = (in->N + want - 1) / want;
     nch = (nch + XCDS - 1) / XCDS * XCDS;
@@ -366,8 +394,9 @@ static int build_schedule(struct qgtl_batch *B)
                     if (b < B->inst[i].plan->nblocks && B->inst[i].plan->blocks[b].npasses > maxp)
                         maxp = B->inst[i].plan->blocks[b].npasses;
                 for (int j = 0; j < maxp; j++) {
-                    /* one launch per kernel variant (register-phase / LDS body) */
-                    for (int var = 1; var >= 0; var--) {
+                    /* one launch per kernel variant (2: register phases storing the split-plane
+                     * form -- the forward group's final pass; 1: register phases; 0: LDS body) */
+                    for (int var = 2; var >= 0; var--) {
                         qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
                         int k = 0;
                         uint64_t u = 0;
@@ -378,8 +407,22 @@ static int build_schedule(struct qgtl_batch *B)
                             size_t o = 0;
                             for (int b2 = 0; b2 < b; b2++) o += (size_t)p->blocks[b2].npasses;
                             const size_t desc = sc[i].pass_desc + o + (size_t)j;
-                            if (regs_variant(&B->pass_h[desc], B->tune.pass_variant) != var) continue;
-                            const int nstates = b <= in->meet ? 1 + sc[i].fwd_slots : 1;
+                            const int rv = regs_variant(&B->pass_h[desc], B->tune.pass_variant);
+                            const int split = in->cplx && b == in->meet && j == p->blocks[b].npasses - 1 &&
+                                              sc[i].fwd_slots > 0;
+                            if (var == 2) {
+                                if (!split) continue;
+                                /* the forward states (slots 1..) in split-plane form; the primal's item
+                                 * (slot 0, the launch below) moves the arena */
+                                int a = sc[i].psi;
+                                rc = add_pass_item(B, it, &k, &u, i, desc, p->blocks[b].pass_flips[j], 1,
+                                                   sc[i].fwd_slots, &a);
+                                if (rc) goto out;
+                                it[k - 1].variant = 2;
+                                continue;
+                            }
+                            if (rv != var) continue;
+                            const int nstates = (b <= in->meet && !split) ? 1 + sc[i].fwd_slots : 1;
                             rc = add_pass_item(B, it, &k, &u, i, desc, p->blocks[b].pass_flips[j], 0, nstates,
                                                &sc[i].psi);
                             if (rc) goto out;
@@ -402,6 +445,10 @@ static int build_schedule(struct qgtl_batch *B)
                 /* forward states go where the primal is; the states of a later block where the
                  * backward group will be when they join it */
                 const int dst_arena = dir ? bwd_arena(in, b - 1) : sc[i].psi;
+                /* states injected right after the meeting block are final: split-plane form for the
+                 * complex-form Gram */
+                const uint32_t final_split =
+                    (in->cplx && b == (dir ? in->meet + 1 : in->meet)) ? QGTL_INJECT_ITEM_SPLIT : 0u;
                 for (int j0 = 0; j0 < K; j0 += QGTL_INJECT_MAX) {
                     const int c = (K - j0 < QGTL_INJECT_MAX) ? K - j0 : QGTL_INJECT_MAX;
                     const size_t s0 = (size_t)(blk->param_begin + j0);
@@ -418,7 +465,7 @@ static int build_schedule(struct qgtl_batch *B)
                     it[k].tiles = (uint32_t)in->tiles;
                     it[k].K = (uint32_t)c;
                     it[k].unit_begin = (uint32_t)u;
-                    it[k].pad = 0;
+                    it[k].pad = final_split;
                     u += (uint64_t)it[k].tiles * (uint64_t)c;
                     k++;
                 }
@@ -433,7 +480,7 @@ static int build_schedule(struct qgtl_batch *B)
                     it[k].tiles = (uint32_t)in->tiles;
                     it[k].K = 1;
                     it[k].unit_begin = (uint32_t)u;
-                    it[k].pad = 0;
+                    it[k].pad = in->cplx ? QGTL_INJECT_ITEM_SPLIT : 0u;
                     u += (uint64_t)it[k].tiles;
                     k++;
                 }
@@ -451,7 +498,7 @@ static int build_schedule(struct qgtl_batch *B)
     /* backward phase: block b' descending carries the states of blocks > b' through its inverse */
     for (int bb = maxb - 2; bb >= 0; bb--) {
         for (int j = 0; j < maxinv; j++) {
-            for (int var = 1; var >= 0; var--) {
+            for (int var = 2; var >= 0; var--) {
                 qgtl_pass_item_t *it = (qgtl_pass_item_t *)tmp;
                 int k = 0;
                 uint64_t u = 0;
@@ -462,11 +509,15 @@ static int build_schedule(struct qgtl_batch *B)
                     size_t o = 0;
                     for (int b2 = 0; b2 < bb; b2++) o += (size_t)p->blocks[b2].ninv;
                     const size_t desc = sc[i].inv_desc + o + (size_t)j;
-                    if (regs_variant(&B->pass_h[desc], B->tune.pass_variant) != var) continue;
+                    /* the backward group's final pass stores the split-plane form (variant 2) */
+                    const int split = in->cplx && bb == in->meet + 1 && j == p->blocks[bb].ninv - 1;
+                    const int v = split ? 2 : regs_variant(&B->pass_h[desc], B->tune.pass_variant);
+                    if (v != var) continue;
                     const int first = p->blocks[bb + 1].param_begin;
                     rc = add_pass_item(B, it, &k, &u, i, desc, p->blocks[bb].ipass_flips[j], 1 + first, in->P - first,
                                        &sc[i].bwd);
                     if (rc) goto out;
+                    if (split) it[k - 1].variant = 2;
                 }
                 rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_PASS, var, k, u);
                 if (rc) goto out;
@@ -481,58 +532,70 @@ static int build_schedule(struct qgtl_batch *B)
         }
     static const int term_classes[5] = {6, 4, 3, QGTL_GRAM_F16X3, QGTL_GRAM_F16X4};
     for (int tc = 0; tc < 5; tc++) {
-        qgtl_gram_item_t *it = (qgtl_gram_item_t *)tmp;
+        for (int form = 0; form < 2; form++) {          /* 0: real form (complex64 states); 1: complex form */
+            qgtl_gram_item_t *it = (qgtl_gram_item_t *)tmp;
+            int k = 0;
+            uint64_t u = 0;
+            for (int i = 0; i < cnt; i++) {
+                const inst_t *in = &B->inst[i];
+                if (in->terms != term_classes[tc] || in->cplx != form) continue;
+                it[k].D = dev_addr(B->states, (arena_off(in, in->final_arena) + in->N) * c64);
+                it[k].partials = dev_addr(B->gram_partials, in->gp_off * sizeof(float));
+                it[k].N = in->N;
+                it[k].chunk_len = in->chunk_len;
+                it[k].P = in->P + 1;                       /* the derivative states and the primal */
+                it[k].nt = in->nt;
+                it[k].nchunks = in->nchunks;
+                it[k].unit_begin = (uint32_t)u;
+                /* real form: tile pairs x chunks (a multiple of 8, see gram_shape); complex: chunks */
+                u += form ? (uint64_t)in->nchunks : (uint64_t)in->T * (uint64_t)in->nchunks;
+                B->gram_flops += 4.0 * (double)in->N * in->P * (in->P + 1);
+                k++;
+            }
+            rc = push_items(B, it, sizeof(*it) * (size_t)k, form ? BL_GRAM_CPLX : BL_GRAM,
+                            form ? (term_classes[tc] == QGTL_GRAM_F16X4 ? 4 : 3) : term_classes[tc], k, u);
+            if (rc) goto out;
+        }
+    }
+    for (int form = 0; form < 2; form++) {
+        qgtl_gram_reduce_item_t *it = (qgtl_gram_reduce_item_t *)tmp;
         int k = 0;
         uint64_t u = 0;
         for (int i = 0; i < cnt; i++) {
             const inst_t *in = &B->inst[i];
-            if (in->terms != term_classes[tc]) continue;
-            it[k].D = dev_addr(B->states, (arena_off(in, in->final_arena) + in->N) * c64);
+            if (in->cplx != form) continue;
             it[k].partials = dev_addr(B->gram_partials, in->gp_off * sizeof(float));
-            it[k].N = in->N;
-            it[k].chunk_len = in->chunk_len;
-            it[k].P = in->P + 1;                       /* the derivative states and the primal */
+            it[k].C = dev_addr(B->C, in->c_off * sizeof(double));
             it[k].nt = in->nt;
             it[k].nchunks = in->nchunks;
             it[k].unit_begin = (uint32_t)u;
-            u += (uint64_t)in->T * (uint64_t)in->nchunks;   /* a multiple of 8 */
-            B->gram_flops += 4.0 * (double)in->N * in->P * (in->P + 1);
+            it[k].pad = 0;
+            /* units: 256-element groups (real form), 64-entry groups (complex form) */
+            u += (uint64_t)in->T * (form ? GRAM_TILE * GRAM_TILE / 64 : GRAM_TILE * GRAM_TILE / 256);
             k++;
         }
-        rc = push_items(B, it, sizeof(*it) * (size_t)k, BL_GRAM, term_classes[tc], k, u);
+        rc = push_items(B, it, sizeof(*it) * (size_t)k, form ? BL_GRAM_REDUCE_CPLX : BL_GRAM_REDUCE, 0, k, u);
         if (rc) goto out;
     }
-    {
-        qgtl_gram_reduce_item_t *it = (qgtl_gram_reduce_item_t *)tmp;
-        uint64_t u = 0;
-        for (int i = 0; i < cnt; i++) {
-            const inst_t *in = &B->inst[i];
-            it[i].partials = dev_addr(B->gram_partials, in->gp_off * sizeof(float));
-            it[i].C = dev_addr(B->C, in->c_off * sizeof(double));
-            it[i].nt = in->nt;
-            it[i].nchunks = in->nchunks;
-            it[i].unit_begin = (uint32_t)u;
-            it[i].pad = 0;
-            u += (uint64_t)in->T * (GRAM_TILE * GRAM_TILE / 256);
-        }
-        rc = push_items(B, it, sizeof(*it) * (size_t)cnt, BL_GRAM_REDUCE, 0, cnt, u);
-        if (rc) goto out;
-    }
-    {
+    for (int form = 0; form < 2; form++) {
         qgtl_finalize_item_t *it = (qgtl_finalize_item_t *)tmp;
+        int k = 0;
         uint64_t u = 0;
+        const uint64_t per = form ? 512 : 256;           /* entries per unit (the kernels' workgroup sizes) */
         for (int i = 0; i < cnt; i++) {
             const inst_t *in = &B->inst[i];
-            it[i].C = dev_addr(B->C, in->c_off * sizeof(double));
-            it[i].inj = 0;                             /* a_k and the norms from the Gram's primal row */
-            it[i].Q = dev_addr(B->Qdev, in->q_off * 2 * sizeof(double));
-            it[i].R = in->nt * GRAM_TILE;
-            it[i].P = in->P;
-            it[i].unit_begin = (uint32_t)u;
-            it[i].pad = 0;
-            u += ((uint64_t)in->P * in->P + 255) / 256;
+            if (in->cplx != form) continue;
+            it[k].C = dev_addr(B->C, in->c_off * sizeof(double));
+            it[k].inj = 0;                             /* a_k and the norms from the Gram's primal row */
+            it[k].Q = dev_addr(B->Qdev, in->q_off * 2 * sizeof(double));
+            it[k].R = in->nt * GRAM_TILE;
+            it[k].P = in->P;
+            it[k].unit_begin = (uint32_t)u;
+            it[k].pad = 0;
+            u += ((uint64_t)in->P * in->P + per - 1) / per;
+            k++;
         }
-        rc = push_items(B, it, sizeof(*it) * (size_t)cnt, BL_FINAL, 0, cnt, u);
+        rc = push_items(B, it, sizeof(*it) * (size_t)k, form ? BL_FINAL_CPLX : BL_FINAL, 0, k, u);
         if (rc) goto out;
     }
 out:
@@ -548,6 +611,7 @@ static int grid_for(const struct qgtl_batch *B, const launch_t *L)
     case BL_PASS: g = B->tune.pass_grid == 0 ? 2 * c : B->tune.pass_grid; break;
     case BL_INJECT: g = B->tune.inject_grid == 0 ? 8 * c : B->tune.inject_grid; break;
     case BL_GRAM: g = B->tune.gram_grid == 0 ? 2 * c : B->tune.gram_grid; break;
+    case BL_GRAM_CPLX: g = B->tune.gram_grid == 0 ? c : B->tune.gram_grid; break;   /* 128 KB of LDS: one per CU */
     default: g = 8 * c;
     }
     return g < 0 ? 0 : g;    /* 0: one workgroup per unit */
@@ -571,6 +635,15 @@ static int issue(struct qgtl_batch *B)
         case BL_GRAM_REDUCE:
             BTRY(qgtl_hip_gram_reduce_grouped((const qgtl_gram_reduce_item_t *)items, L->nitems, L->units, g));
             break;
+        case BL_GRAM_CPLX:
+            BTRY(qgtl_hip_gram_cplx_grouped((const qgtl_gram_item_t *)items, L->nitems, L->units, g, L->terms));
+            break;
+        case BL_GRAM_REDUCE_CPLX:
+            BTRY(qgtl_hip_gram_cplx_reduce_grouped((const qgtl_gram_reduce_item_t *)items, L->nitems, L->units, g));
+            break;
+        case BL_FINAL_CPLX:
+            BTRY(qgtl_hip_finalize_cplx_grouped((const qgtl_finalize_item_t *)items, L->nitems, L->units, g));
+            break;
         default: BTRY(qgtl_hip_finalize_grouped((const qgtl_finalize_item_t *)items, L->nitems, L->units, g));
         }
     }
@@ -587,7 +660,7 @@ static int rebuild(struct qgtl_batch *B)
         inst_t *in = &B->inst[i];
         gram_shape(B, in);
         in->gp_off = gp;
-        gp += (size_t)in->nchunks * (size_t)in->T * GRAM_TILE * GRAM_TILE;
+        gp += (size_t)in->nchunks * (size_t)in->T * GRAM_TILE * GRAM_TILE * (in->cplx ? 2 : 1);
     }
     if (gp > B->gp_len) {
         qgtl_hip_free(B->gram_partials);

@@ -81,6 +81,25 @@ def test_batch_two_sided_schedule():
     fwd.close()
 
 
+def test_batch_complex_gram_instances():
+    """Instances with 129..256 Gram states (P + 1, with the primal) write their final states in
+    split-plane form and run the complex-form Gram (one workgroup per chunk for the three tile
+    pairs); the others in the same batch keep complex64 states and the real-form Gram."""
+    shapes = [(16, 4), (14, 9), (15, 5), (17, 4), (14, 5), (14, 3), (13, 6)]
+    cir = _circuits(shapes, 3100)
+    b = Q.Batch([(n, Q.hea_ops(n, L, th)) for n, L, th in cir])
+    for rnd in range(2):
+        Qs, st = b.execute()
+        for (n, L, th), got in zip(cir, Qs):
+            ref = O.qgt(n, O.hea_ops(n, L, th))
+            err = Q.rel_err(got, ref)
+            print(f"round {rnd}: n={n} L={L} P={2 * n * L} rel_err={err:.3e}", flush=True)
+            assert err <= TOL, (n, L, err)
+        cir = [(n, L, O.uniform_angles(3200 + i, 2 * n * L)) for i, (n, L, _) in enumerate(cir)]
+        b.set_angles([Q.hea_ops(n, L, th)["angle"] for n, L, th in cir])
+    b.close()
+
+
 def test_batch_graph_replays_match_eager_and_new_angles():
     cir = _circuits([(4, 2), (10, 3), (13, 2), (14, 2), (15, 2)], 950)
     circuits = [(n, Q.hea_ops(n, L, th)) for n, L, th in cir]
