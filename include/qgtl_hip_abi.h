@@ -92,17 +92,29 @@ typedef struct {
      * from LDS (register bits ph_rbits[0]; lanes 0..3 on independent banks
      * through the ring map) */
     int32_t gather_tbits[8];
-    /* Rotation form in the register-phase kernel: the gate on register bit j
-     * of phase p is its rotation with one factor taken out, (a, b) ->
-     * (a - t b, b + t a) with t = s / c (|c| >= |s|), or (t a - b, a + t b)
-     * with t = c / s when bit j of ph_uform[p] is set, t = ph_tan[p][j]; the
-     * product of the factors taken out (c or s of every gate) is scale.  A
-     * phase layer is (the tables at the layout's thread bits) x (its register
-     * part): reg_post[r] / reg_pre[r] is the product, over the register bits
-     * set in r, of phase[1] / phase[0] of the gate on that tile bit (last /
-     * first layout).  has_post = 0: every post phase is 1. */
+    /* Rotation form in the register-phase kernel: every gate on register bit
+     * j of phase p is K(t) = [[1, -t], [t, 1]], (a, b) -> (a - t b, b + t a),
+     * t = ph_tan[p][j].  A rotation with |c| >= |s| is c K(s / c).  One with
+     * |c| < |s| (its tile bit set in xmask) is X Z s K(-c / s) = X s K(c / s) Z,
+     * X = [[0, 1], [1, 0]], Z = diag(1, -1): the swap X of its amplitude pair
+     * is left to the store (element j of the tile is stored at tile index
+     * j ^ xmask, an XOR on the LDS address of the last transpose), and the
+     * sign Z goes into a phase layer:
+     *   xfold = 0 (a post layer is applied): t = -c / s; the post layer reads
+     *     its tables at j ^ xmask and multiplies by (-1)^popcount(j & xmask)
+     *     (X diag(p0, p1) = diag(p1, p0) X);
+     *   xfold = 1 (only a pre layer, has_pre = 1, has_post = 0): t = c / s;
+     *     the pre layer multiplies by (-1)^popcount(j & xmask).
+     * The product of the factors taken out (c or s of every gate) is scale.
+     * A phase layer is (the tables at the layout's thread bits) x (its
+     * register part): reg_post[r] / reg_pre[r] is the product, over the
+     * register bits set in r, of the layer's phase ratio on that tile bit
+     * (last / first layout): phase[1] / phase[0] of the gate, for a bit of
+     * xmask conj(phase[1] / phase[0]) with the sign of the fold.  has_post =
+     * 0: every post phase is 1. */
     float ph_tan[QGTL_MAX_PHASES][5];
-    int32_t ph_uform[QGTL_MAX_PHASES];
+    uint32_t xmask;
+    int32_t xfold;
     int32_t has_post;
     float scale;
     qgtl_c64_t reg_post[32];

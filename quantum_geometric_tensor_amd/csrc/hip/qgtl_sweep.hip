@@ -390,9 +390,9 @@ struct RingLds {
     uint32_t bit[5];
 };
 
-__device__ __forceinline__ RingLds ring_lds(const RegLayout &L, uint32_t par_r) {
+__device__ __forceinline__ RingLds ring_lds(const RegLayout &L, uint32_t par_r, uint32_t xm = 0) {
     RingLds q;
-    q.base = (uint32_t)rpad(ring_lin(L.tdep) ^ par_r);
+    q.base = (uint32_t)rpad(ring_lin(L.tdep ^ xm) ^ par_r);
 #pragma unroll
     for (int i = 0; i < 5; i++) q.bit[i] = (uint32_t)rpad(ring_lin(L.rb[i]));
     return q;
@@ -454,21 +454,17 @@ __device__ __forceinline__ void apply_reg(float2 (&v)[kRegs], const float2 *m) {
         if (!((r >> P) & 1)) apply2(v[r], v[r | (1 << P)], m);
 }
 
-// Rotation with its factor taken out: (a, b) -> (a - t b, b + t a) (UFORM =
-// false, t = s / c) or (t a - b, a + t b) (t = c / s): 2 FMA per amplitude.
-template <int P, bool UFORM>
+// Rotation with its factor taken out, K(t): (a, b) -> (a - t b, b + t a),
+// 2 FMA per amplitude (the host folds the rest of the rotation into the
+// phase layers and the store index).
+template <int P>
 __device__ __forceinline__ void apply_reg_scaled(float2 (&v)[kRegs], float t) {
 #pragma unroll
     for (int r = 0; r < kRegs; r++) {
         if ((r >> P) & 1) continue;
         const float2 a = v[r], b = v[r | (1 << P)];
-        if (UFORM) {
-            v[r] = make_float2(fmaf(t, a.x, -b.x), fmaf(t, a.y, -b.y));
-            v[r | (1 << P)] = make_float2(fmaf(t, b.x, a.x), fmaf(t, b.y, a.y));
-        } else {
-            v[r] = make_float2(fmaf(-t, b.x, a.x), fmaf(-t, b.y, a.y));
-            v[r | (1 << P)] = make_float2(fmaf(t, a.x, b.x), fmaf(t, a.y, b.y));
-        }
+        v[r] = make_float2(fmaf(-t, b.x, a.x), fmaf(-t, b.y, a.y));
+        v[r | (1 << P)] = make_float2(fmaf(t, a.x, b.x), fmaf(t, a.y, b.y));
     }
 }
 
@@ -477,7 +473,7 @@ __device__ __forceinline__ void apply_reg_scaled(float2 (&v)[kRegs], float t) {
 // scalar loads are in flight during the previous phase's work.
 struct PhaseParams {
     int32_t rbits[5], tbits[8];
-    int32_t mask, uform;
+    int32_t mask;
     float tan[5];
 };
 
@@ -488,19 +484,15 @@ __device__ __forceinline__ PhaseParams load_phase(const qgtl_pass_desc_t &d, int
 #pragma unroll
     for (int i = 0; i < 8; i++) q.tbits[i] = d.ph_tbits[ph][i];
     q.mask = d.ph_mask[ph];
-    q.uform = d.ph_uform[ph];
 #pragma unroll
     for (int i = 0; i < 5; i++) q.tan[i] = d.ph_tan[ph][i];
     return q;
 }
 
-// Gate of register bit J in the phase, if any (scalar branches, compile-time pairing).
+// Gate of register bit J in the phase, if any (a scalar branch, compile-time pairing).
 template <int J>
 __device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const PhaseParams &q) {
-    if ((q.mask >> J) & 1) {
-        if ((q.uform >> J) & 1) apply_reg_scaled<J, true>(v, q.tan[J]);
-        else apply_reg_scaled<J, false>(v, q.tan[J]);
-    }
+    apply_reg_scaled<J>(v, q.tan[J]);
 }
 
 template <int J>
@@ -664,12 +656,18 @@ constexpr int kRegTileElems = 1 << kRT;
 // Phase layer of a rotation-form pass in layout L: the tables give the phase
 // of the thread's register-0 element (register bits 0), the descriptor the
 // register part, and scale the factors the scaled-out rotations dropped.
+// The swapped pairs of d.xmask: the post layer reads the tables at j ^ xmask,
+// and the layer of d.xfold takes the sign (-1)^popcount(j & xmask) (its
+// register part is in reg_post / reg_pre).
 template <bool POST>
 __device__ __forceinline__ void phase_layer(const float2 *tab, const RegLayout &L, const qgtl_pass_desc_t &d,
                                             float scale, float2 (&v)[kRegs]) {
-    float2 pt = phase_at(tab, L.tdep);
-    pt.x *= scale;
-    pt.y *= scale;
+    const uint32_t xm = (uint32_t)d.xmask;
+    const bool signed_layer = POST ? d.xfold == 0 : d.xfold == 1;
+    float2 pt = phase_at(tab, POST ? (L.tdep ^ xm) : L.tdep);
+    const float sc = (signed_layer && (__popc(L.tdep & xm) & 1)) ? -scale : scale;
+    pt.x *= sc;
+    pt.y *= sc;
     RegLoop<kRegs>::template phase_split<POST>(pt, d, v);
 }
 
@@ -782,12 +780,18 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
     }
     if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
 
+    // the last transpose stores element j at tile index j ^ xmask (the pair
+    // swaps of the rotation form); without one (a single natural phase) a
+    // swap adds it
+    const uint32_t xm = rotf ? (uint32_t)d.xmask : 0u;
     if (ring_store) {
-        RegLoop<kRegs>::store_lds_ring(tile, ring_lds(Lc, m.f.par_r), v);
+        RegLoop<kRegs>::store_lds_ring(tile, ring_lds(Lc, m.f.par_r, xm), v);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
-    } else if (d.nphase > 1 || ring_load) {
-        RegLoop<kRegs>::store_lds(tile, Lc, v);
+    } else if (d.nphase > 1 || ring_load || xm) {
+        RegLayout Ls = Lc;
+        Ls.lbase ^= (uint32_t)rpad(xm);
+        RegLoop<kRegs>::store_lds(tile, Ls, v);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
     }

@@ -243,49 +243,69 @@ static void finish_form(qgtl_pass_desc_t *p, int general)
 
 /* Register-kernel form of a finished rotation-form pass (see
  * qgtl_pass_desc_t): per gate the scaled-out rotation coefficient and the
- * factor taken out, and the register parts of the phase layers, from the
+ * factor taken out, the pair swaps left to the store (xmask) and where their
+ * signs go (xfold), and the register parts of the phase layers, from the
  * block's matrices u (FP64, rounded once). */
 static void finish_rotation(qgtl_pass_desc_t *p, const qgtl_mat2 *u)
 {
     p->has_post = 0;
     p->scale = 1.f;
-    for (int ph = 0; ph < QGTL_MAX_PHASES; ph++) p->ph_uform[ph] = 0;
+    p->xmask = 0;
+    p->xfold = 0;
+    for (int ph = 0; ph < QGTL_MAX_PHASES; ph++)
+        for (int j = 0; j < 5; j++) p->ph_tan[ph][j] = 0.f;
     for (int r = 0; r < 32; r++) p->reg_post[r] = p->reg_pre[r] = c64(cd(1.0, 0.0));
     if (p->form != 1) return;
-    double coef[QGTL_TILE_BITS_MAX], scale = 1.0;
-    int uform[QGTL_TILE_BITS_MAX], gate_of_bit[QGTL_TILE_BITS_MAX];
+    double cs[QGTL_TILE_BITS_MAX], sns[QGTL_TILE_BITS_MAX], coef[QGTL_TILE_BITS_MAX], scale = 1.0;
+    int swap[QGTL_TILE_BITS_MAX], gate_of_bit[QGTL_TILE_BITS_MAX];
     qgtl_cd rel_pre[QGTL_TILE_BITS_MAX], rel_post[QGTL_TILE_BITS_MAX];
     for (int b = 0; b < QGTL_TILE_BITS_MAX; b++) gate_of_bit[b] = -1;
     for (int g = 0; g < p->ngates; g++) {
         const int tb = p->gate_bit[g];
         const int q = tb < p->nlow ? tb : p->hi_pos[tb - p->nlow];
-        double c, sn;
         qgtl_cd pre[2], post[2];
-        factor_rotation(&u[q], &c, &sn, pre, post);
-        uform[g] = fabs(c) < fabs(sn);
-        coef[g] = uform[g] ? c / sn : sn / c;
-        scale *= uform[g] ? sn : c;
+        factor_rotation(&u[q], &cs[g], &sns[g], pre, post);
         rel_pre[g] = cmul(pre[1], cconj(pre[0]));
         rel_post[g] = cmul(post[1], cconj(post[0]));
         if (post[0].re != 1.0 || post[0].im != 0.0 || post[1].re != 1.0 || post[1].im != 0.0) p->has_post = 1;
         gate_of_bit[tb] = g;
     }
+    /* the sign of a swapped gate goes into the layer the kernel applies:
+     * the post layer unless the pass has a pre layer and no post layer */
+    p->xfold = (p->has_pre && !p->has_post) ? 1 : 0;
+    for (int g = 0; g < p->ngates; g++) {
+        const double c = cs[g], sn = sns[g];
+        swap[g] = fabs(c) < fabs(sn);
+        if (swap[g]) {
+            p->xmask |= 1u << p->gate_bit[g];
+            coef[g] = p->xfold ? c / sn : -c / sn;
+            scale *= sn;
+        } else {
+            coef[g] = sn / c;
+            scale *= c;
+        }
+    }
     p->scale = (float)scale;
     if (p->nphase < 1) return;
     for (int ph = 0; ph < p->nphase; ph++)
-        for (int g = p->ph_first[ph]; g < p->ph_first[ph + 1]; g++) {
-            const int j = p->ph_greg[g];
-            p->ph_tan[ph][j] = (float)coef[g];
-            if (uform[g]) p->ph_uform[ph] |= 1 << j;
-        }
+        for (int g = p->ph_first[ph]; g < p->ph_first[ph + 1]; g++) p->ph_tan[ph][p->ph_greg[g]] = (float)coef[g];
     const int last = p->nphase - 1;
     for (int r = 0; r < 32; r++) {
         qgtl_cd a = cd(1.0, 0.0), b = cd(1.0, 0.0);
         for (int i = 0; i < 5; i++) {
             if (!((r >> i) & 1)) continue;
             const int gl = gate_of_bit[p->ph_rbits[last][i]], g0 = gate_of_bit[p->ph_rbits[0][i]];
-            if (gl >= 0) a = cmul(a, rel_post[gl]);
-            if (g0 >= 0) b = cmul(b, rel_pre[g0]);
+            if (gl >= 0) {
+                /* post layer: a swapped bit reads the other phase, signed under xfold = 0 */
+                qgtl_cd f = swap[gl] ? cconj(rel_post[gl]) : rel_post[gl];
+                if (swap[gl] && !p->xfold) f = cd(-f.re, -f.im);
+                a = cmul(a, f);
+            }
+            if (g0 >= 0) {
+                qgtl_cd f = rel_pre[g0];
+                if (swap[g0] && p->xfold) f = cd(-f.re, -f.im);
+                b = cmul(b, f);
+            }
         }
         p->reg_post[r] = c64(a);
         p->reg_pre[r] = c64(b);
