@@ -336,6 +336,8 @@ __device__ __forceinline__ int rpad(uint32_t j) { return (int)(j ^ (((j >> 4) ^ 
 // | (the register bits of r).  rpad is linear over GF(2) and the thread and
 // register parts are disjoint, so the LDS word of register r is lbase ^ (xor
 // of lbit over the bits of r): one XOR with a uniform value per access.
+// lbase and lbit hold byte offsets (8 bytes per word), so the XOR is the
+// whole address computation.
 struct RegLayout {
     uint32_t tdep, lbase;
     uint32_t rb[5], lbit[5];
@@ -346,14 +348,22 @@ __device__ __forceinline__ RegLayout make_layout(const int32_t *rbits, const int
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         L.rb[i] = qgtl::uniform(1u << rbits[i]);
-        L.lbit[i] = (uint32_t)rpad(L.rb[i]);
+        L.lbit[i] = (uint32_t)rpad(L.rb[i]) << 3;
     }
     uint32_t d = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) d |= ((t >> i) & 1u) << qgtl::uniform(tbits[i]);
     L.tdep = d;
-    L.lbase = (uint32_t)rpad(d);
+    L.lbase = (uint32_t)rpad(d) << 3;
     return L;
+}
+
+// the complex64 word at byte offset b of the LDS tile
+__device__ __forceinline__ float2 &lds_word(float2 *tile, uint32_t b) {
+    return *reinterpret_cast<float2 *>(reinterpret_cast<char *>(tile) + b);
+}
+__device__ __forceinline__ const float2 &lds_word(const float2 *tile, uint32_t b) {
+    return *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + b);
 }
 
 template <int R>
@@ -384,7 +394,8 @@ __device__ __forceinline__ uint32_t ring_lin(uint32_t x) {
 }
 
 // LDS words of the ring positions of a layout's elements: element x sits at
-// rpad(ring_lin(x) ^ par_r) = base ^ (xor of bit over its register bits).
+// rpad(ring_lin(x) ^ par_r) = base ^ (xor of bit over its register bits), as
+// byte offsets.
 struct RingLds {
     uint32_t base;
     uint32_t bit[5];
@@ -392,9 +403,9 @@ struct RingLds {
 
 __device__ __forceinline__ RingLds ring_lds(const RegLayout &L, uint32_t par_r, uint32_t xm = 0) {
     RingLds q;
-    q.base = (uint32_t)rpad(ring_lin(L.tdep ^ xm) ^ par_r);
+    q.base = (uint32_t)rpad(ring_lin(L.tdep ^ xm) ^ par_r) << 3;
 #pragma unroll
-    for (int i = 0; i < 5; i++) q.bit[i] = (uint32_t)rpad(ring_lin(L.rb[i]));
+    for (int i = 0; i < 5; i++) q.bit[i] = (uint32_t)rpad(ring_lin(L.rb[i])) << 3;
     return q;
 }
 
@@ -424,15 +435,25 @@ __device__ __forceinline__ size_t global_index(const GlobalMap &g) {
     return g.lane + u;
 }
 
+// State offset of tile bits nlow.. given as h (bit i of h is tile bit nlow + i,
+// at state bit hi_pos[i]).
+__device__ __forceinline__ uint32_t deposit_hi(const qgtl_pass_desc_t &d, uint32_t h) {
+    const int nhi = kRT - d.nlow;
+    uint32_t off = 0;
+    for (int i = 0; i < nhi; i++) off |= ((h >> i) & 1u) << d.hi_pos[i];
+    return off;
+}
+
 // natural placement: tile bits below nlow are state bits 0..nlow-1, the others
-// go to hi_pos (the deposit table hi_off), the tile index r to base
-__device__ __forceinline__ GlobalMap natural_map(const RegLayout &L, uint32_t base, const uint32_t *hi_off, int nlow) {
+// go to hi_pos, the tile index r to base
+__device__ __forceinline__ GlobalMap natural_map(const RegLayout &L, uint32_t base, const qgtl_pass_desc_t &d) {
     GlobalMap g;
+    const int nlow = d.nlow;
     const uint32_t lowmask = (1u << nlow) - 1u;
-    g.lane = (size_t)(base + hi_off[L.tdep >> nlow] + (L.tdep & lowmask));
+    g.lane = (size_t)(base + deposit_hi(d, L.tdep >> nlow) + (L.tdep & lowmask));
 #pragma unroll
     for (int i = 0; i < 5; i++)
-        g.bit[i] = qgtl::uniform((L.rb[i] & lowmask) ? L.rb[i] : hi_off[L.rb[i] >> nlow]);
+        g.bit[i] = qgtl::uniform((L.rb[i] & lowmask) ? L.rb[i] : deposit_hi(d, L.rb[i] >> nlow));
     return g;
 }
 
@@ -489,10 +510,15 @@ __device__ __forceinline__ PhaseParams load_phase(const qgtl_pass_desc_t &d, int
     return q;
 }
 
-// Gate of register bit J in the phase, if any (a scalar branch, compile-time pairing).
-template <int J>
-__device__ __forceinline__ void apply_phase_rot(float2 (&v)[kRegs], const PhaseParams &q) {
-    apply_reg_scaled<J>(v, q.tan[J]);
+// The gates of a phase (compile-time pairing), one per register bit: t = 0
+// where the phase has no gate (an exact identity), so the phase is straight-line
+// code and no register copies arise where branches would join.
+__device__ __forceinline__ void apply_phase_rots(float2 (&v)[kRegs], const PhaseParams &q) {
+    apply_reg_scaled<0>(v, q.tan[0]);
+    apply_reg_scaled<1>(v, q.tan[1]);
+    apply_reg_scaled<2>(v, q.tan[2]);
+    apply_reg_scaled<3>(v, q.tan[3]);
+    apply_reg_scaled<4>(v, q.tan[4]);
 }
 
 template <int J>
@@ -556,20 +582,20 @@ template <int R>
 struct RegLoop {
     __device__ __forceinline__ static void store_lds(float2 *tile, const RegLayout &L, const float2 (&v)[kRegs]) {
         RegLoop<R - 1>::store_lds(tile, L, v);
-        tile[lds_index<R - 1>(L)] = v[R - 1];
+        lds_word(tile, lds_index<R - 1>(L)) = v[R - 1];
     }
     __device__ __forceinline__ static void load_lds(const float2 *tile, const RegLayout &L, float2 (&v)[kRegs]) {
         RegLoop<R - 1>::load_lds(tile, L, v);
-        v[R - 1] = tile[lds_index<R - 1>(L)];
+        v[R - 1] = lds_word(tile, lds_index<R - 1>(L));
     }
     // element x of the layout to / from its ring position
     __device__ __forceinline__ static void store_lds_ring(float2 *tile, const RingLds &q, const float2 (&v)[kRegs]) {
         RegLoop<R - 1>::store_lds_ring(tile, q, v);
-        tile[ring_index<R - 1>(q)] = v[R - 1];
+        lds_word(tile, ring_index<R - 1>(q)) = v[R - 1];
     }
     __device__ __forceinline__ static void load_lds_ring(const float2 *tile, const RingLds &q, float2 (&v)[kRegs]) {
         RegLoop<R - 1>::load_lds_ring(tile, q, v);
-        v[R - 1] = tile[ring_index<R - 1>(q)];
+        v[R - 1] = lds_word(tile, ring_index<R - 1>(q));
     }
     // phase layer as (thread part pt) x (register part from the descriptor)
     template <bool POST>
@@ -686,19 +712,31 @@ __device__ __forceinline__ RingFrame ring_frame(int n, uint32_t r) {
     return RingFrame{pre & ones, ones, (uint32_t)__popc(r) & 1u};
 }
 
-// Per-workgroup set-up of a register pass: phase tables, the deposit table
-// hi_off of the tile bits above nlow.
-__device__ __forceinline__ void pass_setup(const qgtl_pass_desc_t &d, uint32_t *hi_off, float2 *post_tab,
-                                           float2 *pre_tab, const float2 *gtabs) {
-    const int nhi = kRT - d.nlow;
+// Per-workgroup set-up of a register pass: the phase tables.  The host-built
+// tables (2 kPhaseTab complex64, post then pre) are fetched as one 16-byte
+// word per thread before the tile's global loads are issued and written to
+// LDS after them, so the fetch and the barrier overlap the tile's latency.
+__device__ __forceinline__ float4 fetch_tables(const qgtl_pass_desc_t &d, const float2 *gtabs) {
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (d.form == 1 && gtabs && threadIdx.x < (unsigned)kPhaseTab)
+        q = reinterpret_cast<const float4 *>(gtabs)[threadIdx.x];
+    return q;
+}
+
+__device__ __forceinline__ void pass_setup(const qgtl_pass_desc_t &d, float2 *post_tab, float2 *pre_tab,
+                                           const float2 *gtabs, float4 q) {
+    static_assert(kPhaseTab % 2 == 0 && kPhaseTab <= kThreads, "one 16-byte table word per thread");
     if (d.form == 1) {
-        if (gtabs) load_phase_tables(gtabs, post_tab, pre_tab);
-        else build_phase_tables(d, post_tab, pre_tab);
-    }
-    for (int h = threadIdx.x; h < (1 << nhi); h += kThreads) {
-        uint32_t off = 0;
-        for (int i = 0; i < nhi; i++) off |= ((uint32_t)(h >> i) & 1u) << d.hi_pos[i];
-        hi_off[h] = off;
+        if (gtabs) {
+            const uint32_t e = 2u * threadIdx.x;
+            if (threadIdx.x < (unsigned)kPhaseTab) {
+                float2 *w = e < (uint32_t)kPhaseTab ? post_tab + e : pre_tab + (e - kPhaseTab);
+                w[0] = make_float2(q.x, q.y);
+                w[1] = make_float2(q.z, q.w);
+            }
+        } else {
+            build_phase_tables(d, post_tab, pre_tab);
+        }
     }
     __syncthreads();
 }
@@ -711,15 +749,14 @@ struct TileMaps {
     GlobalMap gin, gout;
 };
 
-__device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_t r, const RegLayout &L0,
-                                              const uint32_t *hi_off) {
+__device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_t r, const RegLayout &L0) {
     uint32_t base = 0;
     for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
     const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
     TileMaps m;
     m.f = RingFrame{0u, 0u, 0u};
     if (ring_load || ring_store) m.f = ring_frame(d.n, r);
-    const GlobalMap gnat = natural_map(L0, base, hi_off, d.nlow);
+    const GlobalMap gnat = natural_map(L0, base, d);
     const GlobalMap gring = (ring_load || ring_store) ? ring_map(L0, m.f.pre, m.f.ones) : gnat;
     m.gin = ring_load ? gring : gnat;
     m.gout = ring_store ? gring : gnat;
@@ -764,11 +801,7 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         }
         const int mask = cur.mask;
         if (rotf) {
-            apply_phase_rot<0>(v, cur);
-            apply_phase_rot<1>(v, cur);
-            apply_phase_rot<2>(v, cur);
-            apply_phase_rot<3>(v, cur);
-            apply_phase_rot<4>(v, cur);
+            apply_phase_rots(v, cur);
         } else {
             apply_phase_mat<0>(v, d, ph, mask);
             apply_phase_mat<1>(v, d, ph, mask);
@@ -790,7 +823,7 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         RegLoop<kRegs>::load_lds(tile, L0, v);
     } else if (d.nphase > 1 || ring_load || xm) {
         RegLayout Ls = Lc;
-        Ls.lbase ^= (uint32_t)rpad(xm);
+        Ls.lbase ^= (uint32_t)rpad(xm) << 3;
         RegLoop<kRegs>::store_lds(tile, Ls, v);
         __syncthreads();
         RegLoop<kRegs>::load_lds(tile, L0, v);
@@ -802,19 +835,21 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
 }
 
 // One tile r of one state.  inj != nullptr: src is the primal state the
-// derivative state *inj is injected from while loading.
+// derivative state *inj is injected from while loading.  The global loads go
+// out first; the phase tables are set up while they are in flight.
 template <bool SPLIT = false, bool NT = false>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
-                                               float2 *dst, uint32_t r, float2 *tile, uint32_t *hi_off,
-                                               float2 *post_tab, float2 *pre_tab, const float2 *gtabs = nullptr,
+                                               float2 *dst, uint32_t r, float2 *tile, float2 *post_tab,
+                                               float2 *pre_tab, const float2 *gtabs = nullptr,
                                                const qgtl_inject_slot_t *inj = nullptr) {
-    pass_setup(d, hi_off, post_tab, pre_tab, gtabs);
+    const float4 tq = fetch_tables(d, gtabs);
     const RegLayout L0 = make_layout(d.ph_rbits[0], d.ph_tbits[0], threadIdx.x);
-    const TileMaps m = tile_maps(d, r, L0, hi_off);
+    const TileMaps m = tile_maps(d, r, L0);
     float2 v[kRegs];
     if (inj && inj->v == 0) RegLoop<kRegs>::load_global_inject_diag(src, m.gin, *inj, v);
     else if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
     else RegLoop<kRegs>::template load_global<NT>(src, m.gin, v);
+    pass_setup(d, post_tab, pre_tab, gtabs, tq);
     pass_compute_store<SPLIT, NT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
 }
 
@@ -824,10 +859,9 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
                                                                  float2 *dst_all, size_t stride,
                                                                  const float2 *__restrict__ gtabs) {
     __shared__ float2 tile[kRegTileElems];
-    __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
     pass_regs_body<SPLIT, NT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
-                              blockIdx.x, tile, hi_off, tabs, tabs + kPhaseTab, gtabs);
+                              blockIdx.x, tile, tabs, tabs + kPhaseTab, gtabs);
 }
 
 // The first pass of a derivative-state chunk with the injection folded into
@@ -843,7 +877,6 @@ __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pa
                                                                    const float2 *__restrict__ gtabs,
                                                                    uint32_t nstates, uint32_t ntiles) {
     __shared__ float2 tile[kRegTileElems];
-    __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
     // one-dimensional grid of nstates x ntiles workgroups.  Workgroup b runs on
     // XCD b % 8 (round-robin dispatch); with ntiles a multiple of 8, XCD x takes
@@ -861,8 +894,7 @@ __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pa
         t = b / nstates;
     }
     const qgtl_inject_slot_t q = inj[s];
-    pass_regs_body<SPLIT, true>(d, psi, dst_all + (size_t)s * stride, t, tile, hi_off, tabs, tabs + kPhaseTab, gtabs,
-                                &q);
+    pass_regs_body<SPLIT, true>(d, psi, dst_all + (size_t)s * stride, t, tile, tabs, tabs + kPhaseTab, gtabs, &q);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -1149,8 +1181,8 @@ __global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pa
         __syncthreads();
         const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
         float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
-        if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
-        else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
+        else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
         else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
     }
 }
