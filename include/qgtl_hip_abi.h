@@ -119,6 +119,22 @@ typedef struct {
     float scale;
     qgtl_c64_t reg_post[32];
     qgtl_c64_t reg_pre[32];
+    /* Phase layers moved between the two register passes of a block (the
+     * register kernel only; the LDS kernel applies every pass's own tables).
+     * nolayer = 1: the pass applies no phase layer and no scale; its layer
+     * (phases, the factors taken out, the signs of its swapped gates) is a
+     * product over its gate bits, which are tile-index bits of its partner
+     * pass, and the partner applies it as one complex factor per tile.
+     * tf_n > 0: the pass multiplies every amplitude of tile r by the product
+     * over i < tf_n of tf[i][bit tf_bit[i] of r], in the layer that carries
+     * its scale; the partner is the previous pass of the block (its post
+     * layer, on the input bits of this pass's tile index) or the next pass
+     * of the inverse block (its pre layer, on the output bits of this pass's
+     * tile index, whose store is natural). */
+    int32_t nolayer;
+    int32_t tf_n;
+    int32_t tf_bit[QGTL_TILE_BITS_MAX];
+    qgtl_c64_t tf[QGTL_TILE_BITS_MAX][2];
 } qgtl_pass_desc_t;
 
 /* Derivative-state injection in the frame after a block:

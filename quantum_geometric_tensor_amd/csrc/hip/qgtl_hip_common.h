@@ -60,6 +60,15 @@ __device__ __forceinline__ T *global_ptr(uint64_t a) {
     return (T *)(gT *)a;
 }
 
+// The same for data that no kernel writes while the launch runs (descriptor
+// tables written before it): the constant address space, so uniform reads
+// through it become scalar loads into SGPRs.
+template <class T>
+__device__ __forceinline__ const T *const_ptr(uint64_t a) {
+    typedef __attribute__((address_space(4))) const T cT;
+    return (const T *)(cT *)a;
+}
+
 }  // namespace qgtl
 
 // Map a HIP status to the library's error codes and remember the message.

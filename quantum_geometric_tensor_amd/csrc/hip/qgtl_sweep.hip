@@ -436,11 +436,17 @@ __device__ __forceinline__ size_t global_index(const GlobalMap &g) {
 }
 
 // State offset of tile bits nlow.. given as h (bit i of h is tile bit nlow + i,
-// at state bit hi_pos[i]).
+// at state bit hi_pos[i]).  Fixed trip count with unconditional reads of the
+// (zero-filled) table and masked bits, so the scalar loads issue together
+// instead of one load and one wait per bit ahead of the tile's loads.
 __device__ __forceinline__ uint32_t deposit_hi(const qgtl_pass_desc_t &d, uint32_t h) {
     const int nhi = kRT - d.nlow;
     uint32_t off = 0;
-    for (int i = 0; i < nhi; i++) off |= ((h >> i) & 1u) << d.hi_pos[i];
+#pragma unroll
+    for (int i = 0; i < kHiTableBits; i++) {
+        const uint32_t bit = i < nhi ? (h >> i) & 1u : 0u;
+        off |= bit << ((uint32_t)d.hi_pos[i] & 31u);
+    }
     return off;
 }
 
@@ -685,16 +691,34 @@ constexpr int kRegTileElems = 1 << kRT;
 // The swapped pairs of d.xmask: the post layer reads the tables at j ^ xmask,
 // and the layer of d.xfold takes the sign (-1)^popcount(j & xmask) (its
 // register part is in reg_post / reg_pre).
+// f: a factor common to the whole tile (the layer moved in from the partner
+// pass, or 1).
 template <bool POST>
 __device__ __forceinline__ void phase_layer(const float2 *tab, const RegLayout &L, const qgtl_pass_desc_t &d,
-                                            float scale, float2 (&v)[kRegs]) {
+                                            float scale, float2 f, float2 (&v)[kRegs]) {
     const uint32_t xm = (uint32_t)d.xmask;
     const bool signed_layer = POST ? d.xfold == 0 : d.xfold == 1;
-    float2 pt = phase_at(tab, POST ? (L.tdep ^ xm) : L.tdep);
+    float2 pt = cmul(phase_at(tab, POST ? (L.tdep ^ xm) : L.tdep), f);
     const float sc = (signed_layer && (__popc(L.tdep & xm) & 1)) ? -scale : scale;
     pt.x *= sc;
     pt.y *= sc;
     RegLoop<kRegs>::template phase_split<POST>(pt, d, v);
+}
+
+// The partner pass's layer as one factor for tile r: the product over the
+// moved gates of tf[i][bit tf_bit[i] of r] (1 when nothing was moved in).
+// Fixed trip count: the bit positions load together, then the selected
+// factors (a computed index, never a select between two loaded values); the
+// entries past tf_n are zero-filled and multiplied in as 1.  It runs after
+// the tile's loads are issued, so its scalar loads overlap them.
+__device__ __forceinline__ float2 tile_factor(const qgtl_pass_desc_t &d, uint32_t r) {
+    float2 f = make_float2(1.f, 0.f);
+    if (d.form != 1 || d.nolayer != 0) return f;
+    for (int i = 0; i < d.tf_n; i++) {
+        const qgtl_c64_t q = d.tf[i][(r >> d.tf_bit[i]) & 1u];
+        f = cmul(f, make_float2(q.re, q.im));
+    }
+    return f;
 }
 
 // Output high bits of a ring store from tile r (bits 0..12 of an n-qubit
@@ -718,7 +742,7 @@ __device__ __forceinline__ RingFrame ring_frame(int n, uint32_t r) {
 // LDS after them, so the fetch and the barrier overlap the tile's latency.
 __device__ __forceinline__ float4 fetch_tables(const qgtl_pass_desc_t &d, const float2 *gtabs) {
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (d.form == 1 && gtabs && threadIdx.x < (unsigned)kPhaseTab)
+    if (d.form == 1 && d.nolayer == 0 && gtabs && threadIdx.x < (unsigned)kPhaseTab)
         q = reinterpret_cast<const float4 *>(gtabs)[threadIdx.x];
     return q;
 }
@@ -726,7 +750,7 @@ __device__ __forceinline__ float4 fetch_tables(const qgtl_pass_desc_t &d, const 
 __device__ __forceinline__ void pass_setup(const qgtl_pass_desc_t &d, float2 *post_tab, float2 *pre_tab,
                                            const float2 *gtabs, float4 q) {
     static_assert(kPhaseTab % 2 == 0 && kPhaseTab <= kThreads, "one 16-byte table word per thread");
-    if (d.form == 1) {
+    if (d.form == 1 && d.nolayer == 0) {
         if (gtabs) {
             const uint32_t e = 2u * threadIdx.x;
             if (threadIdx.x < (unsigned)kPhaseTab) {
@@ -750,8 +774,14 @@ struct TileMaps {
 };
 
 __device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_t r, const RegLayout &L0) {
+    // state offset of the tile index (fixed trip count, as deposit_hi)
     uint32_t base = 0;
-    for (int i = 0; i < d.nrest; i++) base |= ((r >> i) & 1u) << d.rest_pos[i];
+    const int nrest = d.nrest;
+#pragma unroll
+    for (int i = 0; i < QGTL_MAX_QUBITS - kRT; i++) {
+        const uint32_t bit = i < nrest ? (r >> i) & 1u : 0u;
+        base |= bit << ((uint32_t)d.rest_pos[i] & 31u);
+    }
     const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
     TileMaps m;
     m.f = RingFrame{0u, 0u, 0u};
@@ -769,10 +799,15 @@ __device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_
 // form of the result (a final pass).
 template <bool SPLIT, bool NT = false>
 __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], const TileMaps &m,
-                                                   const RegLayout &L0, float2 *dst, float2 *tile,
+                                                   const RegLayout &L0, float2 tfac, float2 *dst, float2 *tile,
                                                    const float2 *post_tab, const float2 *pre_tab) {
     const uint32_t t = threadIdx.x;
     const bool rotf = d.form == 1;
+    // phase layers: none when the partner pass applies this pass's layer; the
+    // scale and the partner's moved-in layer go with the post layer if there
+    // is one, else with the pre layer
+    const bool layers = rotf && d.nolayer == 0;
+    const bool post_on = layers && (d.has_post || !d.has_pre), pre_on = layers && d.has_pre;
     const bool ring_load = d.load == QGTL_LOAD_RING, ring_store = d.store == QGTL_STORE_RING;
     RegLayout Lc = L0;
     if (ring_load) {
@@ -785,7 +820,8 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         RegLoop<kRegs>::load_lds_ring(tile, ring_lds(Lc, m.f.par_r), v);
         __syncthreads();
     }
-    if (rotf && d.has_pre) phase_layer<false>(pre_tab, Lc, d, d.has_post ? 1.f : d.scale, v);
+    if (pre_on)
+        phase_layer<false>(pre_tab, Lc, d, post_on ? 1.f : d.scale, post_on ? make_float2(1.f, 0.f) : tfac, v);
 
     PhaseParams cur = load_phase(d, 0);
     for (int ph = 0; ph < d.nphase; ph++) {
@@ -811,7 +847,7 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
         }
         cur = nxt;
     }
-    if (rotf && (d.has_post || !d.has_pre)) phase_layer<true>(post_tab, Lc, d, d.scale, v);
+    if (post_on) phase_layer<true>(post_tab, Lc, d, d.scale, tfac, v);
 
     // the last transpose stores element j at tile index j ^ xmask (the pair
     // swaps of the rotation form); without one (a single natural phase) a
@@ -849,8 +885,9 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     if (inj && inj->v == 0) RegLoop<kRegs>::load_global_inject_diag(src, m.gin, *inj, v);
     else if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
     else RegLoop<kRegs>::template load_global<NT>(src, m.gin, v);
+    const float2 tfac = tile_factor(d, r);
     pass_setup(d, post_tab, pre_tab, gtabs, tq);
-    pass_compute_store<SPLIT, NT>(d, v, m, L0, dst, tile, post_tab, pre_tab);
+    pass_compute_store<SPLIT, NT>(d, v, m, L0, tfac, dst, tile, post_tab, pre_tab);
 }
 
 template <bool SPLIT, bool NT = false>
@@ -1167,20 +1204,17 @@ __global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pa
     __shared__ float2 tile[kLdsTileElems];
     __shared__ uint32_t hi_off[1 << kHiTableBits];
     __shared__ float2 tabs[2 * kPhaseTab];
-    __shared__ qgtl_pass_desc_t sd;
     static_assert(kLdsTileElems >= kRegTileElems, "LDS tile too small for the register pass");
-    static_assert(sizeof(qgtl_pass_desc_t) % 4 == 0, "descriptor copy works in words");
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const qgtl_pass_item_t it = items[qgtl::find_item(items, nitems, u)];
         const uint32_t local = u - it.unit_begin;
         const uint32_t r = local % it.ntiles, s = local / it.ntiles;
         __syncthreads();   // the previous unit is done with LDS
-        const uint32_t *dw = reinterpret_cast<const uint32_t *>(it.desc);
-        uint32_t *sw = reinterpret_cast<uint32_t *>(&sd);
-        for (int w = threadIdx.x; w < (int)(sizeof(qgtl_pass_desc_t) / 4); w += kThreads) sw[w] = dw[w];
-        __syncthreads();
-        const float2 *src = reinterpret_cast<const float2 *>(it.src) + (size_t)s * it.stride;
-        float2 *dst = reinterpret_cast<float2 *>(it.dst) + (size_t)s * it.stride;
+        // the descriptor table is written before the launch: read it through
+        // the constant address space (scalar loads, no copy)
+        const qgtl_pass_desc_t &sd = *qgtl::const_ptr<qgtl_pass_desc_t>(qgtl::uniform(it.desc));
+        const float2 *src = qgtl::global_ptr<const float2>(qgtl::uniform(it.src)) + (size_t)s * it.stride;
+        float2 *dst = qgtl::global_ptr<float2>(qgtl::uniform(it.dst)) + (size_t)s * it.stride;
         if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
         else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
         else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);

@@ -252,6 +252,8 @@ static void finish_rotation(qgtl_pass_desc_t *p, const qgtl_mat2 *u)
     p->scale = 1.f;
     p->xmask = 0;
     p->xfold = 0;
+    p->nolayer = 0;
+    p->tf_n = 0;
     for (int ph = 0; ph < QGTL_MAX_PHASES; ph++)
         for (int j = 0; j < 5; j++) p->ph_tan[ph][j] = 0.f;
     for (int r = 0; r < 32; r++) p->reg_post[r] = p->reg_pre[r] = c64(cd(1.0, 0.0));
@@ -310,6 +312,60 @@ static void finish_rotation(qgtl_pass_desc_t *p, const qgtl_mat2 *u)
         p->reg_post[r] = c64(a);
         p->reg_pre[r] = c64(b);
     }
+}
+
+/* Register passes that the register kernel runs (the LDS kernel applies its
+ * own tables and ignores the moved layers). */
+static int reg_pass_ok(const qgtl_pass_desc_t *p)
+{
+    return p->form == 1 && p->m == QGTL_TILE_BITS_MAX && p->nphase >= 1 && p->nphase <= QGTL_MAX_PHASES &&
+           p->store != QGTL_STORE_PERM;
+}
+
+/* Move the phase layer of donor pass d into receiver pass r as a per-tile
+ * factor (see qgtl_pass_desc_t): every gate of d must act on a tile-index bit
+ * of r (r's tile is bits 0..12).  fwd = 1: d runs just before r and applies a
+ * post layer only (xfold = 0); its layer in d's output index y is, per gate
+ * bit, c diag(p0, p1) or, for a swapped gate, s diag(-p0, p1) (the post layer
+ * read at y with the sign of y ^ xmask).  fwd = 0: d runs just after r and
+ * applies a pre layer only (xfold = 1); its layer in d's input index is
+ * c diag(q0, q1), or s diag(q0, -q1) for a swapped gate.  u: the matrices d
+ * was finished with. */
+static void fold_layer(qgtl_pass_desc_t *d, qgtl_pass_desc_t *r, const qgtl_mat2 *u, int fwd)
+{
+    if (!reg_pass_ok(d) || !reg_pass_ok(r) || r->nlow != QGTL_TILE_BITS_MAX || r->tf_n || r->nolayer || d->tf_n)
+        return;
+    if (fwd ? (d->has_pre || d->xfold != 0 || r->load != QGTL_LOAD_NATURAL)
+            : (!d->has_pre || d->has_post || d->xfold != 1 || r->store != QGTL_STORE_INPLACE))
+        return;
+    if (d->ngates < 1 || d->ngates > QGTL_TILE_BITS_MAX) return;
+    int bit[QGTL_TILE_BITS_MAX];
+    for (int g = 0; g < d->ngates; g++) {
+        const int tb = d->gate_bit[g];
+        const int q = tb < d->nlow ? tb : d->hi_pos[tb - d->nlow];
+        bit[g] = -1;
+        for (int i = 0; i < r->nrest; i++)
+            if (r->rest_pos[i] == q) bit[g] = i;
+        if (bit[g] < 0) return;                 /* a gate inside r's tile: no fold */
+    }
+    for (int g = 0; g < d->ngates; g++) {
+        const int tb = d->gate_bit[g];
+        const int q = tb < d->nlow ? tb : d->hi_pos[tb - d->nlow];
+        double c, sn;
+        qgtl_cd pre[2], post[2];
+        factor_rotation(&u[q], &c, &sn, pre, post);
+        const int swapped = fabs(c) < fabs(sn);
+        const double f = swapped ? sn : c;
+        const qgtl_cd *ph = fwd ? post : pre;
+        qgtl_cd f0 = cd(f * ph[0].re, f * ph[0].im), f1 = cd(f * ph[1].re, f * ph[1].im);
+        if (swapped && fwd) f0 = cd(-f0.re, -f0.im);
+        if (swapped && !fwd) f1 = cd(-f1.re, -f1.im);
+        r->tf_bit[g] = bit[g];
+        r->tf[g][0] = c64(f0);
+        r->tf[g][1] = c64(f1);
+    }
+    r->tf_n = d->ngates;
+    d->nolayer = 1;
 }
 
 /* LDS bank of tile index j in the register kernel's swizzled tile:
@@ -750,6 +806,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
             finish_form(&b->pass[i], plan->tune.sweep_form == 1);
             finish_rotation(&b->pass[i], b->u);
         }
+        if (b->npasses == 2) fold_layer(&b->pass[0], &b->pass[1], b->u, 1);
         if (b->ninv > 0) {
             qgtl_mat2 uadj[QGTL_MAX_QUBITS];
             for (int q = 0; q < n; q++) uadj[q] = mat_adj(&b->u[q]);
@@ -757,6 +814,7 @@ int qgtl_plan_build(struct qgtl_plan *plan)
                 finish_form(&b->ipass[i], plan->tune.sweep_form == 1);
                 finish_rotation(&b->ipass[i], uadj);
             }
+            if (b->ninv == 2) fold_layer(&b->ipass[1], &b->ipass[0], uadj, 0);
         }
         /* images of the basis vectors under the inverse M-stage (the Z-frame masks below) */
         uint32_t inv_img[QGTL_MAX_QUBITS];
