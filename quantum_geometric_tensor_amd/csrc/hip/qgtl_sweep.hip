@@ -797,6 +797,38 @@ __device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_
 // through m.gin): the inverse ring's in-tile part, the phase layers and gate
 // phases, the ring or natural store.  SPLIT: dst receives the split-plane
 // form of the result (a final pass).
+// The phases of a register pass: phase p > 0 first moves the amplitudes into
+// its layout through the LDS tile; ROT: rotation form, else general matrices.
+template <bool ROT>
+__device__ __forceinline__ void run_phases(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], RegLayout &Lc,
+                                           float2 *tile) {
+    const uint32_t t = threadIdx.x;
+    PhaseParams cur = load_phase(d, 0);
+    for (int ph = 0; ph < d.nphase; ph++) {
+        PhaseParams nxt = cur;
+        if (ph + 1 < d.nphase) nxt = load_phase(d, ph + 1);
+        if (ph > 0) {
+            const RegLayout Ln = make_layout(cur.rbits, cur.tbits, t);
+            RegLoop<kRegs>::store_lds(tile, Lc, v);
+            __syncthreads();
+            RegLoop<kRegs>::load_lds(tile, Ln, v);
+            __syncthreads();
+            Lc = Ln;
+        }
+        if constexpr (ROT) {
+            apply_phase_rots(v, cur);
+        } else {
+            const int mask = cur.mask;
+            apply_phase_mat<0>(v, d, ph, mask);
+            apply_phase_mat<1>(v, d, ph, mask);
+            apply_phase_mat<2>(v, d, ph, mask);
+            apply_phase_mat<3>(v, d, ph, mask);
+            apply_phase_mat<4>(v, d, ph, mask);
+        }
+        cur = nxt;
+    }
+}
+
 template <bool SPLIT, bool NT = false>
 __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], const TileMaps &m,
                                                    const RegLayout &L0, float2 tfac, float2 *dst, float2 *tile,
@@ -823,30 +855,10 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
     if (pre_on)
         phase_layer<false>(pre_tab, Lc, d, post_on ? 1.f : d.scale, post_on ? make_float2(1.f, 0.f) : tfac, v);
 
-    PhaseParams cur = load_phase(d, 0);
-    for (int ph = 0; ph < d.nphase; ph++) {
-        PhaseParams nxt = cur;
-        if (ph + 1 < d.nphase) nxt = load_phase(d, ph + 1);
-        if (ph > 0) {
-            const RegLayout Ln = make_layout(cur.rbits, cur.tbits, t);
-            RegLoop<kRegs>::store_lds(tile, Lc, v);
-            __syncthreads();
-            RegLoop<kRegs>::load_lds(tile, Ln, v);
-            __syncthreads();
-            Lc = Ln;
-        }
-        const int mask = cur.mask;
-        if (rotf) {
-            apply_phase_rots(v, cur);
-        } else {
-            apply_phase_mat<0>(v, d, ph, mask);
-            apply_phase_mat<1>(v, d, ph, mask);
-            apply_phase_mat<2>(v, d, ph, mask);
-            apply_phase_mat<3>(v, d, ph, mask);
-            apply_phase_mat<4>(v, d, ph, mask);
-        }
-        cur = nxt;
-    }
+    // one phase loop per form, so the rotation form's straight-line gates share
+    // no joins with the general matrices' branches
+    if (rotf) run_phases<true>(d, v, Lc, tile);
+    else run_phases<false>(d, v, Lc, tile);
     if (post_on) phase_layer<true>(post_tab, Lc, d, d.scale, tfac, v);
 
     // the last transpose stores element j at tile index j ^ xmask (the pair

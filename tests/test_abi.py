@@ -66,6 +66,18 @@ def test_compute_backend_registers_with_qgtl_registry():
     assert "HIP" in res.stdout
 
 
+def test_register_pass_descriptors_match_dense_blocks():
+    """Every register pass the plan builder emits, applied in FP64 on the host by the
+    descriptor contract of include/qgtl_hip_abi.h (single-form K(t) gates, the store at
+    j ^ xmask, the fold signs, the phase layers moved between a block's two passes), composes
+    to each block and each inverse block applied densely (tests/c/test_pass_form.c): HEA and
+    RX-RY-RZ ansatze at 14, 15 and 17 qubits, max |diff| < 2e-5 (FP32 descriptor values)."""
+    prog = os.path.join(ROOT, "tests", "c", "_build", "test_pass_form")
+    res = subprocess.run([prog], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout
+    assert res.stdout.strip().endswith("ok")
+
+
 def test_plan_validation_errors():
     lib = Q.lib()
     bad_qubit = Q.make_ops([("RY", 5, -1, 0, 0.1)])
