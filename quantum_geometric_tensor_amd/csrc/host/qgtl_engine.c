@@ -237,8 +237,19 @@ int qgtl_num_params(int nops, const qgtl_op_t *ops)
 }
 
 /* ------------------------------------------------------------------ plans */
+static void free_chain(struct qgtl_plan *p)
+{
+    for (int k = 0; k < p->nchain; k++) qgtl_hip_free(p->chain[k]);
+    qgtl_hip_free(p->chain_tmp);
+    free(p->chain);
+    p->chain = NULL;
+    p->chain_tmp = NULL;
+    p->nchain = 0;
+}
+
 static void plan_free_device(struct qgtl_plan *p)
 {
+    free_chain(p);
     qgtl_hip_free(p->psi[0]);
     qgtl_hip_free(p->psi[1]);
     qgtl_hip_free(p->D);
@@ -335,6 +346,8 @@ void qgtl_plan_destroy(qgtl_plan_t *p)
     free(p->inj_slots_host);
     qgtl_hip_event_destroy(p->ev_psi);
     qgtl_hip_event_destroy(p->ev_inj);
+    for (int k = 0; k < p->nev_chain; k++) qgtl_hip_event_destroy(p->ev_chain[k]);
+    free(p->ev_chain);
     free(p->tabs_host);
     free(p->ops);
     free(p->blocks);
@@ -427,6 +440,22 @@ static int gram_terms(const struct qgtl_plan *p)
 
 static int gram_products(const struct qgtl_plan *p) { return gram_terms(p) == QGTL_GRAM_F16X4 ? 4 : 3; }
 
+/* Block-boundary states of the primal chain (depth-first plans), within 2 GiB;
+ * otherwise (or if an allocation fails) the chain keeps two buffers. */
+static void alloc_chain(struct qgtl_plan *p)
+{
+    const size_t bytes = p->N * 8, nchain = (size_t)p->nblocks + 1;
+    if (p->chain || !any_flip(p) || (nchain + 1) * bytes > ((size_t)2 << 30)) return;
+    p->chain = (void **)calloc(nchain, sizeof(void *));
+    if (!p->chain) return;
+    int ok = qgtl_hip_malloc(&p->chain_tmp, bytes) == 0;
+    for (size_t k = 0; ok && k < nchain; k++) {
+        ok = qgtl_hip_malloc(&p->chain[k], bytes) == 0;
+        if (ok) p->nchain = (int)k + 1;
+    }
+    if (!ok) free_chain(p);
+}
+
 static int alloc_mixed(struct qgtl_plan *p)
 {
     if (p->psi[0]) return 0;
@@ -461,6 +490,7 @@ static int alloc_mixed(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
         HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
     }
+    alloc_chain(p);
     return 0;
 }
 
@@ -480,6 +510,7 @@ static int alloc_plain(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_malloc((void **)&p->Qdev, P * P * 2 * sizeof(double)));
         HIPCALL(qgtl_hip_host_alloc((void **)&p->Qhost, P * P * 2 * sizeof(double)));
     }
+    alloc_chain(p);
     return 0;
 }
 
@@ -781,8 +812,88 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
  * ev_psi, and the chain waits on ev_inj (the last injection issued) before a
  * pass overwrites a psi buffer an injection may still read.  With profiling on,
  * everything stays on one stream (per-launch timing). */
+/* Block b applied to one state from src into dst, every pass out of place
+ * (src is left as it is: the chunks read it), tmp the scratch between passes. */
+static int apply_block_to(struct qgtl_plan *p, const qgtl_block_t *b, const void *src, void *dst, void *tmp,
+                          qgtl_stats_t *st)
+{
+    const int np = b->npasses;
+    if (np == 0) LAUNCH(K_OTHER, qgtl_hip_copy_f32x4(dst, src, p->N / 2));
+    const void *in = src;
+    for (int i = 0; i < np; i++) {
+        void *out = ((np - 1 - i) % 2 == 0) ? dst : tmp;
+        LAUNCH(K_SWEEP, qgtl_hip_pass_c64(&b->pass[i], in, out, p->N, 1, 0, 0, pass_tabs(p, b, i, 0)));
+        if (st) {
+            st->sweep_launches++;
+            st->sweep_state_passes += 1;
+            st->sweep_bytes += (double)p->N * 16.0;
+        }
+        in = out;
+    }
+    if (b->mstage_kernels)
+        for (int i = 0; i < b->nperm; i++) {
+            if (b->perm_type[i] == 0)
+                LAUNCH(K_SWEEP, qgtl_hip_cnot_c64(dst, p->N, 1, p->n, b->perm_a[i], b->perm_b[i]));
+            else
+                LAUNCH(K_SWEEP, qgtl_hip_swap_c64(dst, p->N, 1, p->n, b->perm_a[i], b->perm_b[i]));
+            if (st) {
+                st->sweep_launches++;
+                st->sweep_state_passes += 1;
+                st->sweep_bytes += (double)p->N * 16.0;
+            }
+        }
+    return 0;
+}
+
+/* Depth-first execution with the block-boundary states kept: the whole primal
+ * chain is issued on the second stream (chain[k+1] = block k of chain[k], an
+ * event after each), then the chunks on the library stream, each after the
+ * event of the state it reads, so the chain never waits for the chunks. */
+static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
+{
+    int rc;
+    /* the chain starts after what the library stream holds (the phase-table upload) */
+    HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
+    HIPCALL(qgtl_hip_set_lane(1));
+    HIPCALL(qgtl_hip_lane_wait(p->ev_inj));
+    LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(p->chain[0], p->n));
+    HIPCALL(qgtl_hip_lane_signal(p->ev_chain[0]));
+    for (int bi = 0; bi < p->nblocks; bi++) {
+        rc = apply_block_to(p, &p->blocks[bi], p->chain[bi], p->chain[bi + 1], p->chain_tmp, st);
+        if (rc) return rc;
+        if (p->psi_in_gram && bi == p->meet) {
+            /* the primal state at the meeting point joins the Gram as state P */
+            void *slotP = (char *)p->D + (size_t)p->P * p->N * 8;
+            if (p->presplit) LAUNCH(K_OTHER, qgtl_hip_c64_to_split(p->chain[bi + 1], slotP, p->n));
+            else LAUNCH(K_OTHER, qgtl_hip_copy_f32x4(slotP, p->chain[bi + 1], p->N / 2));
+        }
+        HIPCALL(qgtl_hip_lane_signal(p->ev_chain[bi + 1]));
+    }
+    HIPCALL(qgtl_hip_set_lane(0));
+    int slot = 0;
+    for (int bi = 0; bi < p->nblocks; bi++) {
+        const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
+        if (bi > p->meet && K > 0) {                         /* carried back from the state before block bi */
+            HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi]));
+            rc = run_chunks(p, bi, slot, K, p->chain[bi], st, 1, 0);
+            if (rc) return rc;
+        }
+        if (K > 0 && bi <= p->meet) {                        /* carried forward from the state after block bi */
+            HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi + 1]));
+            rc = run_chunks(p, bi, slot, K, p->chain[bi + 1], st, 0, 0);
+            if (rc) return rc;
+        }
+        slot += K;
+    }
+    /* the library stream takes over psi_final and slot P */
+    HIPCALL(qgtl_hip_lane_wait(p->ev_chain[p->nblocks]));
+    p->psi_final = p->chain[p->nblocks];
+    return 0;
+}
+
 static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
 {
+    if (lanes && p->nchain == p->nblocks + 1 && p->nev_chain >= p->nchain) return run_mixed_chain(p, st);
     int rc;
     void *cur = p->psi[0], *alt = p->psi[1];
     const int depth_first = any_flip(p);
@@ -864,6 +975,15 @@ static int prepare_mixed(struct qgtl_plan *p)
     if (depth_first && !p->ev_psi) {
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_psi));
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_inj));
+    }
+    if (p->nchain && p->nev_chain < p->nchain) {
+        qgtl_hip_event_t *ev = (qgtl_hip_event_t *)realloc(p->ev_chain, (size_t)p->nchain * sizeof *ev);
+        if (!ev) return QGT_ERROR_NO_MEMORY;
+        p->ev_chain = ev;
+        while (p->nev_chain < p->nchain) {
+            HIPCALL(qgtl_hip_event_create_sync(&p->ev_chain[p->nev_chain]));
+            p->nev_chain++;
+        }
     }
     return 0;
 }

@@ -99,6 +99,15 @@ struct qgtl_plan {
     uint64_t structure_hash;
     void *psi_final;                     /* primal after the last execute (mixed) */
     qgtl_hip_event_t ev_psi, ev_inj;     /* ordering of the primal chain on the second stream */
+    /* depth-first plans whose block-boundary states fit a small budget: the
+     * primal state after block k (k = 0..nblocks) in chain[k], the chain's
+     * pass scratch, and one event per chain[k], so that the chain runs ahead
+     * on the second stream and each chunk waits only for the state it reads */
+    void **chain;
+    int nchain;                          /* nblocks + 1, or 0 (two-buffer chain) */
+    void *chain_tmp;
+    qgtl_hip_event_t *ev_chain;
+    int nev_chain;
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
     int psi_in_gram;                     /* D slot P holds the primal state at the meeting point; the Gram of
                                             P + 1 states gives the overlaps and norms (depth-first plans) */
