@@ -171,6 +171,8 @@ typedef struct {
     uint32_t unit_begin;    /* units: ntiles * nstates, tile-major within a state */
     uint32_t variant;       /* 0: LDS gate groups; 1: register phases (m = 13, nphase >= 1, no PERM store);
                                2: register phases storing the split-plane form (a final pass) */
+    uint64_t tabs;          /* the descriptor's phase tables (qgtl_hip_phase_tables_grouped), or 0:
+                               built by every workgroup */
 } qgtl_pass_item_t;
 
 typedef struct {
@@ -228,6 +230,10 @@ typedef struct {
 int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t units, int grid);
 /* variant: every item of the launch has this qgtl_pass_item_t.variant */
 int qgtl_hip_pass_grouped(const qgtl_pass_item_t *items, int nitems, uint32_t units, int grid, int variant);
+/* The phase tables of ndesc pass descriptors (rotation form; others are left
+ * alone): descriptor i's post then pre tables, QGTL_PHASE_TAB complex64 each,
+ * at tabs + 2 * QGTL_PHASE_TAB * i (one workgroup per descriptor). */
+int qgtl_hip_phase_tables_grouped(const qgtl_pass_desc_t *descs, int ndesc, void *tabs);
 int qgtl_hip_inject_grouped(const qgtl_inject_item_t *items, int nitems, uint32_t units, int grid);
 int qgtl_hip_reduce_grouped(const qgtl_reduce_item_t *items, int nitems, uint32_t units, int grid);
 /* grid: multiple of 8 (XCD affinity of a chunk's tiles) */

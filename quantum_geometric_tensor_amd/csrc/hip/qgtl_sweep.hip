@@ -1227,9 +1227,28 @@ __global__ __launch_bounds__(kThreads, 2) void pass_grouped_kernel(const qgtl_pa
         const qgtl_pass_desc_t &sd = *qgtl::const_ptr<qgtl_pass_desc_t>(qgtl::uniform(it.desc));
         const float2 *src = qgtl::global_ptr<const float2>(qgtl::uniform(it.src)) + (size_t)s * it.stride;
         float2 *dst = qgtl::global_ptr<float2>(qgtl::uniform(it.dst)) + (size_t)s * it.stride;
-        if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
-        else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab);
-        else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab);
+        const uint64_t ta = qgtl::uniform(it.tabs);
+        const float2 *gt = ta ? qgtl::global_ptr<const float2>(ta) : nullptr;
+        if (VARIANT == 2) pass_regs_body<true, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab, gt);
+        else if (VARIANT == 1) pass_regs_body<false, true>(sd, src, dst, r, tile, tabs, tabs + kPhaseTab, gt);
+        else pass_lds_body(sd, src, dst, r, tile, hi_off, tabs, tabs + kPhaseTab, gt);
+    }
+}
+
+// Phase tables of descriptor blockIdx.x (grid-stride over descriptors), the
+// same entries build_phase_tables puts in LDS, written to global memory.
+__global__ __launch_bounds__(kThreads) void phase_tables_grouped_kernel(const qgtl_pass_desc_t *__restrict__ descs,
+                                                                       int ndesc, float2 *tabs) {
+    __shared__ float2 post_tab[kPhaseTab], pre_tab[kPhaseTab];
+    for (int i = blockIdx.x; i < ndesc; i += gridDim.x) {
+        const qgtl_pass_desc_t &d = *qgtl::const_ptr<qgtl_pass_desc_t>((uint64_t)(uintptr_t)(descs + i));
+        if (d.form != 1) continue;                           // uniform: the whole workgroup skips
+        __syncthreads();                                     // the previous descriptor's copy-out is done
+        build_phase_tables(d, post_tab, pre_tab);
+        __syncthreads();
+        float2 *out = tabs + (size_t)i * 2 * kPhaseTab;
+        for (int t = threadIdx.x; t < 2 * kPhaseTab; t += kThreads)
+            out[t] = t < kPhaseTab ? post_tab[t] : pre_tab[t - kPhaseTab];
     }
 }
 
@@ -1305,6 +1324,16 @@ int qgtl_hip_fill_grouped(const qgtl_fill_item_t *items, int nitems, uint32_t un
     hipLaunchKernelGGL(fill_grouped_kernel, dim3(grouped_grid(units, grid)), dim3(kThreads), 0, qgtl::stream(),
                        items, nitems, units);
     QGTL_LAUNCH_CHECK("fill_grouped");
+    return 0;
+}
+
+int qgtl_hip_phase_tables_grouped(const qgtl_pass_desc_t *descs, int ndesc, void *tabs) {
+    if (!descs || !tabs || ndesc < 0) return qgtl::set_error_msg(-118, "qgtl_hip_phase_tables_grouped: bad arguments");
+    if (ndesc == 0) return 0;
+    const int grid = ndesc < 4096 ? ndesc : 4096;
+    hipLaunchKernelGGL(phase_tables_grouped_kernel, dim3(grid), dim3(kThreads), 0, qgtl::stream(), descs, ndesc,
+                       (float2 *)tabs);
+    QGTL_LAUNCH_CHECK("phase_tables_grouped");
     return 0;
 }
 

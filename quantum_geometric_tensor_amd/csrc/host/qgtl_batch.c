@@ -93,6 +93,7 @@ struct qgtl_batch {
     int cus;
     /* descriptors (host mirror, device copy) */
     qgtl_pass_desc_t *pass_h, *pass_d;
+    void *tabs_d;                        /* phase tables of every pass descriptor, built on the device per execution */
     size_t npass;
     qgtl_inject_desc_t *inj_h, *inj_d;
     size_t ninj;
@@ -326,6 +327,7 @@ static int add_pass_item(struct qgtl_batch *B, qgtl_pass_item_t *it, int *k, uin
     it[*k].nstates = (uint32_t)nstates;
     it[*k].unit_begin = (uint32_t)*u;
     it[*k].variant = (uint32_t)regs_variant(d, B->tune.pass_variant);
+    it[*k].tabs = dev_addr(B->tabs_d, desc * 2 * QGTL_PHASE_TAB * sizeof(qgtl_c64_t));
     *u += (uint64_t)it[*k].ntiles * (uint64_t)nstates;
     B->sweep_bytes += (double)nstates * (double)in->N * 16.0;
     *arena ^= flip;
@@ -619,6 +621,8 @@ static int grid_for(const struct qgtl_batch *B, const launch_t *L)
 
 static int issue(struct qgtl_batch *B)
 {
+    /* the descriptors' phase tables, once per execution, for every pass launch after it */
+    BTRY(qgtl_hip_phase_tables_grouped(B->pass_d, (int)B->npass, B->tabs_d));
     for (int l = 0; l < B->nlaunch; l++) {
         const launch_t *L = &B->launches[l];
         const void *items = B->items_d + L->item_off;
@@ -702,6 +706,7 @@ void qgtl_batch_destroy(qgtl_batch_t *B)
     qgtl_hip_free(B->C);
     qgtl_hip_free(B->Qdev);
     qgtl_hip_free(B->pass_d);
+    qgtl_hip_free(B->tabs_d);
     qgtl_hip_free(B->inj_d);
     qgtl_hip_free(B->items_d);
     free(B->inst);
@@ -810,9 +815,10 @@ qgtl_batch_t *qgtl_batch_create(int count, const int *n, const int *nops, const 
     CTRY(qgtl_hip_malloc((void **)&B->C, co * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->Qdev, qo * 2 * sizeof(double)));
     CTRY(qgtl_hip_malloc((void **)&B->pass_d, (B->npass ? B->npass : 1) * sizeof(qgtl_pass_desc_t)));
+    CTRY(qgtl_hip_malloc(&B->tabs_d, (B->npass ? B->npass : 1) * 2 * QGTL_PHASE_TAB * sizeof(qgtl_c64_t)));
     CTRY(qgtl_hip_malloc((void **)&B->inj_d, (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t)));
     B->device_bytes = xo * 8 + (co + qo * 2) * sizeof(double) +
-                      (B->npass ? B->npass : 1) * sizeof(qgtl_pass_desc_t) +
+                      (B->npass ? B->npass : 1) * (sizeof(qgtl_pass_desc_t) + 2 * QGTL_PHASE_TAB * sizeof(qgtl_c64_t)) +
                       (B->ninj ? B->ninj : 1) * sizeof(qgtl_inject_desc_t);
     fill_descs(B);
     if (rebuild(B)) return fail_create(B);
