@@ -353,7 +353,12 @@ __global__ __launch_bounds__(kThreads, 1) void gram_cplx_kernel(const uint8_t *_
 // G_ab = conj(G_ba) and is stored transposed with Im negated.  The assignment
 // orients the 28 block pairs a != b so that blocks 0..3 serve as B for 2 of
 // them and blocks 4..7 for 5 (with its diagonal pair: 3 and 6 = one and two
-// waves), so every wave has three pairs and every SIMD nine.  The 96
+// waves), so every wave has three pairs and every SIMD nine.  A diagonal pair
+// (always the wave's first, a0 == b) skips its 16 x 16 sub-block below the
+// diagonal, which no consumer reads; waves w and w + 4 k share a SIMD (the
+// round-robin placement of a workgroup's waves), and the eight waves with a
+// diagonal pair are spread two per SIMD, so each SIMD issues 34 sub-block
+// products per K-step instead of 36.  The 96
 // accumulator registers of three pairs leave no room for the outer fp32 sums
 // at three waves per SIMD: the MFMA accumulators hold the whole chunk.  Their
 // rounding grows with the chunk length (each MFMA adds its products into an
@@ -377,10 +382,10 @@ __device__ __forceinline__ Role3 wave_role3(int w) {
     case 5: return Role3{5, 6, 7, 4};
     case 6: return Role3{5, 0, 1, 5};
     case 7: return Role3{2, 3, 6, 5};
-    case 8: return Role3{6, 0, 1, 6};
-    case 9: return Role3{2, 3, 7, 6};
-    case 10: return Role3{7, 0, 1, 7};
-    default: return Role3{2, 3, 5, 7};
+    case 8: return Role3{2, 3, 7, 6};
+    case 9: return Role3{6, 0, 1, 6};
+    case 10: return Role3{2, 3, 5, 7};
+    default: return Role3{7, 0, 1, 7};
     }
 }
 
@@ -428,8 +433,9 @@ __device__ __forceinline__ u32x4 frag16(const uint8_t *buf, int blk, int sub, in
     return *reinterpret_cast<const u32x4 *>(buf + row * Geo<32>::kRowBytes + slot * 16);
 }
 
-// acc[i][2 sa + sb]: pair i, sub-block (sa, sb); Re and Im
-template <int PT>
+// acc[i][2 sa + sb]: pair i, sub-block (sa, sb); Re and Im.  DIAG: pair 0 is a
+// diagonal pair and its sub-block (1, 0), below the diagonal, is not computed.
+template <int PT, bool DIAG>
 __device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &ro, f32x4 (&re)[3][4],
                                                f32x4 (&im)[3][4], uint32_t lane) {
     const int ab[3] = {ro.a0, ro.a1, ro.a2};
@@ -460,6 +466,7 @@ __device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &
                 for (int i = 0; i < 3; i++)
 #pragma unroll
                     for (int sb = 0; sb < 2; sb++) {
+                        if (DIAG && i == 0 && sa == 1 && sb == 0) continue;
                         re[i][2 * sa + sb] = mma16(al[i], bl[sb], re[i][2 * sa + sb]);
                         im[i][2 * sa + sb] = mma16(al[i], yl[sb], im[i][2 * sa + sb]);
                     }
@@ -468,6 +475,7 @@ __device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int sb = 0; sb < 2; sb++) {
+                    if (DIAG && i == 0 && sa == 1 && sb == 0) continue;
                     re[i][2 * sa + sb] = mma16(al[i], bh[sb], re[i][2 * sa + sb]);
                     im[i][2 * sa + sb] = mma16(al[i], yh[sb], im[i][2 * sa + sb]);
                 }
@@ -475,6 +483,7 @@ __device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int sb = 0; sb < 2; sb++) {
+                    if (DIAG && i == 0 && sa == 1 && sb == 0) continue;
                     re[i][2 * sa + sb] = mma16(ah[i], bl[sb], re[i][2 * sa + sb]);
                     im[i][2 * sa + sb] = mma16(ah[i], yl[sb], im[i][2 * sa + sb]);
                 }
@@ -482,6 +491,7 @@ __device__ __forceinline__ void stage_mfma3_16(const uint8_t *buf, const Role3 &
             for (int i = 0; i < 3; i++)
 #pragma unroll
                 for (int sb = 0; sb < 2; sb++) {
+                    if (DIAG && i == 0 && sa == 1 && sb == 0) continue;
                     re[i][2 * sa + sb] = mma16(ah[i], bh[sb], re[i][2 * sa + sb]);
                     im[i][2 * sa + sb] = mma16(ah[i], yh[sb], im[i][2 * sa + sb]);
                 }
@@ -514,15 +524,13 @@ __device__ __forceinline__ void store_pair256_16(float *__restrict__ out, int a,
 
 // One split-K chunk of a two-tile Gram: stages of 32 amplitudes through the two
 // LDS buffers, the chunk's fp32 partials of the three tile pairs at the end.
-template <int PT>
-__device__ __forceinline__ void gram_cplx256_unit(const uint8_t *__restrict__ D, int P, size_t N, size_t chunk_len,
-                                                  int chunk, float *__restrict__ partials, uint8_t *lds) {
+template <int PT, bool DIAG>
+__device__ __forceinline__ void gram_cplx256_chunk(const uint8_t *__restrict__ D, int P, size_t N, size_t chunk_len,
+                                                   int chunk, float *__restrict__ partials, uint8_t *lds,
+                                                   const Role3 &ro, int w, uint32_t lane) {
     typedef Geo<32> G;
     constexpr int kBuf = 2 * G::kTileBytes;                  // 256 rows of one stage: 64 KB
     static_assert(2 * kBuf <= kLdsBytes, "two stage buffers");
-    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const uint32_t lane = threadIdx.x & 63;
-    const Role3 ro = wave_role3(w);
     const size_t SB = N * 8;
     const size_t kbeg = (size_t)chunk * chunk_len;
     const size_t kend = (kbeg + chunk_len < N) ? kbeg + chunk_len : N;
@@ -547,12 +555,24 @@ __device__ __forceinline__ void gram_cplx256_unit(const uint8_t *__restrict__ D,
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         if (st + 1 < nst) issue_stage256(D, SB, P, kbeg + (size_t)(st + 1) * 32, lds + ((st + 1) & 1) * kBuf, w, lane);
-        stage_mfma3_16<PT>(lds + (st & 1) * kBuf, ro, re4, im4, lane);
+        stage_mfma3_16<PT, DIAG>(lds + (st & 1) * kBuf, ro, re4, im4, lane);
     }
     float *out = partials + (size_t)chunk * 3 * 2 * kTileElems;
     store_pair256_16(out, ro.a0, ro.b, re4[0], im4[0], lane);
     store_pair256_16(out, ro.a1, ro.b, re4[1], im4[1], lane);
     store_pair256_16(out, ro.a2, ro.b, re4[2], im4[2], lane);
+}
+
+// The wave's role decides the form once per chunk (a diagonal pair's waves
+// skip one sub-block), so each form keeps its own accumulator allocation.
+template <int PT>
+__device__ __forceinline__ void gram_cplx256_unit(const uint8_t *__restrict__ D, int P, size_t N, size_t chunk_len,
+                                                  int chunk, float *__restrict__ partials, uint8_t *lds) {
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63;
+    const Role3 ro = wave_role3(w);
+    if (ro.a0 == ro.b) gram_cplx256_chunk<PT, true>(D, P, N, chunk_len, chunk, partials, lds, ro, w, lane);
+    else gram_cplx256_chunk<PT, false>(D, P, N, chunk_len, chunk, partials, lds, ro, w, lane);
 }
 
 template <int PT>
