@@ -241,9 +241,11 @@ static void free_chain(struct qgtl_plan *p)
 {
     for (int k = 0; k < p->nchain; k++) qgtl_hip_free(p->chain[k]);
     qgtl_hip_free(p->chain_tmp);
+    qgtl_hip_free(p->tmp_side);
     free(p->chain);
     p->chain = NULL;
     p->chain_tmp = NULL;
+    p->tmp_side = NULL;
     p->nchain = 0;
 }
 
@@ -441,7 +443,10 @@ static int gram_terms(const struct qgtl_plan *p)
 static int gram_products(const struct qgtl_plan *p) { return gram_terms(p) == QGTL_GRAM_F16X4 ? 4 : 3; }
 
 /* Block-boundary states of the primal chain (depth-first plans), within 2 GiB;
- * otherwise (or if an allocation fails) the chain keeps two buffers. */
+ * otherwise (or if an allocation fails) the chain keeps two buffers.  With the
+ * chain, a second chunk buffer (tmp_side, at most 1 GiB) lets the chain's
+ * stream carry the backward chunks while the library stream carries the
+ * forward ones; without it every chunk runs on the library stream. */
 static void alloc_chain(struct qgtl_plan *p)
 {
     const size_t bytes = p->N * 8, nchain = (size_t)p->nblocks + 1;
@@ -453,7 +458,13 @@ static void alloc_chain(struct qgtl_plan *p)
         ok = qgtl_hip_malloc(&p->chain[k], bytes) == 0;
         if (ok) p->nchain = (int)k + 1;
     }
-    if (!ok) free_chain(p);
+    if (!ok) {
+        free_chain(p);
+        return;
+    }
+    if (p->tmp_states > 0 && p->tmp_states * bytes <= ((size_t)1 << 30) &&
+        qgtl_hip_malloc(&p->tmp_side, p->tmp_states * bytes) != 0)
+        p->tmp_side = NULL;
 }
 
 static int alloc_mixed(struct qgtl_plan *p)
@@ -756,7 +767,7 @@ static int inject(struct qgtl_plan *p, int s0, int cnt, const void *psi, void *d
  * (backward) through the inverses of blocks bi-1 .. meet+1, so that it ends
  * in its D slot, in split-plane form when the plan says so. */
 static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *psi, qgtl_stats_t *st, int backward,
-                      int lanes)
+                      int lanes, void *tmp)
 {
     int flips_after = 0;
     if (!backward)
@@ -776,8 +787,8 @@ static int run_chunks(struct qgtl_plan *p, int bi, int slot, int K, const void *
     for (int j0 = 0; j0 < K; j0 += C) {
         const int cnt = (K - j0 < C) ? K - j0 : C;
         void *dslot = (char *)p->D + (size_t)(slot + j0) * p->N * 8;
-        void *c_cur = (flips_after % 2 == 0) ? dslot : p->tmp;
-        void *c_alt = (c_cur == dslot) ? p->tmp : dslot;
+        void *c_cur = (flips_after % 2 == 0) ? dslot : tmp;
+        void *c_alt = (c_cur == dslot) ? tmp : dslot;
         int rc = inject(p, slot + j0, cnt, psi, c_cur, st, p->presplit && direct, backward, !fuse);
         if (rc) return rc;
         const void *ipsi = fuse ? psi : NULL;
@@ -869,22 +880,41 @@ static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
         }
         HIPCALL(qgtl_hip_lane_signal(p->ev_chain[bi + 1]));
     }
-    HIPCALL(qgtl_hip_set_lane(0));
+    /* with a second chunk buffer and a spare event, the chain's stream goes on
+     * with the chunks carried backward from beyond the meeting point (their
+     * primal states are the chain's, issued before them on the same stream);
+     * the library stream takes the forward chunks and the meeting point's
+     * injections, so the two halves of the chunk work overlap */
+    const int side = p->tmp_side && p->nev_chain > p->nchain;
     int slot = 0;
+    if (side) {
+        for (int bi = 0; bi < p->nblocks; bi++) {
+            const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
+            if (bi > p->meet + 1 && K > 0) {
+                rc = run_chunks(p, bi, slot, K, p->chain[bi], st, 1, 0, p->tmp_side);
+                if (rc) return rc;
+            }
+            slot += K;
+        }
+        HIPCALL(qgtl_hip_lane_signal(p->ev_chain[p->nchain]));
+    }
+    HIPCALL(qgtl_hip_set_lane(0));
+    slot = 0;
     for (int bi = 0; bi < p->nblocks; bi++) {
         const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
-        if (bi > p->meet && K > 0) {                         /* carried back from the state before block bi */
+        if (bi > p->meet && K > 0 && !(side && bi > p->meet + 1)) {   /* carried back from the state before block bi */
             HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi]));
-            rc = run_chunks(p, bi, slot, K, p->chain[bi], st, 1, 0);
+            rc = run_chunks(p, bi, slot, K, p->chain[bi], st, 1, 0, p->tmp);
             if (rc) return rc;
         }
         if (K > 0 && bi <= p->meet) {                        /* carried forward from the state after block bi */
             HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi + 1]));
-            rc = run_chunks(p, bi, slot, K, p->chain[bi + 1], st, 0, 0);
+            rc = run_chunks(p, bi, slot, K, p->chain[bi + 1], st, 0, 0, p->tmp);
             if (rc) return rc;
         }
         slot += K;
     }
+    if (side) HIPCALL(qgtl_hip_lane_wait(p->ev_chain[p->nchain]));
     /* the library stream takes over psi_final and slot P */
     HIPCALL(qgtl_hip_lane_wait(p->ev_chain[p->nblocks]));
     p->psi_final = p->chain[p->nblocks];
@@ -913,7 +943,7 @@ static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
                 HIPCALL(qgtl_hip_set_lane(0));
                 HIPCALL(qgtl_hip_lane_wait(p->ev_psi));
             }
-            rc = run_chunks(p, bi, slot, K, cur, st, 1, lanes);
+            rc = run_chunks(p, bi, slot, K, cur, st, 1, lanes, p->tmp);
             if (rc) return rc;
         }
         if (!depth_first && slot > 0) {
@@ -945,7 +975,7 @@ static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
                 HIPCALL(qgtl_hip_set_lane(0));
                 HIPCALL(qgtl_hip_lane_wait(p->ev_psi));
             }
-            rc = run_chunks(p, bi, slot, K, cur, st, 0, lanes);
+            rc = run_chunks(p, bi, slot, K, cur, st, 0, lanes, p->tmp);
             if (rc) return rc;
         }
         slot += K;
@@ -976,11 +1006,12 @@ static int prepare_mixed(struct qgtl_plan *p)
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_psi));
         HIPCALL(qgtl_hip_event_create_sync(&p->ev_inj));
     }
-    if (p->nchain && p->nev_chain < p->nchain) {
-        qgtl_hip_event_t *ev = (qgtl_hip_event_t *)realloc(p->ev_chain, (size_t)p->nchain * sizeof *ev);
+    /* one event per block boundary of the chain, one for the end of the chunks on its stream */
+    if (p->nchain && p->nev_chain < p->nchain + 1) {
+        qgtl_hip_event_t *ev = (qgtl_hip_event_t *)realloc(p->ev_chain, (size_t)(p->nchain + 1) * sizeof *ev);
         if (!ev) return QGT_ERROR_NO_MEMORY;
         p->ev_chain = ev;
-        while (p->nev_chain < p->nchain) {
+        while (p->nev_chain < p->nchain + 1) {
             HIPCALL(qgtl_hip_event_create_sync(&p->ev_chain[p->nev_chain]));
             p->nev_chain++;
         }
