@@ -860,16 +860,63 @@ static int apply_block_to(struct qgtl_plan *p, const qgtl_block_t *b, const void
  * chain is issued on the second stream (chain[k+1] = block k of chain[k], an
  * event after each), then the chunks on the library stream, each after the
  * event of the state it reads, so the chain never waits for the chunks. */
-static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
+/* A chunk the library stream carries in run_mixed_chain: block bi's parameters
+ * (slots from slot), forward from chain[bi + 1] or backward from chain[bi],
+ * after the event that publishes that state. */
+static int chain_lane0_chunk(struct qgtl_plan *p, int bi, int slot, qgtl_stats_t *st)
+{
+    const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
+    if (K <= 0) return 0;
+    const int backward = bi > p->meet;
+    const int src = backward ? bi : bi + 1;
+    HIPCALL(qgtl_hip_set_lane(0));
+    HIPCALL(qgtl_hip_lane_wait(p->ev_chain[src]));
+    return run_chunks(p, bi, slot, K, p->chain[src], st, backward, 0, p->tmp);
+}
+
+/* in_order: issue the library stream's chunks as soon as the chain state they
+ * read is issued (eager launches: each stream starts its work as early as it
+ * can); otherwise the chain's stream first, then the library stream's (the
+ * order a captured graph replays with the two streams' work side by side:
+ * captured in the first order, the graph ran most of both on one queue). */
+/* The library stream's chunks that read chain[bi + 1]: block bi's carried
+ * forward, block bi + 1's carried back (unless the chain's stream takes it). */
+static int chain_lane0_after(struct qgtl_plan *p, int bi, int slot, int K, int side, qgtl_stats_t *st)
 {
     int rc;
+    if (bi <= p->meet) {
+        rc = chain_lane0_chunk(p, bi, slot, st);
+        if (rc) return rc;
+    }
+    const int nb = bi + 1;
+    if (nb < p->nblocks && nb > p->meet && !(side && nb > p->meet + 1)) {
+        rc = chain_lane0_chunk(p, nb, slot + K, st);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st, int in_order)
+{
+    int rc;
+    /* with a second chunk buffer and a spare event, the chain's stream goes on
+     * with the chunks carried backward from beyond the meeting point (their
+     * primal states are the chain's, issued before them on the same stream);
+     * the library stream takes the forward chunks and the meeting point's
+     * injections, so the two halves of the chunk work overlap */
+    const int side = p->tmp_side && p->nev_chain > p->nchain;
     /* the chain starts after what the library stream holds (the phase-table upload) */
     HIPCALL(qgtl_hip_lane_signal(p->ev_inj));
     HIPCALL(qgtl_hip_set_lane(1));
     HIPCALL(qgtl_hip_lane_wait(p->ev_inj));
     LAUNCH(K_OTHER, qgtl_hip_fill_zero_state_c64(p->chain[0], p->n));
     HIPCALL(qgtl_hip_lane_signal(p->ev_chain[0]));
+    /* in_order: after block bi of the chain, the library stream's chunks that
+     * read chain[bi + 1] */
+    int slot = 0;
     for (int bi = 0; bi < p->nblocks; bi++) {
+        const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
+        HIPCALL(qgtl_hip_set_lane(1));
         rc = apply_block_to(p, &p->blocks[bi], p->chain[bi], p->chain[bi + 1], p->chain_tmp, st);
         if (rc) return rc;
         if (p->psi_in_gram && bi == p->meet) {
@@ -879,15 +926,15 @@ static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
             else LAUNCH(K_OTHER, qgtl_hip_copy_f32x4(slotP, p->chain[bi + 1], p->N / 2));
         }
         HIPCALL(qgtl_hip_lane_signal(p->ev_chain[bi + 1]));
+        if (in_order) {
+            rc = chain_lane0_after(p, bi, slot, K, side, st);
+            if (rc) return rc;
+        }
+        slot += K;
     }
-    /* with a second chunk buffer and a spare event, the chain's stream goes on
-     * with the chunks carried backward from beyond the meeting point (their
-     * primal states are the chain's, issued before them on the same stream);
-     * the library stream takes the forward chunks and the meeting point's
-     * injections, so the two halves of the chunk work overlap */
-    const int side = p->tmp_side && p->nev_chain > p->nchain;
-    int slot = 0;
     if (side) {
+        HIPCALL(qgtl_hip_set_lane(1));
+        slot = 0;
         for (int bi = 0; bi < p->nblocks; bi++) {
             const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
             if (bi > p->meet + 1 && K > 0) {
@@ -898,22 +945,16 @@ static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
         }
         HIPCALL(qgtl_hip_lane_signal(p->ev_chain[p->nchain]));
     }
-    HIPCALL(qgtl_hip_set_lane(0));
-    slot = 0;
-    for (int bi = 0; bi < p->nblocks; bi++) {
-        const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
-        if (bi > p->meet && K > 0 && !(side && bi > p->meet + 1)) {   /* carried back from the state before block bi */
-            HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi]));
-            rc = run_chunks(p, bi, slot, K, p->chain[bi], st, 1, 0, p->tmp);
+    if (!in_order) {
+        slot = 0;
+        for (int bi = 0; bi < p->nblocks; bi++) {
+            const int K = p->blocks[bi].param_end - p->blocks[bi].param_begin;
+            rc = chain_lane0_after(p, bi, slot, K, side, st);
             if (rc) return rc;
+            slot += K;
         }
-        if (K > 0 && bi <= p->meet) {                        /* carried forward from the state after block bi */
-            HIPCALL(qgtl_hip_lane_wait(p->ev_chain[bi + 1]));
-            rc = run_chunks(p, bi, slot, K, p->chain[bi + 1], st, 0, 0, p->tmp);
-            if (rc) return rc;
-        }
-        slot += K;
     }
+    HIPCALL(qgtl_hip_set_lane(0));
     if (side) HIPCALL(qgtl_hip_lane_wait(p->ev_chain[p->nchain]));
     /* the library stream takes over psi_final and slot P */
     HIPCALL(qgtl_hip_lane_wait(p->ev_chain[p->nblocks]));
@@ -923,7 +964,7 @@ static int run_mixed_chain(struct qgtl_plan *p, qgtl_stats_t *st)
 
 static int run_mixed_lanes(struct qgtl_plan *p, qgtl_stats_t *st, int lanes)
 {
-    if (lanes && p->nchain == p->nblocks + 1 && p->nev_chain >= p->nchain) return run_mixed_chain(p, st);
+    if (lanes && p->nchain == p->nblocks + 1 && p->nev_chain >= p->nchain) return run_mixed_chain(p, st, !p->capturing);
     int rc;
     void *cur = p->psi[0], *alt = p->psi[1];
     const int depth_first = any_flip(p);
@@ -1156,7 +1197,9 @@ int qgtl_plan_execute(qgtl_plan_t *p, double *Q, qgtl_stats_t *stats)
         } else if (graphable && p->exec_gen == p->build_gen) {
             /* second execute of this build: capture the launches, then replay them */
             HIPCALL(qgtl_hip_capture_begin());
+            p->capturing = 1;
             rc = run_mixed(p, st);
+            p->capturing = 0;
             if (rc) { qgtl_hip_capture_abort(); qgtl_hip_set_lane(0); return rc; }
             rc = qgtl_hip_capture_end_update(&p->graph);
             if (rc) return hip_fail(rc, "graph capture");

@@ -107,6 +107,7 @@ struct qgtl_plan {
     int nchain;                          /* nblocks + 1, or 0 (two-buffer chain) */
     void *chain_tmp;
     void *tmp_side;                      /* chunk buffer of the chain's stream (backward chunks), or NULL */
+    int capturing;                       /* the launches being issued are captured into the plan's graph */
     qgtl_hip_event_t *ev_chain;
     int nev_chain;
     int presplit;                        /* D holds split-plane states (the Gram's fp16 operand pieces) */
