@@ -527,6 +527,29 @@ __device__ __forceinline__ void apply_phase_rots(float2 (&v)[kRegs], const Phase
     apply_reg_scaled<4>(v, q.tan[4]);
 }
 
+// The same with the phase's gate mask known at compile time: the register
+// bits without a gate get no instructions.
+template <uint32_t MASK>
+__device__ __forceinline__ void apply_phase_rots_m(float2 (&v)[kRegs], const PhaseParams &q) {
+    if constexpr ((MASK & 1u) != 0) apply_reg_scaled<0>(v, q.tan[0]);
+    if constexpr ((MASK & 2u) != 0) apply_reg_scaled<1>(v, q.tan[1]);
+    if constexpr ((MASK & 4u) != 0) apply_reg_scaled<2>(v, q.tan[2]);
+    if constexpr ((MASK & 8u) != 0) apply_reg_scaled<3>(v, q.tan[3]);
+    if constexpr ((MASK & 16u) != 0) apply_reg_scaled<4>(v, q.tan[4]);
+}
+
+// Phase shapes of rotation-form register passes with their gate masks fixed
+// at compile time: SHAPE = 0 is the general form (run-time phase count, every
+// register bit rotated, t = 0 where there is no gate); otherwise bits 0..1
+// hold the phase count and bits 2 + 5 p .. 6 + 5 p the mask of phase p.
+constexpr uint32_t shape_code(int nph, uint32_t m0, uint32_t m1, uint32_t m2) {
+    return (uint32_t)nph | (m0 << 2) | (m1 << 7) | (m2 << 12);
+}
+constexpr uint32_t kShape3Full = shape_code(3, 0x1f, 0x1f, 0x07);   // 13 gates over three phases
+constexpr uint32_t kShape2Hi7 = shape_code(2, 0x1e, 0x07, 0);       // 7 gates, register bit 0 without one
+constexpr int shape_phases(uint32_t shape) { return (int)(shape & 3u); }
+constexpr uint32_t shape_mask(uint32_t shape, int ph) { return (shape >> (2 + 5 * ph)) & 31u; }
+
 template <int J>
 __device__ __forceinline__ void apply_phase_mat(float2 (&v)[kRegs], const qgtl_pass_desc_t &d, int ph, int mask) {
     if ((mask >> J) & 1) {
@@ -799,6 +822,27 @@ __device__ __forceinline__ TileMaps tile_maps(const qgtl_pass_desc_t &d, uint32_
 // form of the result (a final pass).
 // The phases of a register pass: phase p > 0 first moves the amplitudes into
 // its layout through the LDS tile; ROT: rotation form, else general matrices.
+// A rotation-form pass of compile-time shape SHAPE: phase PH (and on).
+template <uint32_t SHAPE, int PH>
+__device__ __forceinline__ void run_phases_shaped(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], RegLayout &Lc,
+                                                  float2 *tile, const PhaseParams &cur) {
+    if constexpr (PH < shape_phases(SHAPE)) {
+        const uint32_t t = threadIdx.x;
+        PhaseParams nxt = cur;
+        if constexpr (PH + 1 < shape_phases(SHAPE)) nxt = load_phase(d, PH + 1);
+        if constexpr (PH > 0) {
+            const RegLayout Ln = make_layout(cur.rbits, cur.tbits, t);
+            RegLoop<kRegs>::store_lds(tile, Lc, v);
+            __syncthreads();
+            RegLoop<kRegs>::load_lds(tile, Ln, v);
+            __syncthreads();
+            Lc = Ln;
+        }
+        apply_phase_rots_m<shape_mask(SHAPE, PH)>(v, cur);
+        run_phases_shaped<SHAPE, PH + 1>(d, v, Lc, tile, nxt);
+    }
+}
+
 template <bool ROT>
 __device__ __forceinline__ void run_phases(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], RegLayout &Lc,
                                            float2 *tile) {
@@ -829,7 +873,7 @@ __device__ __forceinline__ void run_phases(const qgtl_pass_desc_t &d, float2 (&v
     }
 }
 
-template <bool SPLIT, bool NT = false>
+template <bool SPLIT, bool NT = false, uint32_t SHAPE = 0>
 __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, float2 (&v)[kRegs], const TileMaps &m,
                                                    const RegLayout &L0, float2 tfac, float2 *dst, float2 *tile,
                                                    const float2 *post_tab, const float2 *pre_tab) {
@@ -857,7 +901,8 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
 
     // one phase loop per form, so the rotation form's straight-line gates share
     // no joins with the general matrices' branches
-    if (rotf) run_phases<true>(d, v, Lc, tile);
+    if constexpr (SHAPE != 0) run_phases_shaped<SHAPE, 0>(d, v, Lc, tile, load_phase(d, 0));
+    else if (rotf) run_phases<true>(d, v, Lc, tile);
     else run_phases<false>(d, v, Lc, tile);
     if (post_on) phase_layer<true>(post_tab, Lc, d, d.scale, tfac, v);
 
@@ -885,7 +930,7 @@ __device__ __forceinline__ void pass_compute_store(const qgtl_pass_desc_t &d, fl
 // One tile r of one state.  inj != nullptr: src is the primal state the
 // derivative state *inj is injected from while loading.  The global loads go
 // out first; the phase tables are set up while they are in flight.
-template <bool SPLIT = false, bool NT = false>
+template <bool SPLIT = false, bool NT = false, uint32_t SHAPE = 0>
 __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const float2 *__restrict__ src,
                                                float2 *dst, uint32_t r, float2 *tile, float2 *post_tab,
                                                float2 *pre_tab, const float2 *gtabs = nullptr,
@@ -899,17 +944,17 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     else RegLoop<kRegs>::template load_global<NT>(src, m.gin, v);
     const float2 tfac = tile_factor(d, r);
     pass_setup(d, post_tab, pre_tab, gtabs, tq);
-    pass_compute_store<SPLIT, NT>(d, v, m, L0, tfac, dst, tile, post_tab, pre_tab);
+    pass_compute_store<SPLIT, NT, SHAPE>(d, v, m, L0, tfac, dst, tile, post_tab, pre_tab);
 }
 
-template <bool SPLIT, bool NT = false>
+template <bool SPLIT, bool NT = false, uint32_t SHAPE = 0>
 __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass_desc_t d,
                                                                  const float2 *__restrict__ src_all,
                                                                  float2 *dst_all, size_t stride,
                                                                  const float2 *__restrict__ gtabs) {
     __shared__ float2 tile[kRegTileElems];
     __shared__ float2 tabs[2 * kPhaseTab];
-    pass_regs_body<SPLIT, NT>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
+    pass_regs_body<SPLIT, NT, SHAPE>(d, src_all + (size_t)blockIdx.y * stride, dst_all + (size_t)blockIdx.y * stride,
                               blockIdx.x, tile, tabs, tabs + kPhaseTab, gtabs);
 }
 
@@ -918,7 +963,7 @@ __global__ __launch_bounds__(kThreads) void pass_c64_regs_kernel(const qgtl_pass
 // shared), so the injected states are never written to or read from HBM.
 constexpr uint32_t kXcd = 8;   // XCDs; workgroups are dispatched to them round-robin
 
-template <bool SPLIT>
+template <bool SPLIT, uint32_t SHAPE = 0>
 __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pass_desc_t d,
                                                                    const float2 *__restrict__ psi,
                                                                    const qgtl_inject_slot_t *__restrict__ inj,
@@ -943,7 +988,8 @@ __global__ __launch_bounds__(kThreads) void pass_c64_inject_kernel(const qgtl_pa
         t = b / nstates;
     }
     const qgtl_inject_slot_t q = inj[s];
-    pass_regs_body<SPLIT, true>(d, psi, dst_all + (size_t)s * stride, t, tile, tabs, tabs + kPhaseTab, gtabs, &q);
+    pass_regs_body<SPLIT, true, SHAPE>(d, psi, dst_all + (size_t)s * stride, t, tile, tabs, tabs + kPhaseTab, gtabs,
+                                       &q);
 }
 
 // Derivative states in the frame after a block (see qgtl_inject_desc_t), with
@@ -1376,6 +1422,16 @@ int qgtl_hip_fill_zero_state_c64(void *psi, int n) {
     return 0;
 }
 
+// The compile-time phase shape matching a rotation-form register pass (0: the
+// general form).
+static uint32_t pass_shape(const qgtl_pass_desc_t *desc) {
+    if (desc->form != 1 || desc->nphase < 1 || desc->nphase > 3) return 0;
+    uint32_t m[3] = {0, 0, 0};
+    for (int p = 0; p < desc->nphase; p++) m[p] = (uint32_t)desc->ph_mask[p] & 31u;
+    const uint32_t code = shape_code(desc->nphase, m[0], m[1], m[2]);
+    return (code == kShape3Full || code == kShape2Hi7) ? code : 0;
+}
+
 static bool regs_pass(const qgtl_pass_desc_t *desc, int tile_variant) {
     return desc->m == kRT && desc->nphase >= 1 && desc->nphase <= QGTL_MAX_PHASES && desc->store != QGTL_STORE_PERM &&
            tile_variant != 1;
@@ -1416,12 +1472,17 @@ int qgtl_hip_pass_c64(const qgtl_pass_desc_t *desc, const void *src, void *dst, 
         return qgtl::set_error_msg(-118, "qgtl_hip_pass_c64: split-plane output needs a register pass, not in place");
     const float2 *gt = (const float2 *)tabs;
     // non-temporal loads and stores: every tile is read once and written once per pass
-    if (regs && split_out)
-        hipLaunchKernelGGL((pass_c64_regs_kernel<true, true>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)src, (float2 *)dst, stride, gt);
-    else if (regs)
-        hipLaunchKernelGGL((pass_c64_regs_kernel<false, true>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)src, (float2 *)dst, stride, gt);
+    const uint32_t shape = regs ? pass_shape(desc) : 0;
+#define QGTL_REGS_LAUNCH(SPLIT, SHAPE)                                                                         \
+    hipLaunchKernelGGL((pass_c64_regs_kernel<SPLIT, true, SHAPE>), grid, dim3(kThreads), 0, qgtl::stream(), *desc, \
+                       (const float2 *)src, (float2 *)dst, stride, gt)
+    if (regs && split_out && shape == kShape3Full) QGTL_REGS_LAUNCH(true, kShape3Full);
+    else if (regs && split_out && shape == kShape2Hi7) QGTL_REGS_LAUNCH(true, kShape2Hi7);
+    else if (regs && split_out) QGTL_REGS_LAUNCH(true, 0);
+    else if (regs && shape == kShape3Full) QGTL_REGS_LAUNCH(false, kShape3Full);
+    else if (regs && shape == kShape2Hi7) QGTL_REGS_LAUNCH(false, kShape2Hi7);
+    else if (regs) QGTL_REGS_LAUNCH(false, 0);
+#undef QGTL_REGS_LAUNCH
     else
         hipLaunchKernelGGL(pass_c64_kernel, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
                            (const float2 *)src, (float2 *)dst, stride, gt);
@@ -1472,12 +1533,17 @@ int qgtl_hip_pass_c64_inject(const qgtl_pass_desc_t *desc, const void *psi, cons
     const dim3 grid((unsigned)nstates * ntiles);   // <= 65535 * 2^15 workgroups
     const float2 *gt = (const float2 *)tabs;
     const qgtl_inject_slot_t *sl = (const qgtl_inject_slot_t *)slots;
-    if (split_out)
-        hipLaunchKernelGGL(pass_c64_inject_kernel<true>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)psi, sl, (float2 *)dst, stride, gt, (uint32_t)nstates, ntiles);
-    else
-        hipLaunchKernelGGL(pass_c64_inject_kernel<false>, grid, dim3(kThreads), 0, qgtl::stream(), *desc,
-                           (const float2 *)psi, sl, (float2 *)dst, stride, gt, (uint32_t)nstates, ntiles);
+    const uint32_t shape = pass_shape(desc);
+#define QGTL_INJECT_LAUNCH(SPLIT, SHAPE)                                                                          \
+    hipLaunchKernelGGL((pass_c64_inject_kernel<SPLIT, SHAPE>), grid, dim3(kThreads), 0, qgtl::stream(), *desc,    \
+                       (const float2 *)psi, sl, (float2 *)dst, stride, gt, (uint32_t)nstates, ntiles)
+    if (split_out && shape == kShape3Full) QGTL_INJECT_LAUNCH(true, kShape3Full);
+    else if (split_out && shape == kShape2Hi7) QGTL_INJECT_LAUNCH(true, kShape2Hi7);
+    else if (split_out) QGTL_INJECT_LAUNCH(true, 0);
+    else if (shape == kShape3Full) QGTL_INJECT_LAUNCH(false, kShape3Full);
+    else if (shape == kShape2Hi7) QGTL_INJECT_LAUNCH(false, kShape2Hi7);
+    else QGTL_INJECT_LAUNCH(false, 0);
+#undef QGTL_INJECT_LAUNCH
     QGTL_LAUNCH_CHECK("pass_c64_inject");
     return 0;
 }
