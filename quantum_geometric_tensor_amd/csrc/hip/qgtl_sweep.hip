@@ -677,6 +677,21 @@ struct RegLoop {
         v[R - 2] = cmadd(make_float2(0.f, 0.f), s0 ? m11 : m00, make_float2(qa.x, qa.y));
         v[R - 1] = cmadd(make_float2(0.f, 0.f), s1 ? m11 : m00, make_float2(qa.z, qa.w));
     }
+    // the same for an anti-diagonal matrix (M[0][0] = M[1][1] = 0): element y is
+    // M[s][1-s] psi[y ^ v], the partner load only
+    __device__ __forceinline__ static void load_global_inject_anti(const float2 *psi, const GlobalMap &g,
+                                                                   const qgtl_inject_slot_t &q, float2 (&v)[kRegs]) {
+        RegLoop<R - 2>::load_global_inject_anti(psi, g, q, v);
+        const size_t a = global_index<R - 2>(g);
+        const float4 qb = *reinterpret_cast<const float4 *>(psi + ((a ^ (size_t)q.v) & ~(size_t)1));
+        const bool odd = (q.v & 1u) != 0;
+        const float2 b0 = odd ? make_float2(qb.z, qb.w) : make_float2(qb.x, qb.y);
+        const float2 b1 = odd ? make_float2(qb.x, qb.y) : make_float2(qb.z, qb.w);
+        const bool s0 = (__popc((uint32_t)a & q.w) & 1) != 0, s1 = s0 != ((q.w & 1u) != 0);
+        const float2 m01 = make_float2(q.M[1].re, q.M[1].im), m10 = make_float2(q.M[2].re, q.M[2].im);
+        v[R - 2] = cmadd(make_float2(0.f, 0.f), s0 ? m10 : m01, b0);
+        v[R - 1] = cmadd(make_float2(0.f, 0.f), s1 ? m10 : m01, b1);
+    }
     // the same stores writing the split-plane form
     template <bool NT = false>
     __device__ __forceinline__ static void store_global_split(uint32_t *planes, const GlobalMap &g,
@@ -699,6 +714,8 @@ struct RegLoop<0> {
     __device__ __forceinline__ static void load_global_inject(const float2 *, const GlobalMap &,
                                                               const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
     __device__ __forceinline__ static void load_global_inject_diag(const float2 *, const GlobalMap &,
+                                                                   const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
+    __device__ __forceinline__ static void load_global_inject_anti(const float2 *, const GlobalMap &,
                                                                    const qgtl_inject_slot_t &, float2 (&)[kRegs]) {}
     template <bool NT = false>
     __device__ __forceinline__ static void store_global(float2 *, const GlobalMap &, const float2 (&)[kRegs]) {}
@@ -726,6 +743,12 @@ __device__ __forceinline__ void phase_layer(const float2 *tab, const RegLayout &
     pt.x *= sc;
     pt.y *= sc;
     RegLoop<kRegs>::template phase_split<POST>(pt, d, v);
+}
+
+// A slot whose matrix has no diagonal (the derivative of a rotation about an
+// axis in the plane that the later gates on its qubit rotate within): uniform.
+__device__ __forceinline__ bool inject_anti(const qgtl_inject_slot_t &q) {
+    return q.M[0].re == 0.f && q.M[0].im == 0.f && q.M[3].re == 0.f && q.M[3].im == 0.f;
 }
 
 // The partner pass's layer as one factor for tile r: the product over the
@@ -940,6 +963,7 @@ __device__ __forceinline__ void pass_regs_body(const qgtl_pass_desc_t &d, const 
     const TileMaps m = tile_maps(d, r, L0);
     float2 v[kRegs];
     if (inj && inj->v == 0) RegLoop<kRegs>::load_global_inject_diag(src, m.gin, *inj, v);
+    else if (inj && inject_anti(*inj)) RegLoop<kRegs>::load_global_inject_anti(src, m.gin, *inj, v);
     else if (inj) RegLoop<kRegs>::load_global_inject(src, m.gin, *inj, v);
     else RegLoop<kRegs>::template load_global<NT>(src, m.gin, v);
     const float2 tfac = tile_factor(d, r);

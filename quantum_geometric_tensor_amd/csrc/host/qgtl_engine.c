@@ -533,6 +533,28 @@ static const void *pass_tabs(const struct qgtl_plan *p, const qgtl_block_t *b, i
     return (const char *)p->tabs_dev + (size_t)k * 2 * QGTL_PHASE_TAB * 8;
 }
 
+/* The float matrix of an injection slot.  A diagonal M needs no partner: v = 0
+ * makes the partner the element itself (the same line, a zero coefficient).
+ * An anti-diagonal M (no diagonal) lets the pass that forms the state load the
+ * partner only.  Entries below 1e-15 of the matrix's largest are the rounding
+ * residue of the products that formed it (B = u^H A u of a rotation whose
+ * later gates turn within its plane) and are dropped, a change far below
+ * float precision. */
+static void slot_matrix(qgtl_inject_slot_t *q, const qgtl_mat2 *M)
+{
+    double big = 0.0;
+    for (int e = 0; e < 4; e++) {
+        const double a = fabs(M->m[e].re) + fabs(M->m[e].im);
+        if (a > big) big = a;
+    }
+    for (int e = 0; e < 4; e++) {
+        const int tiny = fabs(M->m[e].re) + fabs(M->m[e].im) <= 1e-15 * big;
+        q->M[e].re = tiny ? 0.f : (float)M->m[e].re;
+        q->M[e].im = tiny ? 0.f : (float)M->m[e].im;
+    }
+    if (q->M[1].re == 0.f && q->M[1].im == 0.f && q->M[2].re == 0.f && q->M[2].im == 0.f) q->v = 0;
+}
+
 /* Build and upload the phase tables of every pass after the angles changed. */
 static int upload_tabs(struct qgtl_plan *p)
 {
@@ -575,16 +597,8 @@ static int upload_tabs(struct qgtl_plan *p)
             f->v = pp->v;
             f->w = pp->w;
             bk->v = bk->w = 1u << pp->qubit;
-            for (int e = 0; e < 4; e++) {
-                f->M[e].re = (float)pp->A.m[e].re;
-                f->M[e].im = (float)pp->A.m[e].im;
-                bk->M[e].re = (float)pp->B.m[e].re;
-                bk->M[e].im = (float)pp->B.m[e].im;
-            }
-            /* a diagonal M needs no partner: v = 0 makes the partner the element itself
-             * (the same line, and a zero coefficient) */
-            if (pp->A.m[1].re == 0.0 && pp->A.m[1].im == 0.0 && pp->A.m[2].re == 0.0 && pp->A.m[2].im == 0.0) f->v = 0;
-            if (pp->B.m[1].re == 0.0 && pp->B.m[1].im == 0.0 && pp->B.m[2].re == 0.0 && pp->B.m[2].im == 0.0) bk->v = 0;
+            slot_matrix(f, &pp->A);
+            slot_matrix(bk, &pp->B);
         }
         HIPCALL(qgtl_hip_h2d(p->inj_slots_dev, p->inj_slots_host, 2 * P * sizeof(qgtl_inject_slot_t)));
     }
