@@ -330,6 +330,30 @@ def test_ng_step_device_resident(method):
     assert err <= 1e-6 * cond
 
 
+@pytest.mark.parametrize("n,L", [(16, 4), (20, 6)])
+def test_two_stream_chunks_match_one_stream(n, L):
+    """The chain's stream carries the derivative chunks injected beyond the meeting block
+    backward while the library stream carries the forward ones and the meeting point's
+    injections, joined by events before the Gram: eager launches, the captured graph's replays
+    and per-launch timing (every kernel on one stream, one after another) give bit-identical Q."""
+    th, ops, oops = hea(n, L, seed=4500 + n)
+    one = Q.Plan(n, ops, Q.MODE_MIXED, graph=False)
+    Q.set_profiling(True)
+    try:
+        q_one, _ = one.execute()
+    finally:
+        Q.set_profiling(False)
+    one.close()
+    two = Q.Plan(n, ops, Q.MODE_MIXED)
+    for want in (0, 1, 1):                       # eager, then captured and replayed
+        q, st = two.execute()
+        assert st["graph"] == want
+        assert np.array_equal(q, q_one)
+    two.close()
+    if n <= 16:
+        assert Q.rel_err(q_one, O.qgt(n, oops)) <= MIXED_TOL
+
+
 def test_plan_graph_replays_match_eager():
     """From the second execute of an unchanged plan on, the launch sequence (two streams,
     events) runs as one captured HIP graph: every replay, with eager launches of another plan
