@@ -646,8 +646,9 @@ __device__ __forceinline__ void gram_cplx_reduce_group(const float *__restrict__
             float vr[8], vi[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                vr[u] = q[(size_t)(k + u * kGroups) * cs];
-                vi[u] = q[(size_t)(k + u * kGroups) * cs + kTileElems];
+                // streamed: every partial is read once
+                vr[u] = __builtin_nontemporal_load(q + (size_t)(k + u * kGroups) * cs);
+                vi[u] = __builtin_nontemporal_load(q + (size_t)(k + u * kGroups) * cs + kTileElems);
             }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
