@@ -745,8 +745,9 @@ __device__ __forceinline__ void phase_layer(const float2 *tab, const RegLayout &
     RegLoop<kRegs>::template phase_split<POST>(pt, d, v);
 }
 
-// A slot whose matrix has no diagonal (the derivative of a rotation about an
-// axis in the plane that the later gates on its qubit rotate within): uniform.
+// A slot whose matrix has a zero diagonal: an X/Y-plane generator, e.g. the
+// derivative of an RY whose later gates on its qubit are Z rotations.  The
+// test reads the slot's fields, the same for every thread of the workgroup.
 __device__ __forceinline__ bool inject_anti(const qgtl_inject_slot_t &q) {
     return q.M[0].re == 0.f && q.M[0].im == 0.f && q.M[3].re == 0.f && q.M[3].im == 0.f;
 }
